@@ -1,0 +1,983 @@
+// libcnhip: batched bordered-banded factor and solve for MI355X (gfx950), FP64 real and complex.
+//
+// Algorithm (the reference's, core/include/conatus/banded.hpp:1-8,253-341): after moving the r
+// boundary rows to the top, P A = B + U V^T with B banded (kl, ku) whose border rows are s_i e_i,
+// U = [e_0 .. e_{r-1}], V^T = border rows - diag(s).  B = P_B L U (partial pivoting inside the
+// band, LAPACK gbtf2 order), W = B^-1 U, S = I + V^T W, and A x = b is solved as
+// y = B^-1 P b, z = S^-1 V^T y, x = y - W z.
+//
+// MI355X design (not a translation of the reference's warp kernels):
+//  * Storage is interleaved by 16 systems: entry e of system m lives at
+//    ((m/16)*len + e)*16 + m%16, so the solve's lanes (consecutive systems) read 256 contiguous
+//    bytes per factor entry and a 16-system chunk is produced by one workgroup.
+//  * Factor: one group of G (16/32/64) lanes per system, lane c owns band column q = c (mod M),
+//    M = kl+ku+1, and holds the kl+1 active rows of that column in registers (a window that
+//    shifts by one row per step, so that every register index is static).  The
+//    owner of the pivot column finds the pivot, forms the reciprocal and the multipliers, and
+//    broadcasts them over the wavefront (ds_bpermute); every other lane swaps and updates its own
+//    column.  r extra lanes carry the columns of U (unit vectors) through the same elimination,
+//    so the forward half of W = B^-1 U costs nothing extra.  Row j of U leaves the window as one
+//    coalesced row store; the incoming band row is one coalesced row load.  U's diagonal is
+//    stored as its reciprocal so that solves multiply instead of divide.
+//  * Border: one lane per (system, border column) finishes W by back substitution and
+//    accumulates the column of S = I + V^T W; one lane per system LU-factors S (partial pivoting)
+//    and forms cond_1(S) from the explicit inverse, as banded.hpp:403-432.
+//  * Solve: one lane per (system, right-hand side); forward sweep (pivots + L) and backward sweep
+//    (U, window of the last wu solutions in registers), V^T y accumulated during the backward
+//    sweep, z = S^-1 V^T y, then x -= W z.  Gather/scatter maps and the split complex mode of
+//    the reference's device solver are folded into the loads and stores.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cnhip.h"
+#include "cn_common.h"
+
+using cn::zc;
+
+namespace {
+
+constexpr int IL = 16;     // systems per interleave chunk
+constexpr int MAXR = 8;    // largest border rank handled by the kernels
+
+thread_local std::string g_err;
+
+struct CnError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw CnError{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(CN_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t checked_mul(std::initializer_list<size_t> dims) {
+    size_t p = 1;
+    for (size_t d : dims) {
+        if (d != 0 && p > std::numeric_limits<size_t>::max() / d) fail(CN_EOVERFLOW, "array dimensions overflow size_t");
+        p *= d;
+    }
+    return p;
+}
+
+// ------------------------------------------------------------------------------------------
+// Factor kernel: G lanes per system, 256/G systems per workgroup of 256 lanes.
+// Ab: unfactored band rows, row i holds A(i, i-KL+d), d = 0..M-1 (row stride M).
+// Uf: U rows, row j holds U(j, j+d), d = 0..US-1, d = 0 holding 1/U(j, j).
+// Lf: multipliers of step j, Lf(j, i) = l(j+1+i, j), i < KL.  piv: pivot offset 0..KL of step j.
+// Wf: forward half of W (L^-1 P_B e_k), rows k < r of length n.
+// ------------------------------------------------------------------------------------------
+template <class T, int KL>
+__global__ void __launch_bounds__(256)
+k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int* __restrict__ piv,
+          T* __restrict__ Wf, int* __restrict__ info, int B, int n, int M, int r, int G, int US,
+          int dopivot) {
+    const int t = threadIdx.x;
+    const int g = t / G;
+    const int c = t - g * G;
+    const int m = blockIdx.x * (256 / G) + g;
+    const bool live = m < B;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* A0 = Ab + ch * ((size_t)n * M * IL) + ml;
+    T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
+    int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    T* W0 = Wf + ch * ((size_t)r * n * IL) + ml;
+    const bool band = live && c < M;
+    const bool wl = live && c >= M && c < M + r;
+    const int k = c - M;
+    const int gbase = (t & 63) - c;   // wavefront lane of the group's lane 0 (G divides 64)
+
+    T win[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) {
+        T v = cn::zero<T>();
+        if (band && i < n) {
+            const int d = c - i + KL;
+            if (d >= 0 && d < M) v = A0[((size_t)i * M + d) * IL];
+        }
+        if (wl) v = (i == k) ? cn::one<T>() : cn::zero<T>();
+        win[i] = v;
+    }
+    int first_singular = 0;
+    for (int j = 0; j < n; ++j) {
+        // win[i] holds row j+i of this lane's column
+        const int owner = j % M;
+        const int src = gbase + owner;
+        const bool is_owner = (c == owner);
+        // pivot search on the owner's column (first maximum of |re|+|im|, as i?amax)
+        double best = cn::cabs1(win[0]);
+        int ip = 0;
+        if (dopivot) {
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) {
+                const double v = cn::cabs1(win[i]);
+                if (v > best) { best = v; ip = i; }
+            }
+        }
+        ip = __shfl(ip, src, 64);
+        best = __shfl(best, src, 64);
+        if (ip != 0) {   // row interchange in every column of the window and in W
+#pragma unroll
+            for (int i = 1; i <= KL; ++i)
+                if (ip == i) {
+                    const T tmp = win[0];
+                    win[0] = win[i];
+                    win[i] = tmp;
+                }
+        }
+        const bool singular = !(best != 0.0);
+        T rc = cn::recip(win[0]);
+        if (singular) rc = cn::zero<T>();
+        T l[KL];
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) l[i - 1] = cn::shfl(cn::mul(win[i], rc), src);
+        const int d = (c - owner + M) % M;   // column offset of this lane's column from j
+        if (band) {
+            T ov = win[0];
+            if (is_owner) ov = rc;
+            U0[((size_t)j * US + d) * IL] = ov;
+        }
+        if (wl) W0[((size_t)k * n + j) * IL] = win[0];
+        if (band && is_owner) {
+#pragma unroll
+            for (int i = 0; i < KL; ++i) L0[((size_t)j * KL + i) * IL] = l[i];
+            P0[(size_t)j * IL] = ip;
+        }
+        if ((band && !is_owner) || wl) {
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) win[i] = cn::fnms(l[i - 1], win[0], win[i]);
+        }
+        // row j leaves the window; row j+KL+1 enters at the bottom
+        const int jn = j + KL + 1;
+        T v = cn::zero<T>();
+        if (band) {
+            if (is_owner) {   // the owner takes column j+M: rows j+1..j+KL are zero there
+#pragma unroll
+                for (int i = 1; i <= KL; ++i) win[i] = cn::zero<T>();
+            }
+            const int dA = is_owner ? (M - 1) : (d - 1);
+            if (jn < n) v = A0[((size_t)jn * M + dA) * IL];
+        }
+        if (wl && jn == k) v = cn::one<T>();
+#pragma unroll
+        for (int i = 0; i < KL; ++i) win[i] = win[i + 1];
+        win[KL] = v;
+        if (singular && first_singular == 0) first_singular = j + 1;
+    }
+    if (live && c == 0) info[m] = first_singular;
+}
+
+// ------------------------------------------------------------------------------------------
+// Border kernel: lane = (system, border column k); back substitution W_k = U^-1 Wf_k in place and
+// column k of S = I + V^T W.
+// ------------------------------------------------------------------------------------------
+template <class T, int WU>
+__global__ void __launch_bounds__(256)
+k_border(const T* __restrict__ Uf, T* __restrict__ Wf, const T* __restrict__ Vt, T* __restrict__ S,
+         int B, int n, int r, int US) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / r;
+    const int k = gid - m * r;
+    if (m >= B) return;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    T* W0 = Wf + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    T s[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) s[a] = cn::zero<T>();
+    int j = n - 1;
+    while (j >= 0) {
+#pragma unroll
+        for (int u = 0; u <= WU; ++u) {
+            if (j < 0) break;
+            const T* Ur = U0 + (size_t)j * US * IL;
+            T acc = W0[(size_t)j * IL];
+#pragma unroll
+            for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+            const T x = cn::mul(acc, Ur[0]);
+            W0[(size_t)j * IL] = x;
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) s[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, s[a]);
+#pragma unroll
+            for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+            xw[0] = x;
+            --j;
+        }
+    }
+    T* Sm = S + (size_t)m * r * r;
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a)
+        if (a < r) {
+            T v = s[a];
+            if (a == k) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);   // + 1 on the diagonal
+            Sm[(size_t)k * r + a] = v;                                // column-major S(a, k)
+        }
+}
+
+__device__ __forceinline__ double absv(double a) { return fabs(a); }
+__device__ __forceinline__ double absv(zc a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ zc neg(zc a) { return zc{-a.re, -a.im}; }
+__device__ __forceinline__ double neg(double a) { return -a; }
+
+// One lane per system: LU of S in place (column-major, partial pivoting by |re|+|im|), and
+// cond_1(S) = |S|_1 |S^-1|_1 with the inverse from the factors (banded.hpp:403-432).
+template <class T>
+__global__ void __launch_bounds__(256)
+k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, int* __restrict__ info,
+         int B, int r) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= B) return;
+    T a[MAXR * MAXR];
+    T* Sm = S + (size_t)m * r * r;
+    for (int q = 0; q < r * r; ++q) a[q] = Sm[q];
+    double nS = 0.0;
+    for (int b = 0; b < r; ++b) {
+        double s = 0.0;
+        for (int i = 0; i < r; ++i) s += absv(a[b * r + i]);
+        if (!(s <= nS)) nS = s;
+    }
+    int bad = 0;
+    int p[MAXR];
+    for (int jc = 0; jc < r; ++jc) {
+        int ip = jc;
+        double best = cn::cabs1(a[jc * r + jc]);
+        for (int i = jc + 1; i < r; ++i) {
+            const double v = cn::cabs1(a[jc * r + i]);
+            if (v > best) { best = v; ip = i; }
+        }
+        p[jc] = ip;
+        if (best == 0.0) { bad = 1; continue; }
+        if (ip != jc)
+            for (int q = 0; q < r; ++q) { T tmp = a[q * r + jc]; a[q * r + jc] = a[q * r + ip]; a[q * r + ip] = tmp; }
+        const T rc = cn::recip(a[jc * r + jc]);
+        for (int i = jc + 1; i < r; ++i) a[jc * r + i] = cn::mul(a[jc * r + i], rc);
+        for (int q = jc + 1; q < r; ++q)
+            for (int i = jc + 1; i < r; ++i) a[q * r + i] = cn::fnms(a[jc * r + i], a[q * r + jc], a[q * r + i]);
+    }
+    for (int q = 0; q < r * r; ++q) Sm[q] = a[q];
+    for (int q = 0; q < r; ++q) spiv[(size_t)m * r + q] = p[q];
+    double c = 1.0;
+    if (bad) {
+        if (info[m] == 0) info[m] = -1;
+        c = INFINITY;
+    } else {
+        // inverse column by column: solve S x = e_b with the factors
+        double ninv = 0.0;
+        for (int b = 0; b < r; ++b) {
+            T x[MAXR];
+            for (int i = 0; i < r; ++i) x[i] = (i == b) ? cn::one<T>() : cn::zero<T>();
+            for (int i = 0; i < r; ++i)
+                if (p[i] != i) { T tmp = x[i]; x[i] = x[p[i]]; x[p[i]] = tmp; }
+            for (int i = 0; i < r; ++i)
+                for (int q = 0; q < i; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
+            for (int i = r - 1; i >= 0; --i) {
+                for (int q = i + 1; q < r; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
+                x[i] = cn::mul(x[i], cn::recip(a[i * r + i]));
+            }
+            double s = 0.0;
+            for (int i = 0; i < r; ++i) s += absv(x[i]);
+            if (!(s <= ninv)) ninv = s;
+        }
+        c = nS * ninv;
+    }
+    cond[m] = c;
+}
+
+// ------------------------------------------------------------------------------------------
+// Solve kernel.  IO modes: 0 plain in place (x holds b), 1 mapped (rows/cols), 2 split complex.
+// ------------------------------------------------------------------------------------------
+struct SolveIO {
+    int mode;
+    void* x;               // mode 0: in place blocks; mode 1/2: output
+    const void* rhs;       // mode 1: rhs; mode 2: re
+    const int64_t* rows;   // mode 1: rows (or null); mode 2: rows_c
+    const int64_t* rows2;  // mode 2: rows_s
+    const int64_t* cols;   // mode 1: cols (or null); mode 2: cols_c
+    const int64_t* cols2;  // mode 2: cols_s
+    int64_t rhs_stride;    // mode 0
+};
+
+template <class T>
+__device__ __forceinline__ T io_load_b(const SolveIO& io, size_t mn, int k, int i);
+template <class T>
+__device__ __forceinline__ void io_store(const SolveIO& io, size_t mn, int k, int i, T v);
+template <class T>
+__device__ __forceinline__ T io_load_x(const SolveIO& io, size_t mn, int k, int i);
+
+template <>
+__device__ __forceinline__ double io_load_b<double>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const double*>(io.x)[k * io.rhs_stride + mn + i];
+    const size_t q = io.rows ? (size_t)io.rows[mn + i] : mn + i;
+    return static_cast<const double*>(io.rhs)[q];
+}
+template <>
+__device__ __forceinline__ zc io_load_b<zc>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const zc*>(io.x)[k * io.rhs_stride + mn + i];
+    if (io.mode == 1) {
+        const size_t q = io.rows ? (size_t)io.rows[mn + i] : mn + i;
+        return static_cast<const zc*>(io.rhs)[q];
+    }
+    const double* re = static_cast<const double*>(io.rhs);
+    return zc{re[io.rows[mn + i]], re[io.rows2[mn + i]]};
+}
+template <>
+__device__ __forceinline__ double io_load_x<double>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const double*>(io.x)[k * io.rhs_stride + mn + i];
+    const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+    return static_cast<const double*>(io.x)[q];
+}
+template <>
+__device__ __forceinline__ zc io_load_x<zc>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const zc*>(io.x)[k * io.rhs_stride + mn + i];
+    if (io.mode == 1) {
+        const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+        return static_cast<const zc*>(io.x)[q];
+    }
+    const double* x = static_cast<const double*>(io.x);
+    return zc{x[io.cols[mn + i]], x[io.cols2[mn + i]]};
+}
+template <>
+__device__ __forceinline__ void io_store<double>(const SolveIO& io, size_t mn, int k, int i, double v) {
+    if (io.mode == 0) { static_cast<double*>(io.x)[k * io.rhs_stride + mn + i] = v; return; }
+    const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+    static_cast<double*>(io.x)[q] = v;
+}
+template <>
+__device__ __forceinline__ void io_store<zc>(const SolveIO& io, size_t mn, int k, int i, zc v) {
+    if (io.mode == 0) { static_cast<zc*>(io.x)[k * io.rhs_stride + mn + i] = v; return; }
+    if (io.mode == 1) {
+        const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+        static_cast<zc*>(io.x)[q] = v;
+        return;
+    }
+    double* x = static_cast<double*>(io.x);
+    x[io.cols[mn + i]] = v.re;
+    x[io.cols2[mn + i]] = v.im;
+}
+
+template <class T, int KL, int WU>
+__global__ void __launch_bounds__(256)
+k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+        const T* __restrict__ Vt, const T* __restrict__ Wm, const T* __restrict__ S,
+        const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+        SolveIO io, int B, int n, int r, int US, int nrhs) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / nrhs;
+    const int k = gid - m * nrhs;
+    if (m >= B) return;
+    const size_t NL = (size_t)B * nrhs;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
+    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    const size_t mn = (size_t)m * n;
+    T* w0 = ws + gid;
+
+    // forward: y = L^-1 P_B (P b), window of KL+1 rows
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = (i < n) ? io_load_b<T>(io, mn, k, perm[i]) : cn::zero<T>();
+    int j = 0;
+    while (j < n) {
+#pragma unroll
+        for (int u = 0; u <= KL; ++u) {
+            if (j >= n) break;
+            const int p = P0[(size_t)j * IL];
+            if (p != 0) {
+#pragma unroll
+                for (int i = 1; i <= KL; ++i)
+                    if (p == i) { const T tmp = y[0]; y[0] = y[i]; y[i] = tmp; }
+            }
+            const T* Lr = L0 + (size_t)j * KL * IL;
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(Lr[(i - 1) * IL], y[0], y[i]);
+            w0[(size_t)j * NL] = y[0];
+            const int jn = j + KL + 1;
+#pragma unroll
+            for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+            y[KL] = (jn < n) ? io_load_b<T>(io, mn, k, perm[jn]) : cn::zero<T>();
+            ++j;
+        }
+    }
+    // backward: x = U^-1 y, window of the last WU solutions; t = V^T x on the way
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    j = n - 1;
+    while (j >= 0) {
+#pragma unroll
+        for (int u = 0; u <= WU; ++u) {
+            if (j < 0) break;
+            const T* Ur = U0 + (size_t)j * US * IL;
+            T acc = w0[(size_t)j * NL];
+#pragma unroll
+            for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+            const T x = cn::mul(acc, Ur[0]);
+            io_store<T>(io, mn, k, j, x);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, t[a]);
+#pragma unroll
+            for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+            xw[0] = x;
+            --j;
+        }
+    }
+    if (r == 0) return;
+    // z = S^-1 t (S LU-factored, column-major, pivots spiv)
+    const T* Sm = S + (size_t)m * r * r;
+    const int* sp = spiv + (size_t)m * r;
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i >= r) break;
+        const int pi = sp[i];
+#pragma unroll
+        for (int q = i + 1; q < MAXR; ++q)
+            if (q == pi) { const T tmp = t[i]; t[i] = t[q]; t[q] = tmp; }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i >= r) break;
+#pragma unroll
+        for (int q = 0; q < i; ++q) t[i] = cn::fnms(Sm[q * r + i], t[q], t[i]);
+    }
+#pragma unroll
+    for (int i = MAXR - 1; i >= 0; --i) {
+        if (i < r) {
+#pragma unroll
+            for (int q = i + 1; q < MAXR; ++q)
+                if (q < r) t[i] = cn::fnms(Sm[q * r + i], t[q], t[i]);
+            t[i] = cn::mul(t[i], cn::recip(Sm[i * r + i]));
+        }
+    }
+    // x -= W z
+    const T* W0 = Wm + ch * ((size_t)r * n * IL) + ml;
+    for (int i = 0; i < n; ++i) {
+        T v = io_load_x<T>(io, mn, k, i);
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * IL], t[a], v);
+        io_store<T>(io, mn, k, i, v);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+constexpr int KL_SET[] = {1, 2, 4, 6, 8, 12, 16};
+constexpr int WU_SET[] = {4, 8, 16, 24, 32};
+
+int pick_kl(int kl) {
+    for (int v : KL_SET) if (v >= kl) return v;
+    return -1;
+}
+int pick_wu(int w) {
+    for (int v : WU_SET) if (v >= w) return v;
+    return -1;
+}
+
+using FactorFn = void (*)(dim3, dim3, hipStream_t, const void*, void*, void*, int*, void*, int*, int, int,
+                          int, int, int, int, int);
+using BorderFn = void (*)(dim3, dim3, hipStream_t, const void*, void*, const void*, void*, int, int, int, int);
+using SolveFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, const int*, const void*, const void*,
+                         const void*, const int*, const int*, void*, SolveIO, int, int, int, int, int);
+
+template <class T, int KL>
+void launch_factor(dim3 g, dim3 b, hipStream_t s, const void* Ab, void* Uf, void* Lf, int* piv, void* Wf,
+                   int* info, int B, int n, int M, int r, int G, int US, int pivot) {
+    hipLaunchKernelGGL((k_band_lu<T, KL>), g, b, 0, s, static_cast<const T*>(Ab), static_cast<T*>(Uf),
+                       static_cast<T*>(Lf), piv, static_cast<T*>(Wf), info, B, n, M, r, G, US, pivot);
+}
+template <class T, int WU>
+void launch_border(dim3 g, dim3 b, hipStream_t s, const void* Uf, void* Wf, const void* Vt, void* S, int B,
+                   int n, int r, int US) {
+    hipLaunchKernelGGL((k_border<T, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<T*>(Wf),
+                       static_cast<const T*>(Vt), static_cast<T*>(S), B, n, r, US);
+}
+template <class T, int KL, int WU>
+void launch_solve(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf, const int* piv, const void* Vt,
+                  const void* W, const void* S, const int* spiv, const int* perm, void* ws, SolveIO io, int B,
+                  int n, int r, int US, int nrhs) {
+    hipLaunchKernelGGL((k_solve<T, KL, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
+                       piv, static_cast<const T*>(Vt), static_cast<const T*>(W), static_cast<const T*>(S), spiv,
+                       perm, static_cast<T*>(ws), io, B, n, r, US, nrhs);
+}
+
+template <class T>
+FactorFn factor_fn(int KL) {
+    switch (KL) {
+        case 1: return &launch_factor<T, 1>;
+        case 2: return &launch_factor<T, 2>;
+        case 4: return &launch_factor<T, 4>;
+        case 6: return &launch_factor<T, 6>;
+        case 8: return &launch_factor<T, 8>;
+        case 12: return &launch_factor<T, 12>;
+        case 16: return &launch_factor<T, 16>;
+    }
+    return nullptr;
+}
+template <class T>
+BorderFn border_fn(int WU) {
+    switch (WU) {
+        case 4: return &launch_border<T, 4>;
+        case 8: return &launch_border<T, 8>;
+        case 16: return &launch_border<T, 16>;
+        case 24: return &launch_border<T, 24>;
+        case 32: return &launch_border<T, 32>;
+    }
+    return nullptr;
+}
+template <class T, int KL>
+SolveFn solve_fn_kl(int WU) {
+    if (WU < KL) return nullptr;
+    switch (WU) {
+        case 4: if constexpr (KL <= 4) return &launch_solve<T, KL, 4>; else return nullptr;
+        case 8: if constexpr (KL <= 8) return &launch_solve<T, KL, 8>; else return nullptr;
+        case 16: return &launch_solve<T, KL, 16>;
+        case 24: return &launch_solve<T, KL, 24>;
+        case 32: return &launch_solve<T, KL, 32>;
+    }
+    return nullptr;
+}
+template <class T>
+SolveFn solve_fn(int KL, int WU) {
+    switch (KL) {
+        case 1: return solve_fn_kl<T, 1>(WU);
+        case 2: return solve_fn_kl<T, 2>(WU);
+        case 4: return solve_fn_kl<T, 4>(WU);
+        case 6: return solve_fn_kl<T, 6>(WU);
+        case 8: return solve_fn_kl<T, 8>(WU);
+        case 12: return solve_fn_kl<T, 12>(WU);
+        case 16: return solve_fn_kl<T, 16>(WU);
+    }
+    return nullptr;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t b) {
+        release();
+        bytes = b;
+        if (b) hip_check(hipMalloc(&p, b), "hipMalloc");
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct cn_bb {
+    int dtype = CN_C128, B = 0, Bp = 0, n = 0, r = 0, kl = 0, ku = 0, KLP = 0, M = 0, WU = 0, US = 0, G = 0;
+    int pivot = 1;
+    int nrhs_cap = 0;
+    size_t esz = 16;
+    double cond_max = 1.0, factor_ms = 0.0;
+    int device = 0;
+    DevBuf Ab, Uf, Lf, piv, Vt, W, S, spiv, cond, info, perm, ws;
+    std::vector<double> hcond;
+};
+
+namespace {
+
+// Host layout of one system into the interleaved device arrays (banded.hpp:170-251 semantics).
+template <class T>
+void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, const T* dv,
+                const std::vector<int>& perm, const std::vector<int>& pos, std::vector<T>& Ab, std::vector<T>& Vt) {
+    const int n = h.n, r = h.r, M = h.M, KLP = h.KLP;
+    const size_t ch = m / IL, ml = m % IL;
+    auto AB = [&](int i, int d) -> T& { return Ab[(ch * ((size_t)n * M) + (size_t)i * M + d) * IL + ml]; };
+    auto VT = [&](int a, int j) -> T& { return Vt[(ch * ((size_t)r * n) + (size_t)a * n + j) * IL + ml]; };
+    for (int i = 0; i < n; ++i) {
+        const int o = perm[i];
+        if (i < r) {
+            double s = 0.0;
+            for (int p = ip[o]; p < ip[o + 1]; ++p) {
+                s = std::max(s, static_cast<double>(std::abs(dv[p])));
+                VT(i, ix[p]) += dv[p];
+            }
+            if (s == 0.0) s = 1.0;
+            AB(i, KLP) = T(s);
+            VT(i, i) -= T(s);
+        } else {
+            for (int p = ip[o]; p < ip[o + 1]; ++p) {
+                if (dv[p] == T(0)) continue;
+                const int j = ix[p];
+                AB(i, j - i + KLP) += dv[p];
+            }
+        }
+    }
+    (void)pos;
+}
+
+template <class T>
+void run_factor(cn_bb& h, hipStream_t s) {
+    const int spb = 256 / h.G;
+    const dim3 fb(256), fg((h.Bp + spb - 1) / spb);
+    FactorFn f = factor_fn<T>(h.KLP);
+    if (!f) fail(CN_ELOGIC, "no factor kernel for kl " + std::to_string(h.KLP));
+    f(fg, fb, s, h.Ab.p, h.Uf.p, h.Lf.p, static_cast<int*>(h.piv.p), h.W.p, static_cast<int*>(h.info.p), h.B, h.n,
+      h.M, h.r, h.G, h.US, h.pivot);
+    hip_check(hipGetLastError(), "k_band_lu launch");
+    if (h.r > 0) {
+        BorderFn bf = border_fn<T>(h.WU);
+        const int lanes = h.B * h.r;
+        bf(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.W.p, h.Vt.p, h.S.p, h.B, h.n, h.r, h.US);
+        hip_check(hipGetLastError(), "k_border launch");
+        hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.S.p),
+                           static_cast<int*>(h.spiv.p), static_cast<double*>(h.cond.p), static_cast<int*>(h.info.p),
+                           h.B, h.r);
+        hip_check(hipGetLastError(), "k_cap_lu launch");
+    }
+}
+
+void factor_and_check(cn_bb& h, hipStream_t s, bool check) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (check) {
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+    }
+    if (h.dtype == CN_C128) run_factor<zc>(h, s); else run_factor<double>(h, s);
+    if (!check) return;
+    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h.factor_ms = ms;
+    std::vector<int> info(h.B);
+    hip_check(hipMemcpy(info.data(), h.info.p, sizeof(int) * h.B, hipMemcpyDeviceToHost), "hipMemcpy info");
+    h.hcond.assign(h.B, 1.0);
+    if (h.r > 0)
+        hip_check(hipMemcpy(h.hcond.data(), h.cond.p, sizeof(double) * h.B, hipMemcpyDeviceToHost), "hipMemcpy cond");
+    h.cond_max = 1.0;
+    for (int m = 0; m < h.B; ++m) {
+        if (info[m] > 0)
+            fail(CN_ESINGULAR, "BandedBatch: banded part of mode " + std::to_string(m) + " is singular at pivot " +
+                                   std::to_string(info[m]) + (h.pivot ? "" : " (no pivoting was requested)") +
+                                   "; the non-border rows must form an invertible banded operator");
+        if (info[m] < 0)
+            fail(CN_ESINGULAR, "BandedBatch: capacitance matrix of mode " + std::to_string(m) +
+                                   " is singular: the boundary rows are not independent of the banded operator");
+        if (std::isnan(h.hcond[m]))
+            fail(CN_ESINGULAR, "BandedBatch: capacitance condition number of mode " + std::to_string(m) + " is NaN");
+        h.cond_max = std::max(h.cond_max, h.hcond[m]);
+    }
+}
+
+template <class T>
+void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* indices, const void* const* data,
+                 const int64_t* nnz, const int32_t* border) {
+    const int n = h.n, B = h.B, r = h.r;
+    // CSR structure and finiteness (core/bindings/module.cpp:75-95, banded.hpp:173-180)
+    for (int m = 0; m < B; ++m) {
+        const int32_t* ip = indptr[m];
+        const int32_t* ix = indices[m];
+        const T* dv = static_cast<const T*>(data[m]);
+        if (!ip || (nnz[m] > 0 && (!ix || !dv))) fail(CN_EINVAL, "CSR: null array for matrix " + std::to_string(m));
+        if (ip[0] != 0 || (int64_t)ip[n] != nnz[m])
+            fail(CN_EINVAL, "inconsistent CSR arrays (indptr[0] = 0, indptr[n] = nnz = len(indices) = len(data) "
+                            "required) for matrix " + std::to_string(m));
+        for (int i = 0; i < n; ++i)
+            if (ip[i + 1] < ip[i]) fail(CN_EINVAL, "CSR: indptr decreases at row " + std::to_string(i));
+        for (int64_t p = 0; p < nnz[m]; ++p) {
+            if (ix[p] < 0 || ix[p] >= n)
+                fail(CN_EINVAL, "CSR: column index " + std::to_string(ix[p]) + " outside [0, " + std::to_string(n) + ")");
+            const std::complex<double> v(dv[p]);
+            if (!std::isfinite(v.real()) || !std::isfinite(v.imag())) {
+                int row = 0;
+                while (ip[row + 1] <= p) ++row;
+                fail(CN_EINVAL, "BorderedBanded: non-finite matrix entry at (" + std::to_string(row) + ", " +
+                                    std::to_string(ix[p]) + ") of matrix " + std::to_string(m) +
+                                    ": the operator must be finite");
+            }
+        }
+    }
+    std::vector<char> isb(n, 0);
+    for (int a = 0; a < r; ++a) {
+        const int i = border[a];
+        if (i < 0 || i >= n) fail(CN_EINVAL, "border row out of range");
+        if (isb[i]) fail(CN_EINVAL, "duplicate border row");
+        isb[i] = 1;
+    }
+    std::vector<int> perm(border, border + r), pos(n);
+    for (int i = 0; i < n; ++i) if (!isb[i]) perm.push_back(i);
+    for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+    // common band of the bulk rows (banded.hpp:144-168), max over the batch
+    int kl = 0, ku = 0;
+    for (int m = 0; m < B; ++m) {
+        const int32_t* ip = indptr[m];
+        const int32_t* ix = indices[m];
+        const T* dv = static_cast<const T*>(data[m]);
+        for (int i = r; i < n; ++i) {
+            const int o = perm[i];
+            for (int p = ip[o]; p < ip[o + 1]; ++p) {
+                if (dv[p] == T(0)) continue;
+                kl = std::max(kl, i - ix[p]);
+                ku = std::max(ku, ix[p] - i);
+            }
+        }
+    }
+    h.kl = kl;
+    h.ku = ku;
+    h.KLP = pick_kl(std::max(kl, 1));
+    if (h.KLP < 0) fail(CN_EUNSUPPORTED, "BandedBatch: kl = " + std::to_string(kl) + " exceeds the compiled kernels (<= 16)");
+    h.M = h.KLP + ku + 1;
+    int G = 16;
+    while (G < h.M + r) G *= 2;
+    if (G > 64)
+        fail(CN_EUNSUPPORTED, "BandedBatch: band kl+ku+1 = " + std::to_string(h.M) + " plus border rank " +
+                                  std::to_string(r) + " exceeds one wavefront (64 lanes)");
+    h.G = G;
+    h.WU = pick_wu(h.KLP + ku);
+    if (h.WU < 0) fail(CN_EUNSUPPORTED, "BandedBatch: kl+ku = " + std::to_string(h.KLP + ku) + " exceeds 32");
+    h.US = h.WU + 1;
+    h.Bp = (B + IL - 1) / IL * IL;
+    const size_t es = sizeof(T);
+    const size_t nAb = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.M});
+    const size_t nU = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.US});
+    const size_t nL = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.KLP});
+    const size_t nV = checked_mul({(size_t)h.Bp, (size_t)r, (size_t)n});
+    std::vector<T> Ab(nAb, T(0)), Vt(nV, T(0));
+    for (int m = 0; m < B; ++m)
+        layout_one<T>(h, m, indptr[m], indices[m], static_cast<const T*>(data[m]), perm, pos, Ab, Vt);
+    hip_check(hipGetDevice(&h.device), "hipGetDevice");
+    h.Ab.alloc(nAb * es);
+    h.Uf.alloc(nU * es);
+    h.Lf.alloc(nL * es);
+    h.piv.alloc((size_t)h.Bp * n * sizeof(int));
+    h.Vt.alloc(std::max<size_t>(nV, 1) * es);
+    h.W.alloc(std::max<size_t>(nV, 1) * es);
+    h.S.alloc(std::max<size_t>((size_t)h.Bp * r * r, 1) * es);
+    h.spiv.alloc(std::max<size_t>((size_t)h.Bp * r, 1) * sizeof(int));
+    h.cond.alloc((size_t)h.Bp * sizeof(double));
+    h.info.alloc((size_t)h.Bp * sizeof(int));
+    h.perm.alloc((size_t)n * sizeof(int));
+    hip_check(hipMemcpy(h.Ab.p, Ab.data(), nAb * es, hipMemcpyHostToDevice), "hipMemcpy Ab");
+    if (nV) hip_check(hipMemcpy(h.Vt.p, Vt.data(), nV * es, hipMemcpyHostToDevice), "hipMemcpy Vt");
+    hip_check(hipMemcpy(h.perm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy perm");
+    hip_check(hipMemset(h.Uf.p, 0, nU * es), "hipMemset U");
+    hip_check(hipMemset(h.Lf.p, 0, nL * es), "hipMemset L");
+    hip_check(hipMemset(h.info.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info");
+    hip_check(hipMemset(h.cond.p, 0, (size_t)h.Bp * sizeof(double)), "hipMemset cond");
+}
+
+void reserve_impl(cn_bb& h, int nrhs) {
+    if (nrhs <= h.nrhs_cap) return;
+    h.ws.alloc(checked_mul({(size_t)h.B, (size_t)nrhs, (size_t)h.n, h.esz}));
+    h.nrhs_cap = nrhs;
+}
+
+void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
+    if (nrhs > h.nrhs_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            fail(CN_EINVAL, "BandedBatch solve: work space for " + std::to_string(nrhs) +
+                                " right-hand sides not reserved before graph capture (call cn_bb_reserve)");
+        reserve_impl(h, nrhs);
+    }
+    SolveFn f = (h.dtype == CN_C128) ? solve_fn<zc>(h.KLP, h.WU) : solve_fn<double>(h.KLP, h.WU);
+    if (!f) fail(CN_ELOGIC, "no solve kernel for (kl, wu) = (" + std::to_string(h.KLP) + ", " + std::to_string(h.WU) + ")");
+    const int lanes = h.B * nrhs;
+    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p,
+      h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, io, h.B, h.n, h.r, h.US,
+      nrhs);
+    hip_check(hipGetLastError(), "k_solve launch");
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return CN_OK;
+    } catch (const CnError& e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return CN_EOVERFLOW;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CN_ELOGIC;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cn_last_error(void) { return g_err.c_str(); }
+
+const char* cn_version(void) { return "cnhip 0.1.0 gfx950"; }
+
+int cn_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n, const int32_t* const* indptr,
+                 const int32_t* const* indices, const void* const* data, const int64_t* nnz,
+                 const int32_t* border, int r, int pivot, int col_border) {
+    return guarded([&] {
+        if (!out) fail(CN_EINVAL, "cn_bb_create: null output handle");
+        *out = nullptr;
+        if (dtype != CN_F64 && dtype != CN_C128) fail(CN_EINVAL, "cn_bb_create: dtype must be CN_F64 or CN_C128");
+        if (batch < 1) fail(CN_EINVAL, "BandedBatch: empty batch");
+        if (n < 1) fail(CN_EINVAL, "BandedBatch: n must be >= 1, got " + std::to_string(n));
+        if (r < 0 || r > n) fail(CN_EINVAL, "BandedBatch: border rank must be in [0, n]");
+        if (r > MAXR)
+            fail(CN_EUNSUPPORTED, "BandedBatch: border rank " + std::to_string(r) + " > " + std::to_string(MAXR) +
+                                      " not supported on the device");
+        if (col_border) fail(CN_EUNSUPPORTED, "BandedBatch: the column border is not supported by this library");
+        if (!indptr || !indices || !data || !nnz || (r > 0 && !border)) fail(CN_EINVAL, "cn_bb_create: null array");
+        auto h = std::make_unique<cn_bb>();
+        h->dtype = dtype;
+        h->B = batch;
+        h->n = n;
+        h->r = r;
+        h->pivot = pivot ? 1 : 0;
+        h->esz = dtype == CN_C128 ? 16 : 8;
+        if (dtype == CN_C128) create_impl<std::complex<double>>(*h, indptr, indices, data, nnz, border);
+        else create_impl<double>(*h, indptr, indices, data, nnz, border);
+        factor_and_check(*h, nullptr, true);
+        reserve_impl(*h, 1);
+        *out = h.release();
+    });
+}
+
+int cn_bb_factor(cn_bb_t h, void* stream, int check) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_factor: null handle");
+        factor_and_check(*h, static_cast<hipStream_t>(stream), check != 0);
+    });
+}
+
+int cn_bb_reserve(cn_bb_t h, int nrhs) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_reserve: null handle");
+        if (nrhs < 1) fail(CN_EINVAL, "cn_bb_reserve: nrhs must be >= 1");
+        reserve_impl(*h, nrhs);
+    });
+}
+
+int cn_bb_solve(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, int transpose, void* stream) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_solve: null handle");
+        if (!x) fail(CN_EINVAL, "BandedBatch.solve_ptr: null device pointer");
+        if (nrhs < 1) fail(CN_EINVAL, "cn_bb_solve: nrhs must be >= 1");
+        if (nrhs > 1 && rhs_stride < (int64_t)h->B * h->n)
+            fail(CN_EINVAL, "cn_bb_solve: rhs_stride must be >= batch * n");
+        if (transpose) fail(CN_EUNSUPPORTED, "BandedBatch: transposed solves are not built yet");
+        SolveIO io{};
+        io.mode = 0;
+        io.x = x;
+        io.rhs_stride = rhs_stride;
+        solve_impl(*h, io, nrhs, static_cast<hipStream_t>(stream));
+    });
+}
+
+int cn_bb_solve_mapped(cn_bb_t h, const void* rhs, const int64_t* rows, void* x, const int64_t* cols, void* stream) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_solve_mapped: null handle");
+        if (!rhs || !x) fail(CN_EINVAL, "BandedBatch.solve_mapped_ptr: null device pointer");
+        SolveIO io{};
+        io.mode = 1;
+        io.x = x;
+        io.rhs = rhs;
+        io.rows = rows;
+        io.cols = cols;
+        solve_impl(*h, io, 1, static_cast<hipStream_t>(stream));
+    });
+}
+
+int cn_bb_solve_split(cn_bb_t h, const double* re, const int64_t* rows_c, const int64_t* rows_s, double* x,
+                      const int64_t* cols_c, const int64_t* cols_s, void* stream) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_solve_split: null handle");
+        if (h->dtype != CN_C128) fail(CN_EINVAL, "BandedBatch.solve_split_ptr: complex128 factors required");
+        if (!re || !x || !rows_c || !rows_s || !cols_c || !cols_s)
+            fail(CN_EINVAL, "BandedBatch.solve_split_ptr: null device pointer");
+        SolveIO io{};
+        io.mode = 2;
+        io.x = x;
+        io.rhs = re;
+        io.rows = rows_c;
+        io.rows2 = rows_s;
+        io.cols = cols_c;
+        io.cols2 = cols_s;
+        solve_impl(*h, io, 1, static_cast<hipStream_t>(stream));
+    });
+}
+
+int cn_bb_prepare_transpose(cn_bb_t h) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_prepare_transpose: null handle");
+        fail(CN_EUNSUPPORTED, "BandedBatch: transposed solves are not built yet");
+    });
+}
+
+int cn_bb_props_get(cn_bb_t h, cn_bb_props* p) {
+    return guarded([&] {
+        if (!h || !p) fail(CN_EINVAL, "cn_bb_props_get: null argument");
+        p->dtype = h->dtype;
+        p->batch = h->B;
+        p->n = h->n;
+        p->kl = h->kl;
+        p->ku = h->ku;
+        p->rank = h->r;
+        p->pivot = h->pivot;
+        p->kl_kernel = h->KLP;
+        p->wu_kernel = h->WU;
+        p->lanes = h->G;
+        p->nrhs_reserved = h->nrhs_cap;
+        p->transpose_ready = 0;
+        p->cond_max = h->cond_max;
+        p->factor_ms = h->factor_ms;
+        p->factor_bytes = (int64_t)(h->Uf.bytes + h->Lf.bytes + h->piv.bytes + h->Vt.bytes + h->W.bytes + h->S.bytes +
+                                    h->spiv.bytes);
+    });
+}
+
+int cn_bb_cond(cn_bb_t h, double* out) {
+    return guarded([&] {
+        if (!h || !out) fail(CN_EINVAL, "cn_bb_cond: null argument");
+        for (int m = 0; m < h->B; ++m) out[m] = m < (int)h->hcond.size() ? h->hcond[m] : 1.0;
+    });
+}
+
+void cn_bb_destroy(cn_bb_t h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+/* cnhip.h — C ABI of libcnhip.so, the MI355X (gfx950) bordered-banded batch solver.
+ *
+ * Plain pointers, sizes and a hipStream_t passed as void*; no torch or C++ types cross this
+ * boundary.  Every entry point returns a status (CN_OK or a CN_E* code); the message of the
+ * last failure on the calling thread is cn_last_error().  Status codes map to the exception
+ * types of the reference binding (core/bindings/module.cpp:40-95, banded.hpp:45-56):
+ *   CN_EINVAL    -> ValueError     (shapes, CSR structure, non-finite entries, border rows)
+ *   CN_ESINGULAR -> RuntimeError   (message contains "singular", the mode index and the pivot)
+ *   CN_EOVERFLOW -> OverflowError  (size products)
+ *   CN_ELOGIC    -> internal error (a layout bug; never expected)
+ *   CN_EHIP      -> RuntimeError   (a HIP runtime failure)
+ *   CN_EUNSUPPORTED -> NotImplementedError (a shape outside the compiled kernel set)
+ *
+ * Device memory: the handle owns its factor storage (allocated at create).  Right-hand sides
+ * and solutions live in caller-owned device memory (e.g. torch tensors) and are solved in place
+ * on the caller's stream; solve entry points neither allocate nor synchronise the host once
+ * cn_bb_reserve() has sized the work space, so they can be captured in a HIP graph.
+ */
+#ifndef CNHIP_H
+#define CNHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CN_OK 0
+#define CN_EINVAL 1
+#define CN_ESINGULAR 2
+#define CN_EOVERFLOW 3
+#define CN_ELOGIC 4
+#define CN_EHIP 5
+#define CN_EUNSUPPORTED 6
+
+#define CN_F64 0  /* float64 factors and data */
+#define CN_C128 1 /* complex128 (interleaved re, im) */
+
+typedef struct cn_bb* cn_bb_t;
+
+typedef struct cn_bb_props {
+    int32_t dtype;      /* CN_F64 / CN_C128 */
+    int32_t batch;      /* number of systems */
+    int32_t n;          /* size of every system */
+    int32_t kl, ku;     /* common band of the bulk rows (max over the batch) */
+    int32_t rank;       /* border rows r */
+    int32_t pivot;      /* 1: partial pivoting inside the band */
+    int32_t kl_kernel;  /* band widths of the compiled kernel chosen (kl_kernel >= kl, ...) */
+    int32_t wu_kernel;  /* ... wu_kernel >= kl_kernel + ku) */
+    int32_t lanes;      /* lanes per system in the factor kernel */
+    int32_t nrhs_reserved;
+    int32_t transpose_ready;
+    double cond_max;    /* max over the batch of cond_1 of the capacitance matrix */
+    double factor_ms;   /* device time of the last factorization */
+    int64_t factor_bytes; /* device bytes held by the factors */
+} cn_bb_props;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cn_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int cn_device_count(void);
+
+/* Library version string. */
+const char* cn_version(void);
+
+/* Create and factor a batch of bordered-banded systems.
+ * Replaces _core.BorderedBandedBatch{D,Z}.__init__ (core/bindings/module.cpp:164-169) and
+ * cuda.BandedBatch{D,Z}.__init__ (core/bindings/module.cpp:326-332, core/src/cuda/banded.cu:1138-1237).
+ * mats: `batch` square CSR matrices of size n (int32 indptr[n+1], int32 indices[nnz], data
+ *   float64 or complex128 [nnz]); border: r distinct rows moved to the top (boundary rows);
+ * pivot: 1 = partial pivoting in the band (default of the reference), 0 = none (declared
+ *   diagonally dominant).  col_border must be 0 (column border not supported by this library).
+ * The layout (border rows first, common band, s-scaled border, V^T) follows
+ * core/include/conatus/banded.hpp:170-251; the factorization runs on the current device and the
+ * call synchronises once to check singular pivots (CN_ESINGULAR names the mode and pivot). */
+int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n,
+                 const int32_t* const* indptr, const int32_t* const* indices,
+                 const void* const* data, const int64_t* nnz,
+                 const int32_t* border, int r, int pivot, int col_border);
+
+/* Re-run the device factorization from the stored unfactored layout (the factor kernel alone,
+ * for timing and for re-use after cn_bb_set_values).  Asynchronous on `stream` when check = 0;
+ * check = 1 synchronises and reports singular systems like cn_bb_create. */
+int cn_bb_factor(cn_bb_t h, void* stream, int check);
+
+/* Reserve the device work space for solves of up to nrhs right-hand sides per system (the
+ * solve entry points then allocate nothing: graph-capturable). */
+int cn_bb_reserve(cn_bb_t h, int nrhs);
+
+/* In-place batched solve A_m x = b_m on caller device memory.
+ * x: nrhs blocks of (batch, n) C-contiguous values; block k starts at element k * rhs_stride.
+ * transpose = 1 solves A_m^T x = b_m (plain transpose).
+ * Replaces BandedBatch.solve_ptr / solve_transpose_ptr (core/bindings/module.cpp:379-401) and,
+ * with host staging in the Python layer, _core.BorderedBandedBatch{D,Z}.solve
+ * (core/bindings/module.cpp:181-206). */
+int cn_bb_solve(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, int transpose, void* stream);
+
+/* Out-of-place solve with the gather/scatter fused: b_m[i] = rhs[rows[m*n+i]],
+ * x[cols[m*n+i]] = x_m[i]; rows/cols int64 (batch, n) device arrays or NULL (contiguous).
+ * Replaces BandedBatch.solve_mapped_ptr (core/bindings/module.cpp:335-343). */
+int cn_bb_solve_mapped(cn_bb_t h, const void* rhs, const int64_t* rows, void* x,
+                       const int64_t* cols, void* stream);
+
+/* Complex systems of real data: b = re[rows_c] + i re[rows_s]; x[cols_c] = Re, x[cols_s] = Im
+ * (complex128 handles only).  Replaces BandedBatch.solve_split_ptr
+ * (core/bindings/module.cpp:344-353). */
+int cn_bb_solve_split(cn_bb_t h, const double* re, const int64_t* rows_c, const int64_t* rows_s,
+                      double* x, const int64_t* cols_c, const int64_t* cols_s, void* stream);
+
+/* Build the transposed Woodbury data now (before graph capture of transposed solves).
+ * Replaces BandedBatch.prepare_transpose (core/bindings/module.cpp:392-394). */
+int cn_bb_prepare_transpose(cn_bb_t h);
+
+/* Properties (n, kl, ku, rank, batch, cond_max, factor_ms, ...).  Replaces the read-only
+ * properties of cuda.BandedBatch (core/bindings/module.cpp:402-409). */
+int cn_bb_props_get(cn_bb_t h, cn_bb_props* p);
+
+/* Copy the per-system capacitance condition numbers (batch doubles) to host memory. */
+int cn_bb_cond(cn_bb_t h, double* out);
+
+/* Release the handle and its device memory. */
+void cn_bb_destroy(cn_bb_t h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CNHIP_H */

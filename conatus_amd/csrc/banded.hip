@@ -486,6 +486,146 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
 }
 
 // ------------------------------------------------------------------------------------------
+// Transposed solve A^T x = b (plain transpose), the discrete adjoint path (banded.hpp:346-372).
+// A^T = (B^T + V U^T) P, so y = P x solves (B^T + V U^T) y = b:
+//   y0 = B^-T b = P_B L^-T U^-T b,  z = St^-1 y0[:r],  y = y0 - Wt z,  x[perm[i]] = y[i],
+// with Wt = B^-T V and St = I + U^T Wt built once (prepare).  Lane = (system, rhs).
+//   phase 1: U^T w = b (forward; column j of U is read as U(j-d, j) = Uf[j-d][d]);
+//   phase 2: L^T and the row interchanges, backward (gbtrs 'T' order: dot, then swap);
+//   phase 3: Woodbury correction and the scatter through perm.
+// prep = 1 computes Wt instead: lane = (system, border index a), input row a of V^T, output and
+// work space row a of Wt, no phase 3.
+// ------------------------------------------------------------------------------------------
+template <class T, int KL, int WU>
+__global__ void __launch_bounds__(256)
+k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, T* __restrict__ Wt, const T* __restrict__ St,
+          const int* __restrict__ stpiv, const int* __restrict__ perm, T* __restrict__ ws,
+          T* __restrict__ x, int64_t rhs_stride, int B, int n, int r, int US, int nrhs, int prep) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / nrhs;
+    const int k = gid - m * nrhs;
+    if (m >= B) return;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
+    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    // input row, work row (stride between consecutive entries)
+    const T* in;
+    size_t in_st;
+    T* w;
+    size_t w_st;
+    if (prep) {
+        in = Vt + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
+        in_st = IL;
+        w = Wt + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
+        w_st = IL;
+    } else {
+        in = x + k * rhs_stride + (size_t)m * n;
+        in_st = 1;
+        w = ws + gid;
+        w_st = (size_t)B * nrhs;
+    }
+    // phase 1: U^T w = b
+    T uw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) uw[d] = cn::zero<T>();   // uw[d-1] = w_{j-d}
+    for (int j = 0; j < n; ++j) {
+        T acc = in[(size_t)j * in_st];
+#pragma unroll
+        for (int d = 1; d <= WU; ++d)
+            if (j - d >= 0) acc = cn::fnms(U0[((size_t)(j - d) * US + d) * IL], uw[d - 1], acc);
+        const T wj = cn::mul(acc, U0[(size_t)j * US * IL]);
+        w[(size_t)j * w_st] = wj;
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) uw[d] = uw[d - 1];
+        uw[0] = wj;
+    }
+    // phase 2: L^T with interchanges, j = n-1 .. 0; window y[i] = entry j+i
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        // entry j+1+KL leaves the window: no later step touches it, it is final
+        if (j + 1 + KL <= n - 1) w[(size_t)(j + 1 + KL) * w_st] = y[KL];
+#pragma unroll
+        for (int i = KL; i > 0; --i) y[i] = y[i - 1];
+        T v = w[(size_t)j * w_st];
+        const T* Lr = L0 + (size_t)j * KL * IL;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) v = cn::fnms(Lr[(i - 1) * IL], y[i], v);
+        y[0] = v;
+        const int p = P0[(size_t)j * IL];
+        if (p != 0) {
+#pragma unroll
+            for (int i = 1; i <= KL; ++i)
+                if (p == i) { const T tmp = y[0]; y[0] = y[i]; y[i] = tmp; }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i <= KL; ++i)
+        if (i < n) w[(size_t)i * w_st] = y[i];
+    if (prep) return;
+    // phase 3: z = St^-1 y0[:r]; y = y0 - Wt z; x[perm[i]] = y[i]
+    T z[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) z[a] = (a < r) ? w[(size_t)a * w_st] : cn::zero<T>();
+    if (r > 0) {
+        const T* Sm = St + (size_t)m * r * r;
+        const int* sp = stpiv + (size_t)m * r;
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            if (i >= r) break;
+            const int pi = sp[i];
+#pragma unroll
+            for (int q = i + 1; q < MAXR; ++q)
+                if (q == pi) { const T tmp = z[i]; z[i] = z[q]; z[q] = tmp; }
+        }
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            if (i >= r) break;
+#pragma unroll
+            for (int q = 0; q < i; ++q) z[i] = cn::fnms(Sm[q * r + i], z[q], z[i]);
+        }
+#pragma unroll
+        for (int i = MAXR - 1; i >= 0; --i) {
+            if (i < r) {
+#pragma unroll
+                for (int q = i + 1; q < MAXR; ++q)
+                    if (q < r) z[i] = cn::fnms(Sm[q * r + i], z[q], z[i]);
+                z[i] = cn::mul(z[i], cn::recip(Sm[i * r + i]));
+            }
+        }
+    }
+    const T* W0 = Wt + ch * ((size_t)r * n * IL) + ml;
+    T* xo = x + k * rhs_stride + (size_t)m * n;
+    for (int i = 0; i < n; ++i) {
+        T v = w[(size_t)i * w_st];
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * IL], z[a], v);
+        xo[perm[i]] = v;
+    }
+}
+
+// St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
+template <class T>
+__global__ void __launch_bounds__(256)
+k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int B, int n, int r) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / (r * r);
+    const int q = gid - m * r * r;
+    if (m >= B) return;
+    const int b = q / r, a = q - b * r;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    T v = Wt[(ch * ((size_t)r * n) + (size_t)b * n + a) * IL + ml];
+    if (a == b) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);
+    St[(size_t)m * r * r + (size_t)b * r + a] = v;
+}
+
+// ------------------------------------------------------------------------------------------
 // Host side
 // ------------------------------------------------------------------------------------------
 constexpr int KL_SET[] = {1, 2, 4, 6, 8, 12, 16};
@@ -525,6 +665,44 @@ void launch_solve(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf,
     hipLaunchKernelGGL((k_solve<T, KL, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
                        piv, static_cast<const T*>(Vt), static_cast<const T*>(W), static_cast<const T*>(S), spiv,
                        perm, static_cast<T*>(ws), io, B, n, r, US, nrhs);
+}
+
+using SolveTFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, const int*, const void*, void*,
+                          const void*, const int*, const int*, void*, void*, int64_t, int, int, int, int, int, int);
+
+template <class T, int KL, int WU>
+void launch_solve_t(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf, const int* piv, const void* Vt,
+                    void* Wt, const void* St, const int* stpiv, const int* perm, void* ws, void* x, int64_t rs, int B,
+                    int n, int r, int US, int nrhs, int prep) {
+    hipLaunchKernelGGL((k_solve_t<T, KL, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
+                       piv, static_cast<const T*>(Vt), static_cast<T*>(Wt), static_cast<const T*>(St), stpiv, perm,
+                       static_cast<T*>(ws), static_cast<T*>(x), rs, B, n, r, US, nrhs, prep);
+}
+
+template <class T, int KL>
+SolveTFn solve_t_fn_kl(int WU) {
+    if (WU < KL) return nullptr;
+    switch (WU) {
+        case 4: if constexpr (KL <= 4) return &launch_solve_t<T, KL, 4>; else return nullptr;
+        case 8: if constexpr (KL <= 8) return &launch_solve_t<T, KL, 8>; else return nullptr;
+        case 16: return &launch_solve_t<T, KL, 16>;
+        case 24: return &launch_solve_t<T, KL, 24>;
+        case 32: return &launch_solve_t<T, KL, 32>;
+    }
+    return nullptr;
+}
+template <class T>
+SolveTFn solve_t_fn(int KL, int WU) {
+    switch (KL) {
+        case 1: return solve_t_fn_kl<T, 1>(WU);
+        case 2: return solve_t_fn_kl<T, 2>(WU);
+        case 4: return solve_t_fn_kl<T, 4>(WU);
+        case 6: return solve_t_fn_kl<T, 6>(WU);
+        case 8: return solve_t_fn_kl<T, 8>(WU);
+        case 12: return solve_t_fn_kl<T, 12>(WU);
+        case 16: return solve_t_fn_kl<T, 16>(WU);
+    }
+    return nullptr;
 }
 
 template <class T>
@@ -603,6 +781,8 @@ struct cn_bb {
     double cond_max = 1.0, factor_ms = 0.0;
     int device = 0;
     DevBuf Ab, Uf, Lf, piv, Vt, W, S, spiv, cond, info, perm, ws;
+    DevBuf Wt, St, stpiv, cond_t, info_t;   // transposed Woodbury data (built on demand)
+    bool tr_ready = false;
     std::vector<double> hcond;
 };
 
@@ -798,6 +978,66 @@ void reserve_impl(cn_bb& h, int nrhs) {
     h.nrhs_cap = nrhs;
 }
 
+template <class T>
+void prepare_t_impl(cn_bb& h) {
+    const int B = h.B, n = h.n, r = h.r;
+    const size_t es = sizeof(T);
+    h.Wt.alloc(std::max<size_t>((size_t)h.Bp * r * n, 1) * es);
+    h.St.alloc(std::max<size_t>((size_t)h.Bp * r * r, 1) * es);
+    h.stpiv.alloc(std::max<size_t>((size_t)h.Bp * r, 1) * sizeof(int));
+    h.cond_t.alloc((size_t)h.Bp * sizeof(double));
+    h.info_t.alloc((size_t)h.Bp * sizeof(int));
+    hip_check(hipMemset(h.info_t.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info_t");
+    if (r > 0) {
+        SolveTFn f = solve_t_fn<T>(h.KLP, h.WU);
+        if (!f) fail(CN_ELOGIC, "no transposed solve kernel");
+        const int lanes = B * r;
+        f(dim3((lanes + 255) / 256), dim3(256), nullptr, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p,
+          h.Wt.p, h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), nullptr, nullptr, 0,
+          B, n, r, h.US, r, 1);
+        hip_check(hipGetLastError(), "k_solve_t (prepare) launch");
+        const int l2 = B * r * r;
+        hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, nullptr,
+                           static_cast<const T*>(h.Wt.p), static_cast<T*>(h.St.p), B, n, r);
+        hip_check(hipGetLastError(), "k_build_st launch");
+        hipLaunchKernelGGL((k_cap_lu<T>), dim3((B + 255) / 256), dim3(256), 0, nullptr, static_cast<T*>(h.St.p),
+                           static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
+                           static_cast<int*>(h.info_t.p), B, r);
+        hip_check(hipGetLastError(), "k_cap_lu (transposed) launch");
+        hip_check(hipDeviceSynchronize(), "prepare_transpose");
+        std::vector<int> info(B);
+        hip_check(hipMemcpy(info.data(), h.info_t.p, sizeof(int) * B, hipMemcpyDeviceToHost), "hipMemcpy info_t");
+        for (int m = 0; m < B; ++m)
+            if (info[m] != 0)
+                fail(CN_ESINGULAR, "BandedBatch: transposed capacitance matrix of mode " + std::to_string(m) +
+                                       " is singular");
+    }
+    h.tr_ready = true;
+}
+
+void prepare_t(cn_bb& h) {
+    if (h.tr_ready) return;
+    if (h.dtype == CN_C128) prepare_t_impl<zc>(h); else prepare_t_impl<double>(h);
+}
+
+void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
+    if (!h.tr_ready || nrhs > h.nrhs_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            fail(CN_EINVAL, "BandedBatch transposed solve: prepare_transpose / reserve must run before graph capture");
+        prepare_t(h);
+        reserve_impl(h, nrhs);
+    }
+    SolveTFn f = (h.dtype == CN_C128) ? solve_t_fn<zc>(h.KLP, h.WU) : solve_t_fn<double>(h.KLP, h.WU);
+    if (!f) fail(CN_ELOGIC, "no transposed solve kernel");
+    const int lanes = h.B * nrhs;
+    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.Wt.p,
+      h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), h.ws.p, x, rs, h.B, h.n, h.r,
+      h.US, nrhs, 0);
+    hip_check(hipGetLastError(), "k_solve_t launch");
+}
+
 void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
     if (nrhs > h.nrhs_cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -881,6 +1121,7 @@ int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n, const int32_t* const
 int cn_bb_factor(cn_bb_t h, void* stream, int check) {
     return guarded([&] {
         if (!h) fail(CN_EINVAL, "cn_bb_factor: null handle");
+        h->tr_ready = false;   // the transposed data follow the factors
         factor_and_check(*h, static_cast<hipStream_t>(stream), check != 0);
     });
 }
@@ -900,7 +1141,10 @@ int cn_bb_solve(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, int transpose,
         if (nrhs < 1) fail(CN_EINVAL, "cn_bb_solve: nrhs must be >= 1");
         if (nrhs > 1 && rhs_stride < (int64_t)h->B * h->n)
             fail(CN_EINVAL, "cn_bb_solve: rhs_stride must be >= batch * n");
-        if (transpose) fail(CN_EUNSUPPORTED, "BandedBatch: transposed solves are not built yet");
+        if (transpose) {
+            solve_t_impl(*h, x, nrhs, rhs_stride, static_cast<hipStream_t>(stream));
+            return;
+        }
         SolveIO io{};
         io.mode = 0;
         io.x = x;
@@ -945,7 +1189,7 @@ int cn_bb_solve_split(cn_bb_t h, const double* re, const int64_t* rows_c, const 
 int cn_bb_prepare_transpose(cn_bb_t h) {
     return guarded([&] {
         if (!h) fail(CN_EINVAL, "cn_bb_prepare_transpose: null handle");
-        fail(CN_EUNSUPPORTED, "BandedBatch: transposed solves are not built yet");
+        prepare_t(*h);
     });
 }
 
@@ -963,7 +1207,7 @@ int cn_bb_props_get(cn_bb_t h, cn_bb_props* p) {
         p->wu_kernel = h->WU;
         p->lanes = h->G;
         p->nrhs_reserved = h->nrhs_cap;
-        p->transpose_ready = 0;
+        p->transpose_ready = h->tr_ready ? 1 : 0;
         p->cond_max = h->cond_max;
         p->factor_ms = h->factor_ms;
         p->factor_bytes = (int64_t)(h->Uf.bytes + h->Lf.bytes + h->piv.bytes + h->Vt.bytes + h->W.bytes + h->S.bytes +

@@ -19,7 +19,9 @@ CN_F64, CN_C128 = 0, 1
 #: every symbol include/cnhip.h declares (tests check the library exports all of them)
 EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn_bb_factor",
            "cn_bb_reserve", "cn_bb_solve", "cn_bb_solve_mapped", "cn_bb_solve_split",
-           "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy")
+           "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
+           "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
+           "cn_pointwise")
 
 
 class Props(ctypes.Structure):
@@ -37,6 +39,7 @@ _lib = None
 _vp = ctypes.c_void_p
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
 
 
 def _declare(lib):
@@ -56,6 +59,15 @@ def _declare(lib):
         "cn_bb_props_get": (ctypes.c_int, [_vp, ctypes.POINTER(Props)]),
         "cn_bb_cond": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
         "cn_bb_destroy": (None, [_vp]),
+        "cn_csr_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p]),
+        "cn_csr_destroy": (None, [_vp]),
+        "cn_csr_matvec": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp]),
+        "cn_sbdf_stage": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_vp), ctypes.c_int, _dp, _dp, _vp, _vp]),
+        "cn_rk_rhs": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, ctypes.POINTER(_vp),
+                                     ctypes.POINTER(_vp), _dp, _dp, ctypes.c_int, _vp, _vp]),
+        "cn_pointwise": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

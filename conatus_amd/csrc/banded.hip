@@ -773,6 +773,8 @@ struct DevBuf {
 
 }  // namespace
 
+void cn::set_error(const std::string& msg) { g_err = msg; }
+
 struct cn_bb {
     int dtype = CN_C128, B = 0, Bp = 0, n = 0, r = 0, kl = 0, ku = 0, KLP = 0, M = 0, WU = 0, US = 0, G = 0;
     int pivot = 1;

@@ -5,7 +5,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace cn {
+
+// Message of the last failure on this thread (read by cn_last_error).
+void set_error(const std::string& msg);
 
 struct __align__(16) zc {
     double re, im;

@@ -1,0 +1,346 @@
+"""Batched IMEX integrator on the MI355X: M_k dX/dt + L_k X = F for every mode k.
+
+The device twin of the reference's ``cuda.Imex`` (python/conatus/runtime/cuda.py:267-308,
+core/src/cuda/imex.cu:56-244) and of the host ``ImexIntegrator`` step arithmetic
+(core/include/conatus/imex.hpp:73-219): SBDF1..4 with the lower-order startup, RK111 / RK222 /
+RK443, factorizations cached by their LHS coefficients (RK443: one), the continuation rule of
+SBDF histories, and F = g * X + h (pointwise) or an external F writer.
+
+One SBDF step is two launches on the current stream: the fused stage kernel (M X, F, history
+level and right-hand side in one pass) and the out-of-place batched solve into X.  History
+levels live in a ring of fixed device buffers, so a full-order step captured as a HIP graph per
+ring phase replays without copies (``run_graph``).
+"""
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+
+import numpy as np
+import scipy.sparse as sp
+
+from .. import _lib
+from ..errors import OptionError
+from .banded import BatchHandle, _factor_dtype, csr_args, solve_operand
+
+__all__ = ["Imex", "DeviceCsr", "SCHEMES", "sbdf_coefficients", "rk_tableau"]
+
+SCHEMES = ("SBDF1", "SBDF2", "SBDF3", "SBDF4", "RK111", "RK222", "RK443")
+
+
+def sbdf_coefficients(order: int):
+    """(a, c), a scaled by dt: the constant-step SBDF1..4 constants (imex.hpp:63-71)."""
+    table = {1: ((1.0, -1.0), (0.0, 1.0)),
+             2: ((1.5, -2.0, 0.5), (0.0, 2.0, -1.0)),
+             3: ((11.0 / 6, -3.0, 1.5, -1.0 / 3), (0.0, 3.0, -3.0, 1.0)),
+             4: ((25.0 / 12, -4.0, 3.0, -4.0 / 3, 0.25), (0.0, 4.0, -6.0, 4.0, -1.0))}
+    if order not in table:
+        raise OptionError("SBDF order must be 1..4")
+    return table[order]
+
+
+def rk_tableau(name: str):
+    """(A, H, c) of the ARS IMEX tableaux RK111, RK222, RK443 (imex.hpp:31-51)."""
+    if name == "RK111":
+        return ((0.0, 0.0), (1.0, 0.0)), ((0.0, 0.0), (0.0, 1.0)), (0.0, 1.0)
+    if name == "RK222":
+        g = (2.0 - np.sqrt(2.0)) / 2.0
+        d = 1.0 - 1.0 / g / 2.0
+        return (((0.0, 0.0, 0.0), (g, 0.0, 0.0), (d, 1 - d, 0.0)),
+                ((0.0, 0.0, 0.0), (0.0, g, 0.0), (0.0, 1 - g, g)), (0.0, g, 1.0))
+    if name == "RK443":
+        return (((0, 0, 0, 0, 0), (1 / 2, 0, 0, 0, 0), (11 / 18, 1 / 18, 0, 0, 0),
+                 (5 / 6, -5 / 6, 1 / 2, 0, 0), (1 / 4, 7 / 4, 3 / 4, -7 / 4, 0)),
+                ((0, 0, 0, 0, 0), (0, 1 / 2, 0, 0, 0), (0, 1 / 6, 1 / 2, 0, 0),
+                 (0, -1 / 2, 1 / 2, 1 / 2, 0), (0, 3 / 2, -3 / 2, 1 / 2, 1 / 2)),
+                (0, 1 / 2, 2 / 3, 1 / 2, 1))
+    raise OptionError(f"unknown RK scheme: {name}", fix=f"one of {SCHEMES}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream() -> int:
+    return _torch().cuda.current_stream().cuda_stream
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+class DeviceCsr:
+    """A batch of CSR matrices (same n) as one block-diagonal device operator (``cn_csr``)."""
+
+    def __init__(self, mats: Sequence, dtype):
+        self.dtype = np.dtype(dtype)
+        csrs = [csr_args(A, self.dtype) for A in mats]
+        self.batch, self.n = len(csrs), int(csrs[0][0])
+        keep = [(c[1], c[2], np.ascontiguousarray(c[3], dtype=self.dtype)) for c in csrs]
+        P = ctypes.c_void_p * self.batch
+        h = ctypes.c_void_p()
+        code = _lib.CN_C128 if self.dtype == np.complex128 else _lib.CN_F64
+        st = _lib.lib().cn_csr_create(ctypes.byref(h), code, self.batch, self.n,
+                                      P(*[k[0].ctypes.data for k in keep]), P(*[k[1].ctypes.data for k in keep]),
+                                      P(*[k[2].ctypes.data for k in keep]),
+                                      (ctypes.c_int64 * self.batch)(*[int(k[1].size) for k in keep]))
+        _lib.check(st, "DeviceCsr")
+        self._h = h
+
+    def matvec(self, x, y, alpha: float = 1.0, accumulate: bool = False) -> None:
+        _lib.check(_lib.lib().cn_csr_matvec(self._h, ctypes.c_void_p(_ptr(x)), ctypes.c_void_p(_ptr(y)),
+                                            float(alpha), 1 if accumulate else 0, ctypes.c_void_p(_stream())),
+                   "DeviceCsr.matvec")
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                _lib.lib().cn_csr_destroy(h)
+            except Exception:
+                pass
+
+
+class Imex:
+    """Batched IMEX integrator on the GPU (same constructor as python/conatus/runtime/cuda.py:267-286).
+
+    ``g``, ``h``: (batch, n) host arrays of the pointwise explicit term F = g * X + h (None: 0).
+    Scheme and dt are required: nothing is chosen for the user."""
+
+    def __init__(self, Ms: Sequence, Ls: Sequence, border_rows: Sequence[int], scheme: str, dt: float,
+                 g=None, h=None, pivot: bool = True, dtype=None):
+        torch = _torch()
+        if scheme not in SCHEMES:
+            raise OptionError(f"Imex: unknown scheme {scheme!r}", fix=f"one of {SCHEMES}")
+        if not float(dt) > 0:
+            raise OptionError("dt must be > 0")
+        Ms = [sp.csr_matrix(A) for A in Ms]
+        Ls = [sp.csr_matrix(A) for A in Ls]
+        if not Ms or len(Ms) != len(Ls):
+            raise OptionError("Imex: M and L batches differ")
+        self.dtype = np.dtype(dtype) if dtype is not None else np.dtype(_factor_dtype(Ms + Ls))
+        self.n = Ms[0].shape[0]
+        self.batch = len(Ms)
+        for A in Ms + Ls:
+            if A.shape != (self.n, self.n):
+                raise OptionError("Imex: all modes need the same n")
+        self.scheme, self.dt = scheme, float(dt)
+        self.border_rows = [int(i) for i in border_rows]
+        self.pivot = bool(pivot)
+        self._Mh = [A.astype(self.dtype) for A in Ms]
+        self._Lh = [A.astype(self.dtype) for A in Ls]
+        self._M = DeviceCsr(self._Mh, self.dtype)
+        self._L = DeviceCsr(self._Lh, self.dtype)
+        self._tdt = torch.complex128 if self.dtype == np.complex128 else torch.float64
+        N = self.batch * self.n
+
+        def conv(a):
+            if a is None:
+                return None
+            a = solve_operand(a, self.dtype, "Imex (g, h)").ravel()
+            if a.size != N:
+                raise OptionError("pointwise rhs: expected batch*n values")
+            return torch.from_numpy(a).to("cuda")
+        self._g, self._hh = conv(g), conv(h)
+        if scheme.startswith("SBDF"):
+            self.order = int(scheme[4])
+            self.tab = None
+        else:
+            self.order = 0
+            self.tab = rk_tableau(scheme)
+        self._factors: dict = {}
+        z = lambda: torch.zeros(N, dtype=self._tdt, device="cuda")   # noqa: E731
+        self._rhs = z()
+        if self.order:
+            self._ring = [(z(), z()) for _ in range(self.order)]     # (MX, F) levels
+        else:
+            s = len(self.tab[2]) - 1
+            self._X = [z() for _ in range(s)]
+            self._F = [z() for _ in range(s)]
+            self._LX = [z() for _ in range(s)]
+            self._MX0 = z()
+        self._graphs = {}
+        self.reset()
+
+    # -- state ---------------------------------------------------------------------------------
+    def reset(self) -> None:
+        """Forget the SBDF history and the iteration count (factorizations kept)."""
+        self._levels: list[int] = []          # ring slots, newest first
+        self._last = None
+        self.iteration = 0
+
+    @property
+    def history_length(self) -> int:
+        return len(self._levels) if self.order else 0
+
+    @property
+    def n_factorizations(self) -> int:
+        return len(self._factors)
+
+    def _factor(self, alpha: float, beta: float) -> BatchHandle:
+        key = (alpha, beta)
+        f = self._factors.get(key)
+        if f is None:
+            lhs = [csr_args((self.dtype.type(alpha) * M + self.dtype.type(beta) * L).tocsr(), self.dtype)
+                   for M, L in zip(self._Mh, self._Lh)]
+            f = BatchHandle(lhs, self.border_rows, self.dtype, self.pivot)
+            self._factors[key] = f
+        return f
+
+    # -- steps ---------------------------------------------------------------------------------
+    def _sbdf_step(self, x) -> None:
+        # the slot of the new level: a free slot, or the oldest level's
+        if len(self._levels) < self.order:
+            used = set(self._levels)
+            slot = next(i for i in range(self.order) if i not in used)
+            levels = [slot] + self._levels
+        else:
+            slot = self._levels[-1]
+            levels = [slot] + self._levels[:-1]
+        self._sbdf_launch(x, levels)
+        self._levels = levels
+
+    def _sbdf_launch(self, x, levels) -> None:
+        q = len(levels)
+        a, c = sbdf_coefficients(q)
+        mx0, f0 = self._ring[levels[0]]
+        prev = levels[1:]
+        P = ctypes.c_void_p * 4
+        mxh = P(*([_ptr(self._ring[i][0]) for i in prev] + [0] * (4 - len(prev))))
+        fh = P(*([_ptr(self._ring[i][1]) for i in prev] + [0] * (4 - len(prev))))
+        cc = (ctypes.c_double * 5)(*([0.0] + [float(c[j]) for j in range(1, q + 1)] + [0.0] * (4 - q)))
+        adt = (ctypes.c_double * 5)(*([0.0] + [float(a[j] / self.dt) for j in range(1, q + 1)] + [0.0] * (4 - q)))
+        v = ctypes.c_void_p
+        st = _lib.lib().cn_sbdf_stage(self._M._h, v(_ptr(x)), v(_ptr(self._g)), v(_ptr(self._hh)), None,
+                                      v(_ptr(mx0)), v(_ptr(f0)), mxh, fh, q, cc, adt, v(_ptr(self._rhs)),
+                                      v(_stream()))
+        _lib.check(st, "Imex (SBDF stage)")
+        fac = self._factor(a[0] / self.dt, 1.0)
+        fac.solve_mapped_ptr(_ptr(self._rhs), 0, _ptr(x), 0, _stream())
+
+    def _rk_step(self, x) -> None:
+        A, H, cc = self.tab
+        s = len(cc) - 1
+        N = self.batch * self.n
+        code = _lib.CN_C128 if self.dtype == np.complex128 else _lib.CN_F64
+        X = [x] + self._X[: s - 1] + [x]      # X_0 = x; the last stage is written into x
+        self._M.matvec(x, self._MX0)
+        needF = lambda j: any(A[i][j] != 0 for i in range(j + 1, s + 1))    # noqa: E731
+        needLX = lambda j: any(H[i][j] != 0 for i in range(j + 1, s + 1))   # noqa: E731
+        v = ctypes.c_void_p
+        for i in range(1, s + 1):
+            j = i - 1
+            if needF(j):
+                _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(self._F[j])), v(_ptr(X[j])), v(_ptr(self._g)),
+                                                   v(_ptr(self._hh)), v(_stream())), "Imex (F)")
+            if needLX(j):
+                self._L.matvec(X[j], self._LX[j])
+            P = ctypes.c_void_p * 4
+            dA = [self.dt * A[i][k] for k in range(i)]
+            dH = [self.dt * H[i][k] for k in range(i)]
+            st = _lib.lib().cn_rk_rhs(code, N, v(_ptr(self._MX0)),
+                                      P(*[_ptr(self._F[k]) if dA[k] != 0 else 0 for k in range(i)]),
+                                      P(*[_ptr(self._LX[k]) if dH[k] != 0 else 0 for k in range(i)]),
+                                      (ctypes.c_double * 4)(*dA), (ctypes.c_double * 4)(*dH), i,
+                                      v(_ptr(self._rhs)), v(_stream()))
+            _lib.check(st, "Imex (RK stage)")
+            fac = self._factor(1.0, self.dt * H[i][i])
+            fac.solve_mapped_ptr(_ptr(self._rhs), 0, _ptr(X[i]), 0, _stream())
+
+    def _check_state(self, d) -> None:
+        torch = _torch()
+        if not isinstance(d, torch.Tensor) or d.device.type != "cuda" or d.dtype != self._tdt \
+                or tuple(d.shape) != (self.batch, self.n) or not d.is_contiguous():
+            raise OptionError(f"Imex.run: expected a contiguous ({self.batch}, {self.n}) "
+                              f"{np.dtype(self.dtype).name} device tensor")
+
+    def _continuation(self, d, t0: float) -> None:
+        torch = _torch()
+        if self.history_length and not (self._last is not None and self._last[1] == float(t0)
+                                         and torch.equal(self._last[0], d)):
+            raise OptionError(f"Imex.run: {self.scheme} holds the history of a previous run, and (x0, t0) "
+                              "is not the state and time that run returned",
+                              fix="call reset() to start a new trajectory (lower-order startup again), "
+                                  "or pass the returned (x, t) to continue")
+
+    def _remember(self, d, t) -> None:
+        if self.order:
+            self._last = (d.clone(), float(t))
+
+    def run(self, d, t0: float, nsteps: int) -> float:
+        """Advance the (batch, n) device tensor ``d`` in place by ``nsteps``; returns the time
+        t0 + nsteps dt computed as the reference does (one addition per step, imex.hpp:112-116)."""
+        self._check_state(d)
+        if int(nsteps) != nsteps or nsteps < 0:
+            raise OptionError("Imex.run: nsteps must be a whole number >= 0")
+        self._continuation(d, t0)
+        t = float(t0)
+        for _ in range(int(nsteps)):
+            if self.order:
+                self._sbdf_step(d)
+            else:
+                self._rk_step(d)
+            t += self.dt
+            self.iteration += 1
+        self._remember(d, t)
+        return t
+
+    def run_graph(self, d, t0: float, nsteps: int) -> float:
+        """As ``run``, with every full-order step replayed from a captured HIP graph (one graph per
+        ring phase of the SBDF history; one graph for RK).  Startup steps run eagerly."""
+        torch = _torch()
+        self._check_state(d)
+        self._continuation(d, t0)
+        t = float(t0)
+        done = 0
+        if self.order:
+            while self.history_length < self.order - 1 and done < nsteps:   # lower-order startup
+                self._sbdf_step(d)
+                t += self.dt
+                self.iteration += 1
+                done += 1
+            if done < nsteps:
+                a, _ = sbdf_coefficients(self.order)
+                self._factor(a[0] / self.dt, 1.0)                            # built before capture
+        else:
+            A, H, cc = self.tab
+            for i in range(1, len(cc)):
+                self._factor(1.0, self.dt * H[i][i])
+        key = d.data_ptr()
+        while done < nsteps:
+            if self.order:
+                if len(self._levels) < self.order:
+                    used = set(self._levels)
+                    slot = next(i for i in range(self.order) if i not in used)
+                    levels = [slot] + self._levels
+                else:
+                    levels = [self._levels[-1]] + self._levels[:-1]
+                gk = (key, tuple(levels))
+                fn = lambda lv=levels: self._sbdf_launch(d, lv)     # noqa: E731
+            else:
+                gk = (key, "rk")
+                fn = lambda: self._rk_step(d)                       # noqa: E731
+            g = self._graphs.get(gk)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                s = torch.cuda.Stream()
+                s.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.graph(g, stream=s):
+                    fn()
+                torch.cuda.current_stream().wait_stream(s)
+                self._graphs[gk] = g
+            g.replay()
+            if self.order:
+                self._levels = list(gk[1])
+            t += self.dt
+            self.iteration += 1
+            done += 1
+        self._remember(d, t)
+        return t
+
+    def explain(self) -> str:
+        return (f"hip.Imex(scheme={self.scheme}, dt={self.dt:g}, batch={self.batch}, n={self.n}, "
+                f"F={'g*X+h' if (self._g is not None or self._hh is not None) else 'none'}, "
+                f"factorizations={self.n_factorizations}); state resident on the device, fused "
+                "stage kernel + batched bordered banded solve per stage")

@@ -122,6 +122,37 @@ int cn_bb_cond(cn_bb_t h, double* out);
 /* Release the handle and its device memory. */
 void cn_bb_destroy(cn_bb_t h);
 
+/* ---- IMEX stage (core/include/conatus/imex.hpp) -------------------------------------------
+ * Batched block-diagonal CSR operators on the device and the fused right-hand-side assembly of
+ * M dX/dt + L X = F.  Every product and sum is rounded separately, in the reference's order, so
+ * the assembled right-hand sides equal the host integrator's bit for bit.  N = batch * n. */
+typedef struct cn_csr* cn_csr_t;
+
+/* Upload `batch` CSR matrices of size n (same dtype) as one block-diagonal device operator.
+ * Replaces the device copy of M / L in cuda.Imex (core/src/cuda/imex.cu:56-86). */
+int cn_csr_create(cn_csr_t* out, int dtype, int batch, int n, const int32_t* const* indptr,
+                  const int32_t* const* indices, const void* const* data, const int64_t* nnz);
+void cn_csr_destroy(cn_csr_t A);
+
+/* y = alpha * A x (+ y if accumulate), one row per thread, storage order (banded.hpp:70-76;
+ * DeviceCsr.__matmul__, python/conatus/timestep/device.py:78-105). */
+int cn_csr_matvec(cn_csr_t A, const void* x, void* y, double alpha, int accumulate, void* stream);
+
+/* One SBDF stage (imex.hpp:158-176, lincomb.py:146-162): mx0 = M x; f0 = fext, or g x + h
+ * (g, h may be NULL); rhs = sum_{j=1..q} c[j] F^{n-j} - a_over_dt[j] M X^{n-j}, level 1 being
+ * (f0, mx0) and levels 2..q read from f_hist[j-2], mx_hist[j-2].  f0 may be NULL (not stored). */
+int cn_sbdf_stage(cn_csr_t M, const void* x, const void* g, const void* h, const void* fext, void* mx0,
+                  void* f0, const void* const* mx_hist, const void* const* f_hist, int q, const double* c,
+                  const double* a_over_dt, void* rhs, void* stream);
+
+/* RK stage right-hand side (imex.hpp:193-199, lincomb.py:73-139):
+ * rhs = mx0 + sum_{k<stages} dtA[k] F[k] - dtH[k] LX[k] (zero coefficients skipped). */
+int cn_rk_rhs(int dtype, int64_t N, const void* mx0, const void* const* F, const void* const* LX,
+              const double* dtA, const double* dtH, int stages, void* rhs, void* stream);
+
+/* out = g * x + h (pointwise explicit term of cuda.Imex, core/src/cuda/imex.cu:47-55). */
+int cn_pointwise(int dtype, int64_t N, void* out, const void* x, const void* g, const void* h, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_bb_reserve", "cn_bb_solve", "cn_bb_solve_mapped", "cn_bb_solve_split",
            "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
            "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
-           "cn_pointwise")
+           "cn_pointwise", "cn_rb2d_explicit")
 
 
 class Props(ctypes.Structure):
@@ -68,6 +68,9 @@ def _declare(lib):
         "cn_rk_rhs": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, ctypes.POINTER(_vp),
                                      ctypes.POINTER(_vp), _dp, _dp, ctypes.c_int, _vp, _vp]),
         "cn_pointwise": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
+        "cn_rb2d_explicit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_double, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -628,7 +628,7 @@ k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int B, int n, int r) {
 // ------------------------------------------------------------------------------------------
 // Host side
 // ------------------------------------------------------------------------------------------
-constexpr int KL_SET[] = {1, 2, 4, 6, 8, 12, 16};
+constexpr int KL_SET[] = {1, 2, 4, 6, 8, 10, 12, 14, 16};
 constexpr int WU_SET[] = {4, 8, 16, 24, 32};
 
 int pick_kl(int kl) {
@@ -699,7 +699,9 @@ SolveTFn solve_t_fn(int KL, int WU) {
         case 4: return solve_t_fn_kl<T, 4>(WU);
         case 6: return solve_t_fn_kl<T, 6>(WU);
         case 8: return solve_t_fn_kl<T, 8>(WU);
+        case 10: return solve_t_fn_kl<T, 10>(WU);
         case 12: return solve_t_fn_kl<T, 12>(WU);
+        case 14: return solve_t_fn_kl<T, 14>(WU);
         case 16: return solve_t_fn_kl<T, 16>(WU);
     }
     return nullptr;
@@ -713,7 +715,9 @@ FactorFn factor_fn(int KL) {
         case 4: return &launch_factor<T, 4>;
         case 6: return &launch_factor<T, 6>;
         case 8: return &launch_factor<T, 8>;
+        case 10: return &launch_factor<T, 10>;
         case 12: return &launch_factor<T, 12>;
+        case 14: return &launch_factor<T, 14>;
         case 16: return &launch_factor<T, 16>;
     }
     return nullptr;
@@ -749,7 +753,9 @@ SolveFn solve_fn(int KL, int WU) {
         case 4: return solve_fn_kl<T, 4>(WU);
         case 6: return solve_fn_kl<T, 6>(WU);
         case 8: return solve_fn_kl<T, 8>(WU);
+        case 10: return solve_fn_kl<T, 10>(WU);
         case 12: return solve_fn_kl<T, 12>(WU);
+        case 14: return solve_fn_kl<T, 14>(WU);
         case 16: return solve_fn_kl<T, 16>(WU);
     }
     return nullptr;

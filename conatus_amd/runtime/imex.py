@@ -109,7 +109,7 @@ class Imex:
     Scheme and dt are required: nothing is chosen for the user."""
 
     def __init__(self, Ms: Sequence, Ls: Sequence, border_rows: Sequence[int], scheme: str, dt: float,
-                 g=None, h=None, pivot: bool = True, dtype=None):
+                 g=None, h=None, pivot: bool = True, dtype=None, explicit=None):
         torch = _torch()
         if scheme not in SCHEMES:
             raise OptionError(f"Imex: unknown scheme {scheme!r}", fix=f"one of {SCHEMES}")
@@ -143,6 +143,11 @@ class Imex:
                 raise OptionError("pointwise rhs: expected batch*n values")
             return torch.from_numpy(a).to("cuda")
         self._g, self._hh = conv(g), conv(h)
+        # explicit: an object with base() -> (batch*n) device tensor of the constant rows of F and
+        # write(x, out) launching F(x) into the rows it owns (other rows of out keep base())
+        self._explicit = explicit
+        if explicit is not None and (g is not None or h is not None):
+            raise OptionError("Imex: pass either the pointwise g, h or an explicit term, not both")
         if scheme.startswith("SBDF"):
             self.order = int(scheme[4])
             self.tab = None
@@ -152,12 +157,14 @@ class Imex:
         self._factors: dict = {}
         z = lambda: torch.zeros(N, dtype=self._tdt, device="cuda")   # noqa: E731
         self._rhs = z()
+        fz = (lambda: explicit.base().clone()) if explicit is not None else z
         if self.order:
             self._ring = [(z(), z()) for _ in range(self.order)]     # (MX, F) levels
+            self._fnew = fz() if explicit is not None else None
         else:
             s = len(self.tab[2]) - 1
             self._X = [z() for _ in range(s)]
-            self._F = [z() for _ in range(s)]
+            self._F = [fz() for _ in range(s)]
             self._LX = [z() for _ in range(s)]
             self._MX0 = z()
         self._graphs = {}
@@ -212,7 +219,11 @@ class Imex:
         cc = (ctypes.c_double * 5)(*([0.0] + [float(c[j]) for j in range(1, q + 1)] + [0.0] * (4 - q)))
         adt = (ctypes.c_double * 5)(*([0.0] + [float(a[j] / self.dt) for j in range(1, q + 1)] + [0.0] * (4 - q)))
         v = ctypes.c_void_p
-        st = _lib.lib().cn_sbdf_stage(self._M._h, v(_ptr(x)), v(_ptr(self._g)), v(_ptr(self._hh)), None,
+        fext = None
+        if self._explicit is not None:
+            self._explicit.write(x, self._fnew)
+            fext = v(_ptr(self._fnew))
+        st = _lib.lib().cn_sbdf_stage(self._M._h, v(_ptr(x)), v(_ptr(self._g)), v(_ptr(self._hh)), fext,
                                       v(_ptr(mx0)), v(_ptr(f0)), mxh, fh, q, cc, adt, v(_ptr(self._rhs)),
                                       v(_stream()))
         _lib.check(st, "Imex (SBDF stage)")
@@ -232,8 +243,12 @@ class Imex:
         for i in range(1, s + 1):
             j = i - 1
             if needF(j):
-                _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(self._F[j])), v(_ptr(X[j])), v(_ptr(self._g)),
-                                                   v(_ptr(self._hh)), v(_stream())), "Imex (F)")
+                if self._explicit is not None:
+                    self._explicit.write(X[j], self._F[j])
+                else:
+                    _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(self._F[j])), v(_ptr(X[j])),
+                                                       v(_ptr(self._g)), v(_ptr(self._hh)), v(_stream())),
+                               "Imex (F)")
             if needLX(j):
                 self._L.matvec(X[j], self._LX[j])
             P = ctypes.c_void_p * 4
@@ -341,6 +356,6 @@ class Imex:
 
     def explain(self) -> str:
         return (f"hip.Imex(scheme={self.scheme}, dt={self.dt:g}, batch={self.batch}, n={self.n}, "
-                f"F={'g*X+h' if (self._g is not None or self._hh is not None) else 'none'}, "
+                f"F={'g*X+h' if (self._g is not None or self._hh is not None) else type(self._explicit).__name__ if self._explicit is not None else 'none'}, "
                 f"factorizations={self.n_factorizations}); state resident on the device, fused "
                 "stage kernel + batched bordered banded solve per stage")

@@ -153,6 +153,19 @@ int cn_rk_rhs(int dtype, int64_t N, const void* mx0, const void* const* F, const
 /* out = g * x + h (pointwise explicit term of cuda.Imex, core/src/cuda/imex.cu:47-55). */
 int cn_pointwise(int dtype, int64_t N, void* out, const void* x, const void* g, const void* h, void* stream);
 
+/* ---- Fourier x Chebyshev explicit term (2D Rayleigh-Benard) -------------------------------
+ * F rows of the momentum and temperature equations from the state X (K modes x n, complex):
+ * S_{0->2} forward(-u.grad u, -u.grad T) on the 3/2 grid.  Replaces the reference's GPU 2D
+ * pipeline FourierChebyshev2DRHS.evaluate (python/conatus/timestep/fc2d.py:567-643) and the fused
+ * product evaluation (python/conatus/timestep/fused.py:994-1085).  Layout: coefficient m of field f (u, w, T) of
+ * mode k at X[k*n + fidx[f*Nz + m]] (fidx: 3 x Nz device ints), equation e rows at F[k*n + R + 4*m + e]; other rows of F are left
+ * untouched.  Scratch: Z (6 K Mz complex), G (3 Mx Mz real), Gh (3 K Mz complex).  Matrices:
+ * Bz (2 x Nz x Mz: T_n, T_n' at the nodes), Ex (Mx x K complex backward Fourier weights),
+ * Fx (K x Mx complex forward), Az (Mz x Nz: (S02 Fz)^T). */
+int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Z, void* G, void* Gh, const double* Bz,
+                     const void* Ex, const void* Fx, const double* Az, int K, int Nz, int Mx, int Mz, int n, int R,
+                     double kappa, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,16 +41,27 @@ def _rhs(rng, shape, dtype):
     return b
 
 
-def _check_against_oracle(mats, border, b, x, dtype):
+def _check_against_oracle(mats, border, b, x, dtype, well_posed=True):
+    """well_posed: the oracle's own backward error meets the bar, so the GPU must too.  Otherwise
+    (random non-dominant bands, where the Woodbury form of the reference is itself unstable) the
+    reference's cross-backend rule applies: residual within 10x the oracle's plus n eps."""
     for k, A in enumerate(mats):
-        o = OracleBB(A, border, dtype)
+        try:
+            o = OracleBB(A, border, dtype)
+        except RuntimeError:
+            continue                       # singular for the oracle: no reference solution
         xo = o.solve(b[k])
         res_o = np.max(abs(A @ xo - b[k]))
         res = np.max(abs(A @ x[k] - b[k]))
         n = A.shape[0]
-        assert backward_error(A, x[k], b[k]) <= BE_TOL, (k, backward_error(A, x[k], b[k]))
+        be, be_o = backward_error(A, x[k], b[k]), backward_error(A, xo, b[k])
+        if well_posed:
+            assert be_o <= BE_TOL, (k, be_o)
+            assert be <= BE_TOL, (k, be)
+            assert np.max(abs(x[k] - xo)) <= 1e-12 * max(1.0, o.cond) * np.max(abs(xo)) * n
+        else:
+            assert be <= max(BE_TOL, 10 * be_o), (k, be, be_o)
         assert res <= 10 * res_o + 1e-13 * n * np.max(abs(b[k])), (k, res, res_o)
-        assert np.max(abs(x[k] - xo)) <= 1e-12 * max(1.0, o.cond) * np.max(abs(xo)) * n
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
@@ -70,7 +81,8 @@ def test_batch_matches_oracle(dtype, r, rng):
 @pytest.mark.parametrize("kl,ku", [(1, 0), (2, 7), (3, 3), (5, 9), (7, 1), (10, 14), (14, 17), (16, 16)])
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
 def test_band_widths_cover_every_kernel(kl, ku, dtype, rng):
-    """Every compiled (kl, wu) bucket, with pivoting exercised (no diagonal dominance)."""
+    """Every compiled (kl, wu) bucket, with pivoting exercised (no diagonal dominance); the
+    reference's relative rule (tests/cross_backend/test_banded_level.py:71-76)."""
     from conatus_amd.runtime.hip import BandedBatch, to_device
     n, r, B = 150, 3, 5
     print(f"case kl={kl} ku={ku} dtype={np.dtype(dtype).name}", flush=True)
@@ -84,7 +96,23 @@ def test_band_widths_cover_every_kernel(kl, ku, dtype, rng):
     d = to_device(b)
     bb.solve(d)
     x = d.cpu().numpy()
-    _check_against_oracle(mats, list(range(r)), b, x, dtype)
+    _check_against_oracle(mats, list(range(r)), b, x, dtype, well_posed=False)
+
+
+@pytest.mark.parametrize("kl,ku", [(1, 0), (2, 7), (3, 3), (5, 9), (7, 1), (10, 14), (14, 17), (16, 16)])
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_band_widths_conditioned(kl, ku, dtype, rng):
+    """Every compiled (kl, wu) bucket on the 'conditioned' family (diagonal + 8): the 1e-13
+    backward-error bar."""
+    from conatus_amd.runtime.hip import BandedBatch, to_device
+    n, r, B = 150, 3, 5
+    mats = [conditioned(rng, n, kl, ku, r, shift=8.0 + kl + ku, dtype=complex if dtype == np.complex128 else float)
+            for _ in range(B)]
+    bb = BandedBatch(mats, list(range(r)))
+    b = _rhs(rng, (B, n), dtype)
+    d = to_device(b)
+    bb.solve(d)
+    _check_against_oracle(mats, list(range(r)), b, d.cpu().numpy(), dtype)
 
 
 def test_single_system_multiple_rhs(rng):

@@ -116,12 +116,18 @@ def test_sbdf_stage_rhs_is_bitwise_the_oracle(rng):
                                       v(rhs.data_ptr()), v(torch.cuda.current_stream().cuda_stream))
         _lib.check(st, "test")
         torch.cuda.synchronize()
-        MX = np.stack([Ms[k] @ x0[k] for k in range(B)])
-        F = g * x0 + h
+        MX = np.stack([Ms[k] @ x0[k] for k in range(B)])      # scipy: in-order, separately rounded
+        # F = g x + h with the separately rounded complex product of std::complex (numpy's own
+        # complex multiply may fuse)
+        if cplx:
+            F = (g.real * x0.real - g.imag * x0.imag) + 1j * (g.real * x0.imag + g.imag * x0.real)
+            F = (F.real + h.real) + 1j * (F.imag + h.imag)
+        else:
+            F = g * x0 + h
         levels_f = [F] + hist_f
         levels_mx = [MX] + hist_mx
         R = np.zeros((B, n), dtype)
-        for j in range(1, q + 1):
+        for j in range(1, q + 1):              # real scalars times values: no fusion possible
             R = R + (c[j] * levels_f[j - 1] - (a[j] / dt) * levels_mx[j - 1])
         assert np.array_equal(mx0.cpu().numpy(), MX)
         assert np.array_equal(f0.cpu().numpy(), F)

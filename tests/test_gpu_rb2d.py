@@ -44,23 +44,26 @@ def test_explicit_term_matches_oracle(Nx, Nz, rng):
     assert rel(F, Fo) < 1e-13, rel(F, Fo)
 
 
-@pytest.mark.parametrize("Nx,Nz,steps", [(32, 16, 50), (128, 64, 200)])
-def test_trajectory_matches_oracle(Nx, Nz, steps):
-    """Config-0 physics (Ra = 1e5, Pr = 1, walls at z = -1, 1, SBDF2, dt = 1e-3) from the seeded
-    conduction + noise state; (128, 64, 200) is the full configuration."""
+@pytest.mark.parametrize("Nx,Nz,dt,steps", [(32, 16, 1e-3, 50), (128, 64, 1e-3, 40), (128, 64, 1e-4, 200)])
+def test_trajectory_matches_oracle(Nx, Nz, dt, steps):
+    """Config-0 physics (Ra = 1e5, Pr = 1, walls at z = -1, 1, SBDF2) from the seeded conduction +
+    noise state.  At dt = 1e-3 the developed flow breaks the explicit-advection stability limit
+    after about 60 steps (the oracle blows up as well), so the full 128 x 64 grid is checked over
+    the 40 steps before saturation at dt = 1e-3 and over 200 steps at dt = 1e-4."""
     from conatus_amd.timestep import IVPSolver
     pb = RayleighBenard2D(Nx=Nx, Nz=Nz, Lx=4.0, Ra=1e5, Pr=1.0)
     X0 = pb.initial_state(seed=0)
-    print(f"case {Nx}x{Nz} {steps} steps", flush=True)
-    s = IVPSolver(pb, "SBDF2", 1e-3)
+    print(f"case {Nx}x{Nz} dt={dt} {steps} steps", flush=True)
+    s = IVPSolver(pb, "SBDF2", dt)
     s.set_state(X0)
     s.run(steps)
     Xg = s.state()
-    Xo = RB2DOracle(pb, "SBDF2", 1e-3).run(X0, steps)
+    Xo = RB2DOracle(pb, "SBDF2", dt).run(X0, steps)
+    assert np.all(np.isfinite(Xo))
     r = rel(Xg, Xo)
     print(f"relative difference after {steps} steps: {r:.3e}", flush=True)
     assert r < 1e-11
-    assert abs(s.t - steps * 1e-3) < 1e-12
+    assert abs(s.t - steps * dt) < 1e-12
 
 
 def test_graph_replay_equals_eager_steps():

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -129,15 +130,7 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
         }
         ip = __shfl(ip, src, 64);
         best = __shfl(best, src, 64);
-        if (ip != 0) {   // row interchange in every column of the window and in W
-#pragma unroll
-            for (int i = 1; i <= KL; ++i)
-                if (ip == i) {
-                    const T tmp = win[0];
-                    win[0] = win[i];
-                    win[i] = tmp;
-                }
-        }
+        if (ip != 0) cn::swap_first<T, KL>(win, ip);   // row interchange in every column of the window and in W
         const bool singular = !(best != 0.0);
         T rc = cn::recip(win[0]);
         if (singular) rc = cn::zero<T>();
@@ -178,59 +171,6 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
         if (singular && first_singular == 0) first_singular = j + 1;
     }
     if (live && c == 0) info[m] = first_singular;
-}
-
-// ------------------------------------------------------------------------------------------
-// Border kernel: lane = (system, border column k); back substitution W_k = U^-1 Wf_k in place and
-// column k of S = I + V^T W.
-// ------------------------------------------------------------------------------------------
-template <class T, int WU>
-__global__ void __launch_bounds__(256)
-k_border(const T* __restrict__ Uf, T* __restrict__ Wf, const T* __restrict__ Vt, T* __restrict__ S,
-         int B, int n, int r, int US) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = gid / r;
-    const int k = gid - m * r;
-    if (m >= B) return;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
-    T* W0 = Wf + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
-    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
-    T xw[WU];
-#pragma unroll
-    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    T s[MAXR];
-#pragma unroll
-    for (int a = 0; a < MAXR; ++a) s[a] = cn::zero<T>();
-    int j = n - 1;
-    while (j >= 0) {
-#pragma unroll
-        for (int u = 0; u <= WU; ++u) {
-            if (j < 0) break;
-            const T* Ur = U0 + (size_t)j * US * IL;
-            T acc = W0[(size_t)j * IL];
-#pragma unroll
-            for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
-            const T x = cn::mul(acc, Ur[0]);
-            W0[(size_t)j * IL] = x;
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) s[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, s[a]);
-#pragma unroll
-            for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
-            xw[0] = x;
-            --j;
-        }
-    }
-    T* Sm = S + (size_t)m * r * r;
-#pragma unroll
-    for (int a = 0; a < MAXR; ++a)
-        if (a < r) {
-            T v = s[a];
-            if (a == k) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);   // + 1 on the diagonal
-            Sm[(size_t)k * r + a] = v;                                // column-major S(a, k)
-        }
 }
 
 __device__ __forceinline__ double absv(double a) { return fabs(a); }
@@ -374,90 +314,20 @@ __device__ __forceinline__ void io_store<zc>(const SolveIO& io, size_t mn, int k
     x[io.cols2[mn + i]] = v.im;
 }
 
-template <class T, int KL, int WU>
-__global__ void __launch_bounds__(256)
-k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
-        const T* __restrict__ Vt, const T* __restrict__ Wm, const T* __restrict__ S,
-        const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
-        SolveIO io, int B, int n, int r, int US, int nrhs) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = gid / nrhs;
-    const int k = gid - m * nrhs;
-    if (m >= B) return;
-    const size_t NL = (size_t)B * nrhs;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
-    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
-    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
-    const size_t mn = (size_t)m * n;
-    T* w0 = ws + gid;
-
-    // forward: y = L^-1 P_B (P b), window of KL+1 rows
-    T y[KL + 1];
-#pragma unroll
-    for (int i = 0; i <= KL; ++i) y[i] = (i < n) ? io_load_b<T>(io, mn, k, perm[i]) : cn::zero<T>();
-    int j = 0;
-    while (j < n) {
-#pragma unroll
-        for (int u = 0; u <= KL; ++u) {
-            if (j >= n) break;
-            const int p = P0[(size_t)j * IL];
-            if (p != 0) {
-#pragma unroll
-                for (int i = 1; i <= KL; ++i)
-                    if (p == i) { const T tmp = y[0]; y[0] = y[i]; y[i] = tmp; }
-            }
-            const T* Lr = L0 + (size_t)j * KL * IL;
-#pragma unroll
-            for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(Lr[(i - 1) * IL], y[0], y[i]);
-            w0[(size_t)j * NL] = y[0];
-            const int jn = j + KL + 1;
-#pragma unroll
-            for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
-            y[KL] = (jn < n) ? io_load_b<T>(io, mn, k, perm[jn]) : cn::zero<T>();
-            ++j;
-        }
-    }
-    // backward: x = U^-1 y, window of the last WU solutions; t = V^T x on the way
-    T xw[WU];
-#pragma unroll
-    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    T t[MAXR];
-#pragma unroll
-    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
-    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
-    j = n - 1;
-    while (j >= 0) {
-#pragma unroll
-        for (int u = 0; u <= WU; ++u) {
-            if (j < 0) break;
-            const T* Ur = U0 + (size_t)j * US * IL;
-            T acc = w0[(size_t)j * NL];
-#pragma unroll
-            for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
-            const T x = cn::mul(acc, Ur[0]);
-            io_store<T>(io, mn, k, j, x);
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) t[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, t[a]);
-#pragma unroll
-            for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
-            xw[0] = x;
-            --j;
-        }
-    }
-    if (r == 0) return;
-    // z = S^-1 t (S LU-factored, column-major, pivots spiv)
-    const T* Sm = S + (size_t)m * r * r;
-    const int* sp = spiv + (size_t)m * r;
+// z = S^-1 t in registers (S LU-factored column-major with pivots sp; r <= MAXR)
+template <class T>
+__device__ __forceinline__ void cap_solve(T (&t)[MAXR], const T* __restrict__ Sm, const int* __restrict__ sp, int r) {
 #pragma unroll
     for (int i = 0; i < MAXR; ++i) {
         if (i >= r) break;
         const int pi = sp[i];
 #pragma unroll
-        for (int q = i + 1; q < MAXR; ++q)
-            if (q == pi) { const T tmp = t[i]; t[i] = t[q]; t[q] = tmp; }
+        for (int q = i + 1; q < MAXR; ++q) {
+            const bool c = (q == pi);
+            const T ti = t[i], tq = t[q];
+            t[i] = cn::sel(c, tq, ti);
+            t[q] = cn::sel(c, ti, tq);
+        }
     }
 #pragma unroll
     for (int i = 0; i < MAXR; ++i) {
@@ -474,14 +344,294 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
             t[i] = cn::mul(t[i], cn::recip(Sm[i * r + i]));
         }
     }
-    // x -= W z
-    const T* W0 = Wm + ch * ((size_t)r * n * IL) + ml;
-    for (int i = 0; i < n; ++i) {
-        T v = io_load_x<T>(io, mn, k, i);
+}
+
+// Direct solve (any shape and IO mode).  Woodbury without a correction pass:
+//   c = P b; t = Z^T c (Z^T = V^T B^-1, accumulated as c streams in); y_c = L^-1 P_B c;
+//   z = S^-1 t; x = U^-1 (y_c - Wf z) with Wf = L^-1 P_B U (the factor's forward half of W).
+// Mathematically x = B^-1 (c - U z) = (P A)^-1 c.
+template <class T, int KL, int WU>
+__global__ void __launch_bounds__(256)
+k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+        const T* __restrict__ Zt, const T* __restrict__ Wf, const T* __restrict__ S,
+        const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+        SolveIO io, int B, int n, int r, int US, int nrhs) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / nrhs;
+    const int k = gid - m * nrhs;
+    if (m >= B) return;
+    const size_t NL = (size_t)B * nrhs;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
+    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    const T* Z0 = Zt + ch * ((size_t)r * n * IL) + ml;
+    const T* W0 = Wf + ch * ((size_t)r * n * IL) + ml;
+    const size_t mn = (size_t)m * n;
+    T* w0 = ws + gid;
+
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    // forward: y = L^-1 P_B c, window of KL+1 rows; t += Z^T c as c enters
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) {
+        y[i] = cn::zero<T>();
+        if (i < n) {
+            y[i] = io_load_b<T>(io, mn, k, perm[i]);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + i) * IL], y[i], t[a]);
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        const int p = P0[(size_t)j * IL];
+        if (p != 0) cn::swap_first<T, KL>(y, p);
+        const T* Lr = L0 + (size_t)j * KL * IL;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(Lr[(i - 1) * IL], y[0], y[i]);
+        w0[(size_t)j * NL] = y[0];
+        const int jn = j + KL + 1;
+#pragma unroll
+        for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+        T c = cn::zero<T>();
+        if (jn < n) {
+            c = io_load_b<T>(io, mn, k, perm[jn]);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + jn) * IL], c, t[a]);
+        }
+        y[KL] = c;
+    }
+    if (r > 0) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    // backward: x = U^-1 (y_c - Wf z), window of the last WU solutions
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        const T* Ur = U0 + (size_t)j * US * IL;
+        T acc = w0[(size_t)j * NL];
 #pragma unroll
         for (int a = 0; a < MAXR; ++a)
-            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * IL], t[a], v);
-        io_store<T>(io, mn, k, i, v);
+            if (a < r) acc = cn::fnms(W0[((size_t)a * n + j) * IL], t[a], acc);
+#pragma unroll
+        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T x = cn::mul(acc, Ur[0]);
+        io_store<T>(io, mn, k, j, x);
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+        xw[0] = x;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Streamed solve (the hot kernel for many narrow-band systems): one wavefront per workgroup,
+// 64 / NR systems x NR right-hand sides (NR in {1, 2, 4}); the per-system factor streams
+// (L rows, pivots, Z^T for the forward sweep; U rows, Wf for the backward sweep) are copied in
+// blocks of SP steps from HBM straight into LDS with global_load_lds_dwordx4, double-buffered:
+// the copy of block b+1 is in flight while block b is computed from LDS.  The per-lane
+// right-hand side and forward results are prefetched one block ahead in registers.  Arithmetic
+// identical to k_solve.  IO: in place (mode 0) or contiguous out-of-place (mode 1, no maps).
+// ------------------------------------------------------------------------------------------
+constexpr int SP = 8;   // steps per streamed block
+
+__device__ __forceinline__ void dma16(const char* __restrict__ src, char* lds_dst, int nbytes, int lane) {
+    for (int off = 0; off < nbytes; off += 1024) {
+        const int my = off + lane * 16;
+        if (my < nbytes)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my),
+                                             (__attribute__((address_space(3))) void*)(lds_dst + off), 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+template <class T>
+__device__ __forceinline__ T lds_at(const char* p) { return *reinterpret_cast<const T*>(p); }
+
+// forward block blk of the streamed solve into LDS buffer buf: per chunk [L: SP x KL][Z: r x SP][piv]
+template <class T, int KL, int C, int CE>
+__device__ __forceinline__ void solve_fwd_dma(const T* __restrict__ Lf, const T* __restrict__ Zt,
+                                              const int* __restrict__ piv, char* lds, size_t ch0, int n, int r,
+                                              int blk, int buf, int bufbytes, int fchunk, int fL, int fZ, int lane) {
+    const int j0 = blk * SP;
+    const int cnt = min(SP, n - j0);
+    const int jz0 = j0 + KL + 1;
+    const int cz = max(0, min(SP, n - jz0));
+    char* base = lds + buf * bufbytes;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const size_t ch = ch0 + c;
+        char* cb = base + c * fchunk;
+        dma16(reinterpret_cast<const char*>(Lf + (ch * n * KL + (size_t)j0 * KL) * IL), cb, cnt * KL * CE, lane);
+        for (int a = 0; a < r && cz > 0; ++a)
+            dma16(reinterpret_cast<const char*>(Zt + (ch * r * n + (size_t)a * n + jz0) * IL), cb + fL + a * SP * CE,
+                  cz * CE, lane);
+        dma16(reinterpret_cast<const char*>(piv + (ch * n + j0) * IL), cb + fL + fZ, cnt * IL * 4, lane);
+    }
+}
+
+// backward block blk (rows n-1-blk*SP downwards) into LDS buffer buf: per chunk [U: SP x US][W: r x SP]
+template <class T, int C, int CE>
+__device__ __forceinline__ void solve_bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Wf, char* lds,
+                                              size_t ch0, int n, int r, int US, int blk, int buf, int bufbytes,
+                                              int bchunk, int bU, int lane) {
+    const int jhi = n - 1 - blk * SP;
+    const int jlo = max(0, jhi - SP + 1);
+    const int cnt = jhi - jlo + 1;
+    char* base = lds + buf * bufbytes;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const size_t ch = ch0 + c;
+        char* cb = base + c * bchunk;
+        dma16(reinterpret_cast<const char*>(Uf + (ch * n * US + (size_t)jlo * US) * IL), cb, cnt * US * CE, lane);
+        for (int a = 0; a < r; ++a)
+            dma16(reinterpret_cast<const char*>(Wf + (ch * r * n + (size_t)a * n + jlo) * IL), cb + bU + a * SP * CE,
+                  cnt * CE, lane);
+    }
+}
+
+template <class T, int KL, int WU, int NR>
+__global__ void __launch_bounds__(64)
+k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Zt, const T* __restrict__ Wf, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+          const T* __restrict__ bin, T* __restrict__ xout, int64_t rhs_stride, int B, int n, int r, int US,
+          int bufbytes) {
+    static_assert(KL + 1 <= SP, "the window lookahead must stay within the next block");
+    constexpr int NS = 64 / NR;                 // systems per workgroup
+    constexpr int C = NS / IL;                  // interleave chunks per workgroup
+    constexpr int CE = IL * (int)sizeof(T);     // bytes of one entry across a chunk
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int mloc = lane / NR, k = lane - mloc * NR;
+    const int m = blockIdx.x * NS + mloc;
+    const bool live = m < B;
+    const int cl = mloc / IL, ml = mloc % IL;
+    const size_t ch0 = (size_t)blockIdx.x * C;
+    const size_t NL = (size_t)B * NR;
+    const size_t gid = (size_t)m * NR + k;
+    const T* bl = bin + k * rhs_stride + (size_t)m * n;
+    T* xl = xout + k * rhs_stride + (size_t)m * n;
+    T* wl = ws + gid;
+    // forward layout of one buffer: per chunk [L: SP x KL][Z: r x SP][piv: SP x 16 ints]
+    const int fL = SP * KL * CE, fZ = r * SP * CE, fP = SP * IL * 4;
+    const int fchunk = fL + fZ + fP;
+    // backward layout: per chunk [U: SP x US][W: r x SP]
+    const int bU = SP * US * CE;
+    const int bchunk = bU + r * SP * CE;
+
+
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    const T* Z0 = Zt + ((size_t)(m / IL) * r * n) * IL + ml;
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) {
+        y[i] = cn::zero<T>();
+        if (live && i < n) {
+            y[i] = bl[perm[i]];
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + i) * IL], y[i], t[a]);
+        }
+    }
+    const int nblk = (n + SP - 1) / SP;
+    // per-lane incoming values of block b: c_{b SP + KL + 1 + s}
+    T cur[SP], nxt[SP];
+#pragma unroll
+    for (int s = 0; s < SP; ++s) {
+        const int jn = KL + 1 + s;
+        cur[s] = (live && jn < n) ? bl[perm[jn]] : cn::zero<T>();
+    }
+    solve_fwd_dma<T, KL, C, CE>(Lf, Zt, piv, lds, ch0, n, r, 0, 0, bufbytes, fchunk, fL, fZ, lane);
+    dma_wait();
+    for (int b = 0; b < nblk; ++b) {
+        const int buf = b & 1;
+        if (b + 1 < nblk) {
+            solve_fwd_dma<T, KL, C, CE>(Lf, Zt, piv, lds, ch0, n, r, b + 1, buf ^ 1, bufbytes, fchunk, fL, fZ, lane);
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                const int jn = (b + 1) * SP + KL + 1 + s;
+                nxt[s] = (live && jn < n) ? bl[perm[jn]] : cn::zero<T>();
+            }
+        }
+        const char* fb = lds + buf * bufbytes + cl * fchunk;
+#pragma unroll
+        for (int s = 0; s < SP; ++s) {
+            const int j = b * SP + s;
+            if (j >= n) break;
+            const int p = *reinterpret_cast<const int*>(fb + fL + fZ + (s * IL + ml) * 4);
+            if (p != 0) cn::swap_first<T, KL>(y, p);
+#pragma unroll
+            for (int i = 1; i <= KL; ++i)
+                y[i] = cn::fnms(lds_at<T>(fb + ((s * KL + i - 1) * IL + ml) * sizeof(T)), y[0], y[i]);
+            if (live) wl[(size_t)j * NL] = y[0];
+            const int jn = j + KL + 1;
+#pragma unroll
+            for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+            const T c = cur[s];
+            y[KL] = c;
+            if (jn < n) {
+#pragma unroll
+                for (int a = 0; a < MAXR; ++a)
+                    if (a < r) t[a] = cn::fmac(lds_at<T>(fb + fL + ((a * SP + s) * IL + ml) * sizeof(T)), c, t[a]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SP; ++s) cur[s] = nxt[s];
+        dma_wait();
+    }
+    if (r > 0 && live) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    // backward sweep, blocks of SP rows from the bottom
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    T wc[SP], wn[SP];
+#pragma unroll
+    for (int s = 0; s < SP; ++s) {
+        const int j = n - 1 - s;
+        wc[s] = (live && j >= 0) ? wl[(size_t)j * NL] : cn::zero<T>();
+    }
+    solve_bwd_dma<T, C, CE>(Uf, Wf, lds, ch0, n, r, US, 0, 0, bufbytes, bchunk, bU, lane);
+    dma_wait();
+    for (int b = 0; b < nblk; ++b) {
+        const int buf = b & 1;
+        const int jhi = n - 1 - b * SP;
+        const int jlo = max(0, jhi - SP + 1);
+        if (b + 1 < nblk) {
+            solve_bwd_dma<T, C, CE>(Uf, Wf, lds, ch0, n, r, US, b + 1, buf ^ 1, bufbytes, bchunk, bU, lane);
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                const int j = jhi - SP - s;
+                wn[s] = (live && j >= 0) ? wl[(size_t)j * NL] : cn::zero<T>();
+            }
+        }
+        const char* bb = lds + buf * bufbytes + cl * bchunk;
+#pragma unroll
+        for (int s = 0; s < SP; ++s) {
+            const int j = jhi - s;
+            if (j < jlo) break;
+            const int row = j - jlo;
+            T acc = wc[s];
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) acc = cn::fnms(lds_at<T>(bb + bU + ((a * SP + row) * IL + ml) * sizeof(T)), t[a], acc);
+            const char* ur = bb + (size_t)row * US * CE + ml * sizeof(T);
+#pragma unroll
+            for (int d = 1; d <= WU; ++d) acc = cn::fnms(lds_at<T>(ur + d * CE), xw[d - 1], acc);
+            const T x = cn::mul(acc, lds_at<T>(ur));
+            if (live) xl[j] = x;
+#pragma unroll
+            for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+            xw[0] = x;
+        }
+#pragma unroll
+        for (int s = 0; s < SP; ++s) wc[s] = wn[s];
+        dma_wait();
     }
 }
 
@@ -557,11 +707,7 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         for (int i = 1; i <= KL; ++i) v = cn::fnms(Lr[(i - 1) * IL], y[i], v);
         y[0] = v;
         const int p = P0[(size_t)j * IL];
-        if (p != 0) {
-#pragma unroll
-            for (int i = 1; i <= KL; ++i)
-                if (p == i) { const T tmp = y[0]; y[0] = y[i]; y[i] = tmp; }
-        }
+        if (p != 0) cn::swap_first<T, KL>(y, p);
     }
 #pragma unroll
     for (int i = 0; i <= KL; ++i)
@@ -579,8 +725,12 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             if (i >= r) break;
             const int pi = sp[i];
 #pragma unroll
-            for (int q = i + 1; q < MAXR; ++q)
-                if (q == pi) { const T tmp = z[i]; z[i] = z[q]; z[q] = tmp; }
+            for (int q = i + 1; q < MAXR; ++q) {
+                const bool c = (q == pi);
+                const T zi = z[i], zq = z[q];
+                z[i] = cn::sel(c, zq, zi);
+                z[q] = cn::sel(c, zi, zq);
+            }
         }
 #pragma unroll
         for (int i = 0; i < MAXR; ++i) {
@@ -609,14 +759,32 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     }
 }
 
-// St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
+// S = I + V^T B^-1 U (column-major r x r per system) from Z^T = V^T B^-1 (rows of Wt):
+// S(a, b) = delta_ab + Z^T[a][b].
 template <class T>
 __global__ void __launch_bounds__(256)
-k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int B, int n, int r) {
+k_build_s(const T* __restrict__ Zt, T* __restrict__ S, int B, int n, int r) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gid / (r * r);
     const int q = gid - m * r * r;
     if (m >= B) return;
+    const int b = q / r, a = q - b * r;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    T v = Zt[(ch * ((size_t)r * n) + (size_t)a * n + b) * IL + ml];
+    if (a == b) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);
+    S[(size_t)m * r * r + (size_t)b * r + a] = v;
+}
+
+// St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
+template <class T>
+__global__ void __launch_bounds__(256)
+k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int* __restrict__ info_t, int B, int n, int r) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / (r * r);
+    const int q = gid - m * r * r;
+    if (m >= B) return;
+    if (q == 0) info_t[m] = 0;
     const int b = q / r, a = q - b * r;
     const size_t ch = m / IL;
     const int ml = m % IL;
@@ -642,7 +810,6 @@ int pick_wu(int w) {
 
 using FactorFn = void (*)(dim3, dim3, hipStream_t, const void*, void*, void*, int*, void*, int*, int, int,
                           int, int, int, int, int);
-using BorderFn = void (*)(dim3, dim3, hipStream_t, const void*, void*, const void*, void*, int, int, int, int);
 using SolveFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, const int*, const void*, const void*,
                          const void*, const int*, const int*, void*, SolveIO, int, int, int, int, int);
 
@@ -651,12 +818,6 @@ void launch_factor(dim3 g, dim3 b, hipStream_t s, const void* Ab, void* Uf, void
                    int* info, int B, int n, int M, int r, int G, int US, int pivot) {
     hipLaunchKernelGGL((k_band_lu<T, KL>), g, b, 0, s, static_cast<const T*>(Ab), static_cast<T*>(Uf),
                        static_cast<T*>(Lf), piv, static_cast<T*>(Wf), info, B, n, M, r, G, US, pivot);
-}
-template <class T, int WU>
-void launch_border(dim3 g, dim3 b, hipStream_t s, const void* Uf, void* Wf, const void* Vt, void* S, int B,
-                   int n, int r, int US) {
-    hipLaunchKernelGGL((k_border<T, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<T*>(Wf),
-                       static_cast<const T*>(Vt), static_cast<T*>(S), B, n, r, US);
 }
 template <class T, int KL, int WU>
 void launch_solve(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf, const int* piv, const void* Vt,
@@ -722,17 +883,6 @@ FactorFn factor_fn(int KL) {
     }
     return nullptr;
 }
-template <class T>
-BorderFn border_fn(int WU) {
-    switch (WU) {
-        case 4: return &launch_border<T, 4>;
-        case 8: return &launch_border<T, 8>;
-        case 16: return &launch_border<T, 16>;
-        case 24: return &launch_border<T, 24>;
-        case 32: return &launch_border<T, 32>;
-    }
-    return nullptr;
-}
 template <class T, int KL>
 SolveFn solve_fn_kl(int WU) {
     if (WU < KL) return nullptr;
@@ -745,6 +895,49 @@ SolveFn solve_fn_kl(int WU) {
     }
     return nullptr;
 }
+using SolveSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, const int*, const void*,
+                          const void*, const void*, const int*, const int*, void*, const void*, void*, int64_t, int,
+                          int, int, int, int);
+
+template <class T, int KL, int WU, int NR>
+void launch_solve_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Lf, const int* piv,
+                    const void* Zt, const void* Wf, const void* S, const int* spiv, const int* perm, void* ws,
+                    const void* bin, void* xout, int64_t rs, int B, int n, int r, int US, int bufbytes) {
+    hipLaunchKernelGGL((k_solve_s<T, KL, WU, NR>), g, b, lds, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
+                       piv, static_cast<const T*>(Zt), static_cast<const T*>(Wf), static_cast<const T*>(S), spiv, perm,
+                       static_cast<T*>(ws), static_cast<const T*>(bin), static_cast<T*>(xout), rs, B, n, r, US,
+                       bufbytes);
+}
+
+template <class T, int KL, int WU>
+SolveSFn solve_s_nr(int nr) {
+    switch (nr) {
+        case 1: return &launch_solve_s<T, KL, WU, 1>;
+        case 2: return &launch_solve_s<T, KL, WU, 2>;
+        case 4: return &launch_solve_s<T, KL, WU, 4>;
+    }
+    return nullptr;
+}
+template <class T, int KL>
+SolveSFn solve_s_wu(int WU, int nr) {
+    switch (WU) {
+        case 4: if constexpr (KL <= 4) return solve_s_nr<T, KL, 4>(nr); else return nullptr;
+        case 8: return solve_s_nr<T, KL, 8>(nr);
+        case 16: return solve_s_nr<T, KL, 16>(nr);
+    }
+    return nullptr;
+}
+template <class T>
+SolveSFn solve_s_fn(int KL, int WU, int nr) {
+    switch (KL) {
+        case 1: return solve_s_wu<T, 1>(WU, nr);
+        case 2: return solve_s_wu<T, 2>(WU, nr);
+        case 4: return solve_s_wu<T, 4>(WU, nr);
+        case 6: return solve_s_wu<T, 6>(WU, nr);
+    }
+    return nullptr;
+}
+
 template <class T>
 SolveFn solve_fn(int KL, int WU) {
     switch (KL) {
@@ -784,6 +977,7 @@ void cn::set_error(const std::string& msg) { g_err = msg; }
 struct cn_bb {
     int dtype = CN_C128, B = 0, Bp = 0, n = 0, r = 0, kl = 0, ku = 0, KLP = 0, M = 0, WU = 0, US = 0, G = 0;
     int pivot = 1;
+    int streamed = 1;   // 0: always use the direct solve kernel (CNHIP_SOLVE=direct)
     int nrhs_cap = 0;
     size_t esz = 16;
     double cond_max = 1.0, factor_ms = 0.0;
@@ -832,19 +1026,37 @@ void run_factor(cn_bb& h, hipStream_t s) {
     const dim3 fb(256), fg((h.Bp + spb - 1) / spb);
     FactorFn f = factor_fn<T>(h.KLP);
     if (!f) fail(CN_ELOGIC, "no factor kernel for kl " + std::to_string(h.KLP));
+    // band LU, with the forward half Wf = L^-1 P_B U of W in the r extra lanes
     f(fg, fb, s, h.Ab.p, h.Uf.p, h.Lf.p, static_cast<int*>(h.piv.p), h.W.p, static_cast<int*>(h.info.p), h.B, h.n,
       h.M, h.r, h.G, h.US, h.pivot);
     hip_check(hipGetLastError(), "k_band_lu launch");
     if (h.r > 0) {
-        BorderFn bf = border_fn<T>(h.WU);
+        // Z^T = V^T B^-1 (rows B^-T v_a), used by the forward solve (t = Z^T c) and, as Wt, by the
+        // transposed solve; then S = I + Z^T U and St = I + U^T Wt, both LU-factored
+        SolveTFn ft = solve_t_fn<T>(h.KLP, h.WU);
+        if (!ft) fail(CN_ELOGIC, "no transposed solve kernel");
         const int lanes = h.B * h.r;
-        bf(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.W.p, h.Vt.p, h.S.p, h.B, h.n, h.r, h.US);
-        hip_check(hipGetLastError(), "k_border launch");
+        ft(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.Wt.p,
+           h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), nullptr, nullptr, 0, h.B, h.n,
+           h.r, h.US, h.r, 1);
+        hip_check(hipGetLastError(), "k_solve_t (Z^T) launch");
+        const int l2 = h.B * h.r * h.r;
+        hipLaunchKernelGGL((k_build_s<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
+                           static_cast<T*>(h.S.p), h.B, h.n, h.r);
+        hip_check(hipGetLastError(), "k_build_s launch");
         hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.S.p),
                            static_cast<int*>(h.spiv.p), static_cast<double*>(h.cond.p), static_cast<int*>(h.info.p),
                            h.B, h.r);
         hip_check(hipGetLastError(), "k_cap_lu launch");
+        hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
+                           static_cast<T*>(h.St.p), static_cast<int*>(h.info_t.p), h.B, h.n, h.r);
+        hip_check(hipGetLastError(), "k_build_st launch");
+        hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.St.p),
+                           static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
+                           static_cast<int*>(h.info_t.p), h.B, h.r);
+        hip_check(hipGetLastError(), "k_cap_lu (transposed) launch");
     }
+    h.tr_ready = true;
 }
 
 void factor_and_check(cn_bb& h, hipStream_t s, bool check) {
@@ -880,6 +1092,14 @@ void factor_and_check(cn_bb& h, hipStream_t s, bool check) {
         if (std::isnan(h.hcond[m]))
             fail(CN_ESINGULAR, "BandedBatch: capacitance condition number of mode " + std::to_string(m) + " is NaN");
         h.cond_max = std::max(h.cond_max, h.hcond[m]);
+    }
+    if (h.r > 0) {
+        std::vector<int> info_t(h.B);
+        hip_check(hipMemcpy(info_t.data(), h.info_t.p, sizeof(int) * h.B, hipMemcpyDeviceToHost), "hipMemcpy info_t");
+        for (int m = 0; m < h.B; ++m)
+            if (info_t[m] != 0)
+                fail(CN_ESINGULAR, "BandedBatch: transposed capacitance matrix of mode " + std::to_string(m) +
+                                       " is singular");
     }
 }
 
@@ -950,7 +1170,7 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     h.WU = pick_wu(h.KLP + ku);
     if (h.WU < 0) fail(CN_EUNSUPPORTED, "BandedBatch: kl+ku = " + std::to_string(h.KLP + ku) + " exceeds 32");
     h.US = h.WU + 1;
-    h.Bp = (B + IL - 1) / IL * IL;
+    h.Bp = (B + 63) / 64 * 64;   // whole 64-system workgroups of the streamed solve stay in bounds
     const size_t es = sizeof(T);
     const size_t nAb = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.M});
     const size_t nU = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.US});
@@ -971,6 +1191,14 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     h.cond.alloc((size_t)h.Bp * sizeof(double));
     h.info.alloc((size_t)h.Bp * sizeof(int));
     h.perm.alloc((size_t)n * sizeof(int));
+    h.Wt.alloc(std::max<size_t>(nV, 1) * es);
+    h.St.alloc(std::max<size_t>((size_t)h.Bp * r * r, 1) * es);
+    h.stpiv.alloc(std::max<size_t>((size_t)h.Bp * r, 1) * sizeof(int));
+    h.cond_t.alloc((size_t)h.Bp * sizeof(double));
+    h.info_t.alloc((size_t)h.Bp * sizeof(int));
+    hip_check(hipMemset(h.info_t.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info_t");
+    hip_check(hipMemset(h.W.p, 0, std::max<size_t>(nV, 1) * es), "hipMemset W");
+    hip_check(hipMemset(h.Wt.p, 0, std::max<size_t>(nV, 1) * es), "hipMemset Wt");
     hip_check(hipMemcpy(h.Ab.p, Ab.data(), nAb * es, hipMemcpyHostToDevice), "hipMemcpy Ab");
     if (nV) hip_check(hipMemcpy(h.Vt.p, Vt.data(), nV * es, hipMemcpyHostToDevice), "hipMemcpy Vt");
     hip_check(hipMemcpy(h.perm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy perm");
@@ -986,46 +1214,9 @@ void reserve_impl(cn_bb& h, int nrhs) {
     h.nrhs_cap = nrhs;
 }
 
-template <class T>
-void prepare_t_impl(cn_bb& h) {
-    const int B = h.B, n = h.n, r = h.r;
-    const size_t es = sizeof(T);
-    h.Wt.alloc(std::max<size_t>((size_t)h.Bp * r * n, 1) * es);
-    h.St.alloc(std::max<size_t>((size_t)h.Bp * r * r, 1) * es);
-    h.stpiv.alloc(std::max<size_t>((size_t)h.Bp * r, 1) * sizeof(int));
-    h.cond_t.alloc((size_t)h.Bp * sizeof(double));
-    h.info_t.alloc((size_t)h.Bp * sizeof(int));
-    hip_check(hipMemset(h.info_t.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info_t");
-    if (r > 0) {
-        SolveTFn f = solve_t_fn<T>(h.KLP, h.WU);
-        if (!f) fail(CN_ELOGIC, "no transposed solve kernel");
-        const int lanes = B * r;
-        f(dim3((lanes + 255) / 256), dim3(256), nullptr, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p,
-          h.Wt.p, h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), nullptr, nullptr, 0,
-          B, n, r, h.US, r, 1);
-        hip_check(hipGetLastError(), "k_solve_t (prepare) launch");
-        const int l2 = B * r * r;
-        hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, nullptr,
-                           static_cast<const T*>(h.Wt.p), static_cast<T*>(h.St.p), B, n, r);
-        hip_check(hipGetLastError(), "k_build_st launch");
-        hipLaunchKernelGGL((k_cap_lu<T>), dim3((B + 255) / 256), dim3(256), 0, nullptr, static_cast<T*>(h.St.p),
-                           static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
-                           static_cast<int*>(h.info_t.p), B, r);
-        hip_check(hipGetLastError(), "k_cap_lu (transposed) launch");
-        hip_check(hipDeviceSynchronize(), "prepare_transpose");
-        std::vector<int> info(B);
-        hip_check(hipMemcpy(info.data(), h.info_t.p, sizeof(int) * B, hipMemcpyDeviceToHost), "hipMemcpy info_t");
-        for (int m = 0; m < B; ++m)
-            if (info[m] != 0)
-                fail(CN_ESINGULAR, "BandedBatch: transposed capacitance matrix of mode " + std::to_string(m) +
-                                       " is singular");
-    }
-    h.tr_ready = true;
-}
-
 void prepare_t(cn_bb& h) {
-    if (h.tr_ready) return;
-    if (h.dtype == CN_C128) prepare_t_impl<zc>(h); else prepare_t_impl<double>(h);
+    // the transposed Woodbury data (Wt = Z^T, LU of St) are built with every factorization
+    if (!h.tr_ready) fail(CN_ELOGIC, "BandedBatch: factorization missing");
 }
 
 void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
@@ -1046,6 +1237,20 @@ void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
     hip_check(hipGetLastError(), "k_solve_t launch");
 }
 
+// LDS bytes of one buffer of the streamed solve (see k_solve_s), or -1 if it does not apply
+int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int nrhs) {
+    if (!(io.mode == 0 || (io.mode == 1 && !io.rows && !io.cols))) return -1;
+    if (!(nrhs == 1 || nrhs == 2 || nrhs == 4)) return -1;
+    if (h.KLP + 1 > SP || h.WU > 16) return -1;
+    const int C = (64 / nrhs) / IL;
+    const int CE = IL * (int)h.esz;
+    const int fchunk = SP * h.KLP * CE + h.r * SP * CE + SP * IL * 4;
+    const int bchunk = SP * h.US * CE + h.r * SP * CE;
+    const int buf = (std::max(C * fchunk, C * bchunk) + 15) / 16 * 16;
+    if (2 * buf > 64 * 1024) return -1;
+    return buf;
+}
+
 void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
     if (nrhs > h.nrhs_cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -1055,10 +1260,23 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
                                 " right-hand sides not reserved before graph capture (call cn_bb_reserve)");
         reserve_impl(h, nrhs);
     }
+    const int bufbytes = h.streamed ? streamed_bufbytes(h, io, nrhs) : -1;
+    if (bufbytes > 0) {
+        SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nrhs) : solve_s_fn<double>(h.KLP, h.WU, nrhs);
+        if (fs) {
+            const int NS = 64 / nrhs;
+            const void* bin = io.mode == 0 ? io.x : io.rhs;
+            fs(dim3((h.B + NS - 1) / NS), dim3(64), 2 * bufbytes, s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p),
+               h.Wt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, bin,
+               io.x, io.mode == 0 ? io.rhs_stride : 0, h.B, h.n, h.r, h.US, bufbytes);
+            hip_check(hipGetLastError(), "k_solve_s launch");
+            return;
+        }
+    }
     SolveFn f = (h.dtype == CN_C128) ? solve_fn<zc>(h.KLP, h.WU) : solve_fn<double>(h.KLP, h.WU);
     if (!f) fail(CN_ELOGIC, "no solve kernel for (kl, wu) = (" + std::to_string(h.KLP) + ", " + std::to_string(h.WU) + ")");
     const int lanes = h.B * nrhs;
-    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p,
+    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Wt.p, h.W.p,
       h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, io, h.B, h.n, h.r, h.US,
       nrhs);
     hip_check(hipGetLastError(), "k_solve launch");
@@ -1118,6 +1336,7 @@ int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n, const int32_t* const
         h->r = r;
         h->pivot = pivot ? 1 : 0;
         h->esz = dtype == CN_C128 ? 16 : 8;
+        if (const char* e = std::getenv("CNHIP_SOLVE")) h->streamed = std::string(e) != "direct";
         if (dtype == CN_C128) create_impl<std::complex<double>>(*h, indptr, indices, data, nnz, border);
         else create_impl<double>(*h, indptr, indices, data, nnz, border);
         factor_and_check(*h, nullptr, true);

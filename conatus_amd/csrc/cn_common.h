@@ -70,6 +70,24 @@ __device__ __forceinline__ zc recip(zc p) {
 __device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ zc shfl(zc v, int src) { return zc{__shfl(v.re, src, 64), __shfl(v.im, src, 64)}; }
 
+// select without branches or memory (keeps register arrays in registers)
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+__device__ __forceinline__ zc sel(bool c, zc a, zc b) { return zc{c ? a.re : b.re, c ? a.im : b.im}; }
+
+// swap v[0] and v[p] (p in 0..K) with per-slot selects: static register indices only
+template <class T, int K>
+__device__ __forceinline__ void swap_first(T (&v)[K + 1], int p) {
+    const T v0 = v[0];
+    T nv0 = v0;
+#pragma unroll
+    for (int i = 1; i <= K; ++i) {
+        const bool c = (p == i);
+        nv0 = sel(c, v[i], nv0);
+        v[i] = sel(c, v0, v[i]);
+    }
+    v[0] = nv0;
+}
+
 __device__ __forceinline__ bool is_zero(double a) { return a == 0.0; }
 __device__ __forceinline__ bool is_zero(zc a) { return a.re == 0.0 && a.im == 0.0; }
 

@@ -19,13 +19,16 @@
 //    so the forward half of W = B^-1 U costs nothing extra.  Row j of U leaves the window as one
 //    coalesced row store; the incoming band row is one coalesced row load.  U's diagonal is
 //    stored as its reciprocal so that solves multiply instead of divide.
-//  * Border: one lane per (system, border column) finishes W by back substitution and
-//    accumulates the column of S = I + V^T W; one lane per system LU-factors S (partial pivoting)
-//    and forms cond_1(S) from the explicit inverse, as banded.hpp:403-432.
-//  * Solve: one lane per (system, right-hand side); forward sweep (pivots + L) and backward sweep
-//    (U, window of the last wu solutions in registers), V^T y accumulated during the backward
-//    sweep, z = S^-1 V^T y, then x -= W z.  Gather/scatter maps and the split complex mode of
-//    the reference's device solver are folded into the loads and stores.
+//  * Border: one lane per (system, border column) finishes W by back substitution (in place over
+//    its forward half) and accumulates the column of S = I + V^T W; one lane per system LU-factors
+//    S (partial pivoting) and forms cond_1(S) from the explicit inverse, as banded.hpp:403-432.
+//  * Solve: one lane per (system, right-hand side); forward sweep (pivots + L), backward sweep
+//    (U, window of the last wu solutions in registers) with V^T y accumulated as y is produced,
+//    z = S^-1 V^T y, then x = y - W z.  For narrow bands the sweeps are streamed: the factor rows
+//    of the next block of steps are copied HBM -> LDS (global_load_lds) while the current block
+//    is computed.  Gather/scatter maps and the split complex mode of the reference's device
+//    solver are folded into the loads and stores of the direct kernel.
+//  * Transposed solves (the adjoint path) use Wt = B^-T V and St = I + U^T Wt, built on demand.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,6 +52,7 @@ namespace {
 
 constexpr int IL = 16;     // systems per interleave chunk
 constexpr int MAXR = 8;    // largest border rank handled by the kernels
+constexpr int LDS_CAP = 64 * 1024;    // LDS of one streamed workgroup (two buffers)
 
 thread_local std::string g_err;
 
@@ -346,14 +350,15 @@ __device__ __forceinline__ void cap_solve(T (&t)[MAXR], const T* __restrict__ Sm
     }
 }
 
-// Direct solve (any shape and IO mode).  Woodbury without a correction pass:
-//   c = P b; t = Z^T c (Z^T = V^T B^-1, accumulated as c streams in); y_c = L^-1 P_B c;
-//   z = S^-1 t; x = U^-1 (y_c - Wf z) with Wf = L^-1 P_B U (the factor's forward half of W).
-// Mathematically x = B^-1 (c - U z) = (P A)^-1 c.
+// Direct solve (any shape and IO mode), the Woodbury order of banded.hpp:312-341:
+//   c = P b;  y = U^-1 L^-1 P_B c (forward sweep, then backward sweep);  t = V^T y accumulated
+//   as the backward sweep produces y;  z = S^-1 t;  x = y - W z (a third pass over x).
+// z is formed from the y that x is built from, so the residual of x is that of the two band
+// sweeps and of W alone (no cond(B) amplification through the border rows).
 template <class T, int KL, int WU>
 __global__ void __launch_bounds__(256)
 k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
-        const T* __restrict__ Zt, const T* __restrict__ Wf, const T* __restrict__ S,
+        const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
         const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
         SolveIO io, int B, int n, int r, int US, int nrhs) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,26 +371,15 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
     const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
     const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
     const int* P0 = piv + ch * ((size_t)n * IL) + ml;
-    const T* Z0 = Zt + ch * ((size_t)r * n * IL) + ml;
-    const T* W0 = Wf + ch * ((size_t)r * n * IL) + ml;
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    const T* W0 = W + ch * ((size_t)r * n * IL) + ml;
     const size_t mn = (size_t)m * n;
     T* w0 = ws + gid;
 
-    T t[MAXR];
-#pragma unroll
-    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
-    // forward: y = L^-1 P_B c, window of KL+1 rows; t += Z^T c as c enters
+    // forward: y = L^-1 P_B c, window of KL+1 rows
     T y[KL + 1];
 #pragma unroll
-    for (int i = 0; i <= KL; ++i) {
-        y[i] = cn::zero<T>();
-        if (i < n) {
-            y[i] = io_load_b<T>(io, mn, k, perm[i]);
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + i) * IL], y[i], t[a]);
-        }
-    }
+    for (int i = 0; i <= KL; ++i) y[i] = (i < n) ? io_load_b<T>(io, mn, k, perm[i]) : cn::zero<T>();
     for (int j = 0; j < n; ++j) {
         const int p = P0[(size_t)j * IL];
         if (p != 0) cn::swap_first<T, KL>(y, p);
@@ -396,17 +390,12 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
         const int jn = j + KL + 1;
 #pragma unroll
         for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
-        T c = cn::zero<T>();
-        if (jn < n) {
-            c = io_load_b<T>(io, mn, k, perm[jn]);
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + jn) * IL], c, t[a]);
-        }
-        y[KL] = c;
+        y[KL] = (jn < n) ? io_load_b<T>(io, mn, k, perm[jn]) : cn::zero<T>();
     }
-    if (r > 0) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
-    // backward: x = U^-1 (y_c - Wf z), window of the last WU solutions
+    // backward: y = U^-1 y_c, window of the last WU solutions; t = V^T y
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
@@ -414,35 +403,91 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
         const T* Ur = U0 + (size_t)j * US * IL;
         T acc = w0[(size_t)j * NL];
 #pragma unroll
-        for (int a = 0; a < MAXR; ++a)
-            if (a < r) acc = cn::fnms(W0[((size_t)a * n + j) * IL], t[a], acc);
-#pragma unroll
         for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
         const T x = cn::mul(acc, Ur[0]);
         io_store<T>(io, mn, k, j, x);
 #pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) t[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, t[a]);
+#pragma unroll
         for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
         xw[0] = x;
     }
+    if (r == 0) return;
+    cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    // x = y - W z
+    for (int j = 0; j < n; ++j) {
+        T v = io_load_x<T>(io, mn, k, j);
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + j) * IL], t[a], v);
+        io_store<T>(io, mn, k, j, v);
+    }
+}
+
+// Direct completion of W = B^-1 U from its forward half (in place) and S = I + V^T W:
+// one lane per (system, border column a), back substitution with U.
+template <class T, int WU>
+__global__ void __launch_bounds__(256)
+k_border(const T* __restrict__ Uf, const T* __restrict__ Vt, T* __restrict__ W, T* __restrict__ S, int B,
+         int n, int r, int US) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / r;
+    const int a = gid - m * r;
+    if (m >= B) return;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    T* W0 = W + ch * ((size_t)r * n * IL) + ml + (size_t)a * n * IL;
+    T t[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        const T* Ur = U0 + (size_t)j * US * IL;
+        T acc = W0[(size_t)j * IL];
+#pragma unroll
+        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T x = cn::mul(acc, Ur[0]);
+        W0[(size_t)j * IL] = x;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q < r) t[q] = cn::fmac(V0[((size_t)q * n + j) * IL], x, t[q]);
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+        xw[0] = x;
+    }
+    // column a of S (column-major): S(q, a) = delta_qa + (V^T W)(q, a)
+    T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q)
+        if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
 }
 
 // ------------------------------------------------------------------------------------------
-// Streamed solve (the hot kernel for many narrow-band systems): one wavefront per workgroup,
-// 64 / NR systems x NR right-hand sides (NR in {1, 2, 4}); the per-system factor streams
-// (L rows, pivots, Z^T for the forward sweep; U rows, Wf for the backward sweep) are copied in
-// blocks of SP steps from HBM straight into LDS with global_load_lds_dwordx4, double-buffered:
-// the copy of block b+1 is in flight while block b is computed from LDS.  The per-lane
-// right-hand side and forward results are prefetched one block ahead in registers.  Arithmetic
-// identical to k_solve.  IO: in place (mode 0) or contiguous out-of-place (mode 1, no maps).
+// Streamed sweeps (the hot kernels for many narrow-band systems).  A workgroup is one wavefront
+// and one interleave chunk of 16 systems, with NR lanes per system (NR in {1, 2, 4} right-hand
+// sides or border columns).  Every sweep runs in blocks of SP steps: the factor rows of block
+// b+1 (chunk-contiguous, copied with global_load_lds_dwordx4) and the per-lane operands of block
+// b+1 (gathered by the same instruction, one slot per lane) travel from HBM into the other half
+// of a double-buffered LDS area while block b is computed from LDS; the results of block b are
+// held in registers and stored at the start of block b+1, so that the one wait per block
+// (s_waitcnt vmcnt(0)) covers memory operations issued a whole block earlier.
 // ------------------------------------------------------------------------------------------
 constexpr int SP = 8;   // steps per streamed block
 
-__device__ __forceinline__ void dma16(const char* __restrict__ src, char* lds_dst, int nbytes, int lane) {
+using lds_ptr = __attribute__((address_space(3))) void*;
+
+// contiguous copy of nbytes (multiple of 16) from global memory into LDS, 1 KiB per instruction
+__device__ __forceinline__ void dma16(const void* src_, char* lds_dst, int nbytes, int lane) {
+    const char* src = static_cast<const char*>(src_);
     for (int off = 0; off < nbytes; off += 1024) {
         const int my = off + lane * 16;
         if (my < nbytes)
-            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my),
-                                             (__attribute__((address_space(3))) void*)(lds_dst + off), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my), (lds_ptr)(lds_dst + off), 16, 0, 0);
     }
 }
 
@@ -451,188 +496,268 @@ __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" :
 template <class T>
 __device__ __forceinline__ T lds_at(const char* p) { return *reinterpret_cast<const T*>(p); }
 
-// forward block blk of the streamed solve into LDS buffer buf: per chunk [L: SP x KL][Z: r x SP][piv]
-template <class T, int KL, int C, int CE>
-__device__ __forceinline__ void solve_fwd_dma(const T* __restrict__ Lf, const T* __restrict__ Zt,
-                                              const int* __restrict__ piv, char* lds, size_t ch0, int n, int r,
-                                              int blk, int buf, int bufbytes, int fchunk, int fL, int fZ, int lane) {
-    const int j0 = blk * SP;
-    const int cnt = min(SP, n - j0);
-    const int jz0 = j0 + KL + 1;
-    const int cz = max(0, min(SP, n - jz0));
-    char* base = lds + buf * bufbytes;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const size_t ch = ch0 + c;
-        char* cb = base + c * fchunk;
-        dma16(reinterpret_cast<const char*>(Lf + (ch * n * KL + (size_t)j0 * KL) * IL), cb, cnt * KL * CE, lane);
-        for (int a = 0; a < r && cz > 0; ++a)
-            dma16(reinterpret_cast<const char*>(Zt + (ch * r * n + (size_t)a * n + jz0) * IL), cb + fL + a * SP * CE,
-                  cz * CE, lane);
-        dma16(reinterpret_cast<const char*>(piv + (ch * n + j0) * IL), cb + fL + fZ, cnt * IL * 4, lane);
+// One value per lane gathered from an arbitrary global address into LDS slot `slot` (lane l's
+// value at slot + l * sizeof(T); a double travels as two dwords into two 256-byte halves).
+template <class T>
+struct LaneSlot;
+template <>
+struct LaneSlot<zc> {
+    static constexpr int BYTES = 64 * 16;
+    __device__ static void load(const zc* src, char* slot) {
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src), (lds_ptr)slot, 16, 0, 0);
     }
-}
+    __device__ static zc read(const char* slot, int lane) { return lds_at<zc>(slot + lane * 16); }
+};
+template <>
+struct LaneSlot<double> {
+    static constexpr int BYTES = 64 * 8;
+    __device__ static void load(const double* src, char* slot) {
+        const char* s = reinterpret_cast<const char*>(src);
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(s), (lds_ptr)slot, 4, 0, 0);
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(s + 4), (lds_ptr)(slot + 256), 4, 0, 0);
+    }
+    __device__ static double read(const char* slot, int lane) {
+        const unsigned lo = lds_at<unsigned>(slot + lane * 4);
+        const unsigned hi = lds_at<unsigned>(slot + 256 + lane * 4);
+        return __hiloint2double((int)hi, (int)lo);
+    }
+};
 
-// backward block blk (rows n-1-blk*SP downwards) into LDS buffer buf: per chunk [U: SP x US][W: r x SP]
-template <class T, int C, int CE>
-__device__ __forceinline__ void solve_bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Wf, char* lds,
-                                              size_t ch0, int n, int r, int US, int blk, int buf, int bufbytes,
-                                              int bchunk, int bU, int lane) {
+// LDS bytes of one buffer of each streamed sweep
+__host__ __device__ constexpr int fwd_bytes(int KL, int esz) { return SP * KL * IL * esz + SP * IL * 4 + SP * 64 * esz; }
+__host__ __device__ constexpr int bwd_bytes(int US, int r, int esz) { return (SP * US + r * SP) * IL * esz + SP * 64 * esz; }
+__host__ __device__ constexpr int cor_bytes(int r, int esz) { return r * SP * IL * esz + SP * 64 * esz; }
+
+// backward block blk (rows jhi = n-1-blk*SP down to jlo) into buf:
+// [U: SP rows x US][V^T: r x SP][in: SP lane slots, slot s = row jhi - s]
+template <class T>
+__device__ __forceinline__ void bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
+                                        int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
+    constexpr int CE = IL * (int)sizeof(T);
     const int jhi = n - 1 - blk * SP;
     const int jlo = max(0, jhi - SP + 1);
     const int cnt = jhi - jlo + 1;
-    char* base = lds + buf * bufbytes;
+    const int bU = SP * US * CE, bV = r * SP * CE;
+    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
+    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SP * CE, cnt * CE, lane);
+    if (live) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const size_t ch = ch0 + c;
-        char* cb = base + c * bchunk;
-        dma16(reinterpret_cast<const char*>(Uf + (ch * n * US + (size_t)jlo * US) * IL), cb, cnt * US * CE, lane);
-        for (int a = 0; a < r; ++a)
-            dma16(reinterpret_cast<const char*>(Wf + (ch * r * n + (size_t)a * n + jlo) * IL), cb + bU + a * SP * CE,
-                  cnt * CE, lane);
+        for (int s = 0; s < SP; ++s)
+            if (jhi - s >= jlo) LaneSlot<T>::load(in + (size_t)(jhi - s) * in_st, buf + bU + bV + s * LaneSlot<T>::BYTES);
     }
 }
 
-template <class T, int KL, int WU, int NR>
-__global__ void __launch_bounds__(64)
-k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
-          const T* __restrict__ Zt, const T* __restrict__ Wf, const T* __restrict__ S,
-          const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
-          const T* __restrict__ bin, T* __restrict__ xout, int64_t rhs_stride, int B, int n, int r, int US,
-          int bufbytes) {
-    static_assert(KL + 1 <= SP, "the window lookahead must stay within the next block");
-    constexpr int NS = 64 / NR;                 // systems per workgroup
-    constexpr int C = NS / IL;                  // interleave chunks per workgroup
-    constexpr int CE = IL * (int)sizeof(T);     // bytes of one entry across a chunk
-    extern __shared__ __align__(16) char lds[];
-    const int lane = threadIdx.x;
-    const int mloc = lane / NR, k = lane - mloc * NR;
-    const int m = blockIdx.x * NS + mloc;
-    const bool live = m < B;
-    const int cl = mloc / IL, ml = mloc % IL;
-    const size_t ch0 = (size_t)blockIdx.x * C;
-    const size_t NL = (size_t)B * NR;
-    const size_t gid = (size_t)m * NR + k;
-    const T* bl = bin + k * rhs_stride + (size_t)m * n;
-    T* xl = xout + k * rhs_stride + (size_t)m * n;
-    T* wl = ws + gid;
-    // forward layout of one buffer: per chunk [L: SP x KL][Z: r x SP][piv: SP x 16 ints]
-    const int fL = SP * KL * CE, fZ = r * SP * CE, fP = SP * IL * 4;
-    const int fchunk = fL + fZ + fP;
-    // backward layout: per chunk [U: SP x US][W: r x SP]
-    const int bU = SP * US * CE;
-    const int bchunk = bU + r * SP * CE;
-
-
-    T t[MAXR];
-#pragma unroll
-    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
-    const T* Z0 = Zt + ((size_t)(m / IL) * r * n) * IL + ml;
-    T y[KL + 1];
-#pragma unroll
-    for (int i = 0; i <= KL; ++i) {
-        y[i] = cn::zero<T>();
-        if (live && i < n) {
-            y[i] = bl[perm[i]];
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) t[a] = cn::fmac(Z0[((size_t)a * n + i) * IL], y[i], t[a]);
-        }
-    }
+// Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
+// t[a] += V^T(a, j) out_j.  `in` and `out` may be the same array (in place).
+template <class T, int WU>
+__device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int bufbytes,
+                                          size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
+                                          size_t out_st, bool live, int lane, int mlx, T (&t)[MAXR]) {
+    constexpr int CE = IL * (int)sizeof(T);
+    const int bU = SP * US * CE, bV = r * SP * CE;
     const int nblk = (n + SP - 1) / SP;
-    // per-lane incoming values of block b: c_{b SP + KL + 1 + s}
-    T cur[SP], nxt[SP];
-#pragma unroll
-    for (int s = 0; s < SP; ++s) {
-        const int jn = KL + 1 + s;
-        cur[s] = (live && jn < n) ? bl[perm[jn]] : cn::zero<T>();
-    }
-    solve_fwd_dma<T, KL, C, CE>(Lf, Zt, piv, lds, ch0, n, r, 0, 0, bufbytes, fchunk, fL, fZ, lane);
-    dma_wait();
-    for (int b = 0; b < nblk; ++b) {
-        const int buf = b & 1;
-        if (b + 1 < nblk) {
-            solve_fwd_dma<T, KL, C, CE>(Lf, Zt, piv, lds, ch0, n, r, b + 1, buf ^ 1, bufbytes, fchunk, fL, fZ, lane);
-#pragma unroll
-            for (int s = 0; s < SP; ++s) {
-                const int jn = (b + 1) * SP + KL + 1 + s;
-                nxt[s] = (live && jn < n) ? bl[perm[jn]] : cn::zero<T>();
-            }
-        }
-        const char* fb = lds + buf * bufbytes + cl * fchunk;
-#pragma unroll
-        for (int s = 0; s < SP; ++s) {
-            const int j = b * SP + s;
-            if (j >= n) break;
-            const int p = *reinterpret_cast<const int*>(fb + fL + fZ + (s * IL + ml) * 4);
-            if (p != 0) cn::swap_first<T, KL>(y, p);
-#pragma unroll
-            for (int i = 1; i <= KL; ++i)
-                y[i] = cn::fnms(lds_at<T>(fb + ((s * KL + i - 1) * IL + ml) * sizeof(T)), y[0], y[i]);
-            if (live) wl[(size_t)j * NL] = y[0];
-            const int jn = j + KL + 1;
-#pragma unroll
-            for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
-            const T c = cur[s];
-            y[KL] = c;
-            if (jn < n) {
-#pragma unroll
-                for (int a = 0; a < MAXR; ++a)
-                    if (a < r) t[a] = cn::fmac(lds_at<T>(fb + fL + ((a * SP + s) * IL + ml) * sizeof(T)), c, t[a]);
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < SP; ++s) cur[s] = nxt[s];
-        dma_wait();
-    }
-    if (r > 0 && live) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
-    // backward sweep, blocks of SP rows from the bottom
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    T wc[SP], wn[SP];
-#pragma unroll
-    for (int s = 0; s < SP; ++s) {
-        const int j = n - 1 - s;
-        wc[s] = (live && j >= 0) ? wl[(size_t)j * NL] : cn::zero<T>();
-    }
-    solve_bwd_dma<T, C, CE>(Uf, Wf, lds, ch0, n, r, US, 0, 0, bufbytes, bchunk, bU, lane);
+    T o[SP];
+    bwd_dma<T>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
     dma_wait();
     for (int b = 0; b < nblk; ++b) {
         const int buf = b & 1;
         const int jhi = n - 1 - b * SP;
         const int jlo = max(0, jhi - SP + 1);
-        if (b + 1 < nblk) {
-            solve_bwd_dma<T, C, CE>(Uf, Wf, lds, ch0, n, r, US, b + 1, buf ^ 1, bufbytes, bchunk, bU, lane);
+        if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SP)
 #pragma unroll
-            for (int s = 0; s < SP; ++s) {
-                const int j = jhi - SP - s;
-                wn[s] = (live && j >= 0) ? wl[(size_t)j * NL] : cn::zero<T>();
-            }
+            for (int s = 0; s < SP; ++s) out[(size_t)(jhi + SP - s) * out_st] = o[s];
         }
-        const char* bb = lds + buf * bufbytes + cl * bchunk;
+        if (b + 1 < nblk) bwd_dma<T>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
+        const char* bb = lds + buf * bufbytes;
 #pragma unroll
         for (int s = 0; s < SP; ++s) {
             const int j = jhi - s;
             if (j < jlo) break;
             const int row = j - jlo;
-            T acc = wc[s];
-#pragma unroll
-            for (int a = 0; a < MAXR; ++a)
-                if (a < r) acc = cn::fnms(lds_at<T>(bb + bU + ((a * SP + row) * IL + ml) * sizeof(T)), t[a], acc);
-            const char* ur = bb + (size_t)row * US * CE + ml * sizeof(T);
+            T acc = LaneSlot<T>::read(bb + bU + bV + s * LaneSlot<T>::BYTES, lane);
+            const char* ur = bb + row * US * CE + mlx * (int)sizeof(T);
 #pragma unroll
             for (int d = 1; d <= WU; ++d) acc = cn::fnms(lds_at<T>(ur + d * CE), xw[d - 1], acc);
             const T x = cn::mul(acc, lds_at<T>(ur));
-            if (live) xl[j] = x;
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(lds_at<T>(bb + bU + (a * SP + row) * CE + mlx * (int)sizeof(T)), x, t[a]);
+            o[s] = x;
 #pragma unroll
             for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
             xw[0] = x;
         }
-#pragma unroll
-        for (int s = 0; s < SP; ++s) wc[s] = wn[s];
         dma_wait();
     }
+    if (live) {   // the last block holds rows (nblk-1)*SP.. counted from the bottom: jhi = n-1-(nblk-1)*SP
+        const int jhi = n - 1 - (nblk - 1) * SP;
+#pragma unroll
+        for (int s = 0; s < SP; ++s)
+            if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = o[s];
+    }
+    dma_wait();
+}
+
+// Streamed solve: forward sweep (pivots, L) into the work space, backward sweep (U) into x with
+// t = V^T y, z = S^-1 t, then the streamed pass x = y - W z.  Arithmetic as k_solve.
+// IO: in place (bin == xout, right-hand side k at k * rhs_stride) or contiguous out of place.
+template <class T, int KL, int WU, int NR>
+__global__ void __launch_bounds__(64)
+k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+          const T* bin, T* xout, int64_t rhs_stride, int B, int n, int r, int US, int bufbytes) {
+    static_assert(KL + 1 <= SP, "the window lookahead must stay within the next block");
+    constexpr int CE = IL * (int)sizeof(T);
+    constexpr int GB = LaneSlot<T>::BYTES;
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int mloc = lane / NR, k = lane - mloc * NR;
+    const int mlx = mloc & (IL - 1);            // LDS column of this lane's system (dead lanes alias a live one)
+    const size_t ch = blockIdx.x;
+    const int m = (int)ch * IL + mloc;
+    const bool live = mloc < IL && m < B;
+    const size_t NL = (size_t)B * NR;
+    const T* bl = bin + k * rhs_stride + (size_t)m * n;
+    T* xl = xout + k * rhs_stride + (size_t)m * n;
+    T* wl = ws + (size_t)m * NR + k;
+    const int fL = SP * KL * CE, fP = SP * IL * 4;
+    const int nblk = (n + SP - 1) / SP;
+
+    // ---- forward sweep: y_c = L^-1 P_B P b into the work space
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
+    T o[SP];
+    for (int b = -1; b < nblk; ++b) {
+        const int buf = b & 1;
+        if (b >= 1 && live) {   // results of block b-1
+#pragma unroll
+            for (int s = 0; s < SP; ++s) wl[(size_t)((b - 1) * SP + s) * NL] = o[s];
+        }
+        if (b + 1 < nblk) {     // block b+1: [L: SP x KL][piv: SP x 16][c: SP lane slots]
+            char* nb = lds + ((b + 1) & 1) * bufbytes;
+            const int j0 = (b + 1) * SP;
+            const int cnt = min(SP, n - j0);
+            dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
+            dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                const int jn = j0 + KL + 1 + s;
+                if (jn < n) {
+                    const int pj = perm[jn];
+                    if (live) LaneSlot<T>::load(bl + pj, nb + fL + fP + s * GB);
+                }
+            }
+        }
+        if (b >= 0) {
+            const char* fb = lds + buf * bufbytes;
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                const int j = b * SP + s;
+                if (j >= n) break;
+                const int p = lds_at<int>(fb + fL + (s * IL + mlx) * 4);
+                if (__ballot(p != 0)) cn::swap_first<T, KL>(y, p);
+#pragma unroll
+                for (int i = 1; i <= KL; ++i)
+                    y[i] = cn::fnms(lds_at<T>(fb + (s * KL + i - 1) * CE + mlx * (int)sizeof(T)), y[0], y[i]);
+                o[s] = y[0];
+#pragma unroll
+                for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+                y[KL] = (j + KL + 1 < n) ? LaneSlot<T>::read(fb + fL + fP + s * GB, lane) : cn::zero<T>();
+            }
+        }
+        dma_wait();
+    }
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < SP; ++s) {
+            const int j = (nblk - 1) * SP + s;
+            if (j < n) wl[(size_t)j * NL] = o[s];
+        }
+    }
+    dma_wait();
+
+    // ---- backward sweep: y = U^-1 y_c into x, t = V^T y
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    bwd_sweep<T, WU>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, xl, 1, live, lane, mlx, t);
+    if (r == 0) return;
+
+    // ---- z = S^-1 t, then x = y - W z: block [W: r x SP][y: SP lane slots]
+    if (live) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    const int cW = r * SP * CE;
+    for (int b = -1; b < nblk; ++b) {
+        const int buf = b & 1;
+        if (b >= 1 && live) {
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                const int j = (b - 1) * SP + s;
+                xl[j] = o[s];
+            }
+        }
+        if (b + 1 < nblk) {
+            char* nb = lds + ((b + 1) & 1) * bufbytes;
+            const int j0 = (b + 1) * SP;
+            const int cnt = min(SP, n - j0);
+            for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SP * CE, cnt * CE, lane);
+            if (live) {
+#pragma unroll
+                for (int s = 0; s < SP; ++s)
+                    if (j0 + s < n) LaneSlot<T>::load(xl + j0 + s, nb + cW + s * GB);
+            }
+        }
+        if (b >= 0) {
+            const char* cb = lds + buf * bufbytes;
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                if (b * SP + s >= n) break;
+                T v = LaneSlot<T>::read(cb + cW + s * GB, lane);
+#pragma unroll
+                for (int a = 0; a < MAXR; ++a)
+                    if (a < r) v = cn::fnms(lds_at<T>(cb + (a * SP + s) * CE + mlx * (int)sizeof(T)), t[a], v);
+                o[s] = v;
+            }
+        }
+        dma_wait();
+    }
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < SP; ++s) {
+            const int j = (nblk - 1) * SP + s;
+            if (j < n) xl[j] = o[s];
+        }
+    }
+}
+
+// Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
+// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s.
+template <class T, int WU, int NR>
+__global__ void __launch_bounds__(64)
+k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
+           int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int mloc = lane / NR, k = lane - mloc * NR;
+    const int mlx = mloc & (IL - 1);
+    const size_t ch = blockIdx.x;
+    const int m = (int)ch * IL + mloc;
+    const int a = blockIdx.y * NR + k;
+    const bool live = mloc < IL && m < B && a < r;
+    T* wa = W + (ch * r * n + (size_t)(live ? a : 0) * n) * IL + mlx;
+    T t[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
+    bwd_sweep<T, WU>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, lane, mlx, t);
+    if (!live) return;
+    T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q)
+        if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -757,23 +882,6 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * IL], z[a], v);
         xo[perm[i]] = v;
     }
-}
-
-// S = I + V^T B^-1 U (column-major r x r per system) from Z^T = V^T B^-1 (rows of Wt):
-// S(a, b) = delta_ab + Z^T[a][b].
-template <class T>
-__global__ void __launch_bounds__(256)
-k_build_s(const T* __restrict__ Zt, T* __restrict__ S, int B, int n, int r) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = gid / (r * r);
-    const int q = gid - m * r * r;
-    if (m >= B) return;
-    const int b = q / r, a = q - b * r;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    T v = Zt[(ch * ((size_t)r * n) + (size_t)a * n + b) * IL + ml];
-    if (a == b) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);
-    S[(size_t)m * r * r + (size_t)b * r + a] = v;
 }
 
 // St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
@@ -901,10 +1009,10 @@ using SolveSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const vo
 
 template <class T, int KL, int WU, int NR>
 void launch_solve_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Lf, const int* piv,
-                    const void* Zt, const void* Wf, const void* S, const int* spiv, const int* perm, void* ws,
+                    const void* Vt, const void* W, const void* S, const int* spiv, const int* perm, void* ws,
                     const void* bin, void* xout, int64_t rs, int B, int n, int r, int US, int bufbytes) {
     hipLaunchKernelGGL((k_solve_s<T, KL, WU, NR>), g, b, lds, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
-                       piv, static_cast<const T*>(Zt), static_cast<const T*>(Wf), static_cast<const T*>(S), spiv, perm,
+                       piv, static_cast<const T*>(Vt), static_cast<const T*>(W), static_cast<const T*>(S), spiv, perm,
                        static_cast<T*>(ws), static_cast<const T*>(bin), static_cast<T*>(xout), rs, B, n, r, US,
                        bufbytes);
 }
@@ -934,6 +1042,52 @@ SolveSFn solve_s_fn(int KL, int WU, int nr) {
         case 2: return solve_s_wu<T, 2>(WU, nr);
         case 4: return solve_s_wu<T, 4>(WU, nr);
         case 6: return solve_s_wu<T, 6>(WU, nr);
+    }
+    return nullptr;
+}
+
+using BorderFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, void*, void*, int, int, int, int);
+template <class T, int WU>
+void launch_border(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Vt, void* W, void* S, int B, int n,
+                   int r, int US) {
+    hipLaunchKernelGGL((k_border<T, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<const T*>(Vt),
+                       static_cast<T*>(W), static_cast<T*>(S), B, n, r, US);
+}
+template <class T>
+BorderFn border_fn(int WU) {
+    switch (WU) {
+        case 4: return &launch_border<T, 4>;
+        case 8: return &launch_border<T, 8>;
+        case 16: return &launch_border<T, 16>;
+        case 24: return &launch_border<T, 24>;
+        case 32: return &launch_border<T, 32>;
+    }
+    return nullptr;
+}
+
+using BorderSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, void*, void*, int, int, int, int,
+                           int);
+template <class T, int WU, int NR>
+void launch_border_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Vt, void* W, void* S, int B,
+                     int n, int r, int US, int bufbytes) {
+    hipLaunchKernelGGL((k_border_s<T, WU, NR>), g, b, lds, s, static_cast<const T*>(Uf), static_cast<const T*>(Vt),
+                       static_cast<T*>(W), static_cast<T*>(S), B, n, r, US, bufbytes);
+}
+template <class T, int WU>
+BorderSFn border_s_nr(int nr) {
+    switch (nr) {
+        case 1: return &launch_border_s<T, WU, 1>;
+        case 2: return &launch_border_s<T, WU, 2>;
+        case 4: return &launch_border_s<T, WU, 4>;
+    }
+    return nullptr;
+}
+template <class T>
+BorderSFn border_s_fn(int WU, int nr) {
+    switch (WU) {
+        case 4: return border_s_nr<T, 4>(nr);
+        case 8: return border_s_nr<T, 8>(nr);
+        case 16: return border_s_nr<T, 16>(nr);
     }
     return nullptr;
 }
@@ -1020,34 +1174,60 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
     (void)pos;
 }
 
+// LDS bytes of one buffer of the streamed border completion, or -1 if it does not apply
+int border_bufbytes(const cn_bb& h) {
+    if (!h.streamed || h.WU > 16) return -1;
+    const int buf = bwd_bytes(h.US, h.r, (int)h.esz);
+    return 2 * buf <= LDS_CAP ? buf : -1;
+}
+
+int nr_for(int r) { return r >= 4 ? 4 : (r >= 2 ? 2 : 1); }
+
 template <class T>
 void run_factor(cn_bb& h, hipStream_t s) {
     const int spb = 256 / h.G;
     const dim3 fb(256), fg((h.Bp + spb - 1) / spb);
     FactorFn f = factor_fn<T>(h.KLP);
     if (!f) fail(CN_ELOGIC, "no factor kernel for kl " + std::to_string(h.KLP));
-    // band LU, with the forward half Wf = L^-1 P_B U of W in the r extra lanes
+    // band LU, with the forward half L^-1 P_B U of W in the r extra lanes
     f(fg, fb, s, h.Ab.p, h.Uf.p, h.Lf.p, static_cast<int*>(h.piv.p), h.W.p, static_cast<int*>(h.info.p), h.B, h.n,
       h.M, h.r, h.G, h.US, h.pivot);
     hip_check(hipGetLastError(), "k_band_lu launch");
+    h.tr_ready = false;   // the transposed Woodbury data follow the factors (built on demand)
+    if (h.r == 0) return;
+    // W = U^-1 (L^-1 P_B U) in place and S = I + V^T W, then the LU of S
+    const int bb = border_bufbytes(h);
+    BorderSFn fs = bb > 0 ? border_s_fn<T>(h.WU, nr_for(h.r)) : nullptr;
+    if (fs) {
+        const int nr = nr_for(h.r);
+        fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(64), 2 * (size_t)bb, s, h.Uf.p, h.Vt.p, h.W.p, h.S.p,
+           h.B, h.n, h.r, h.US, bb);
+        hip_check(hipGetLastError(), "k_border_s launch");
+    } else {
+        BorderFn fd = border_fn<T>(h.WU);
+        if (!fd) fail(CN_ELOGIC, "no border kernel for wu " + std::to_string(h.WU));
+        const int lanes = h.B * h.r;
+        fd(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Vt.p, h.W.p, h.S.p, h.B, h.n, h.r, h.US);
+        hip_check(hipGetLastError(), "k_border launch");
+    }
+    hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.S.p),
+                       static_cast<int*>(h.spiv.p), static_cast<double*>(h.cond.p), static_cast<int*>(h.info.p),
+                       h.B, h.r);
+    hip_check(hipGetLastError(), "k_cap_lu launch");
+}
+
+// Transposed Woodbury data: Wt = B^-T V (rows B^-T v_a), St = I + U^T Wt and its LU, checked.
+template <class T>
+void run_prepare_t(cn_bb& h, hipStream_t s) {
     if (h.r > 0) {
-        // Z^T = V^T B^-1 (rows B^-T v_a), used by the forward solve (t = Z^T c) and, as Wt, by the
-        // transposed solve; then S = I + Z^T U and St = I + U^T Wt, both LU-factored
         SolveTFn ft = solve_t_fn<T>(h.KLP, h.WU);
         if (!ft) fail(CN_ELOGIC, "no transposed solve kernel");
         const int lanes = h.B * h.r;
         ft(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.Wt.p,
            h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), nullptr, nullptr, 0, h.B, h.n,
            h.r, h.US, h.r, 1);
-        hip_check(hipGetLastError(), "k_solve_t (Z^T) launch");
+        hip_check(hipGetLastError(), "k_solve_t (Wt) launch");
         const int l2 = h.B * h.r * h.r;
-        hipLaunchKernelGGL((k_build_s<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
-                           static_cast<T*>(h.S.p), h.B, h.n, h.r);
-        hip_check(hipGetLastError(), "k_build_s launch");
-        hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.S.p),
-                           static_cast<int*>(h.spiv.p), static_cast<double*>(h.cond.p), static_cast<int*>(h.info.p),
-                           h.B, h.r);
-        hip_check(hipGetLastError(), "k_cap_lu launch");
         hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
                            static_cast<T*>(h.St.p), static_cast<int*>(h.info_t.p), h.B, h.n, h.r);
         hip_check(hipGetLastError(), "k_build_st launch");
@@ -1055,6 +1235,13 @@ void run_factor(cn_bb& h, hipStream_t s) {
                            static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
                            static_cast<int*>(h.info_t.p), h.B, h.r);
         hip_check(hipGetLastError(), "k_cap_lu (transposed) launch");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        std::vector<int> info_t(h.B);
+        hip_check(hipMemcpy(info_t.data(), h.info_t.p, sizeof(int) * h.B, hipMemcpyDeviceToHost), "hipMemcpy info_t");
+        for (int m = 0; m < h.B; ++m)
+            if (info_t[m] != 0)
+                fail(CN_ESINGULAR, "BandedBatch: transposed capacitance matrix of mode " + std::to_string(m) +
+                                       " is singular");
     }
     h.tr_ready = true;
 }
@@ -1092,14 +1279,6 @@ void factor_and_check(cn_bb& h, hipStream_t s, bool check) {
         if (std::isnan(h.hcond[m]))
             fail(CN_ESINGULAR, "BandedBatch: capacitance condition number of mode " + std::to_string(m) + " is NaN");
         h.cond_max = std::max(h.cond_max, h.hcond[m]);
-    }
-    if (h.r > 0) {
-        std::vector<int> info_t(h.B);
-        hip_check(hipMemcpy(info_t.data(), h.info_t.p, sizeof(int) * h.B, hipMemcpyDeviceToHost), "hipMemcpy info_t");
-        for (int m = 0; m < h.B; ++m)
-            if (info_t[m] != 0)
-                fail(CN_ESINGULAR, "BandedBatch: transposed capacitance matrix of mode " + std::to_string(m) +
-                                       " is singular");
     }
 }
 
@@ -1205,6 +1384,8 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     hip_check(hipMemset(h.Uf.p, 0, nU * es), "hipMemset U");
     hip_check(hipMemset(h.Lf.p, 0, nL * es), "hipMemset L");
     hip_check(hipMemset(h.info.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info");
+    hip_check(hipMemset(h.piv.p, 0, (size_t)h.Bp * n * sizeof(int)), "hipMemset piv");
+    hip_check(hipMemset(h.S.p, 0, std::max<size_t>((size_t)h.Bp * r * r, 1) * es), "hipMemset S");
     hip_check(hipMemset(h.cond.p, 0, (size_t)h.Bp * sizeof(double)), "hipMemset cond");
 }
 
@@ -1214,9 +1395,9 @@ void reserve_impl(cn_bb& h, int nrhs) {
     h.nrhs_cap = nrhs;
 }
 
-void prepare_t(cn_bb& h) {
-    // the transposed Woodbury data (Wt = Z^T, LU of St) are built with every factorization
-    if (!h.tr_ready) fail(CN_ELOGIC, "BandedBatch: factorization missing");
+void prepare_t(cn_bb& h, hipStream_t s) {
+    if (h.tr_ready) return;
+    if (h.dtype == CN_C128) run_prepare_t<zc>(h, s); else run_prepare_t<double>(h, s);
 }
 
 void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
@@ -1225,7 +1406,7 @@ void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
         hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
         if (cs != hipStreamCaptureStatusNone)
             fail(CN_EINVAL, "BandedBatch transposed solve: prepare_transpose / reserve must run before graph capture");
-        prepare_t(h);
+        prepare_t(h, s);
         reserve_impl(h, nrhs);
     }
     SolveTFn f = (h.dtype == CN_C128) ? solve_t_fn<zc>(h.KLP, h.WU) : solve_t_fn<double>(h.KLP, h.WU);
@@ -1237,18 +1418,14 @@ void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
     hip_check(hipGetLastError(), "k_solve_t launch");
 }
 
-// LDS bytes of one buffer of the streamed solve (see k_solve_s), or -1 if it does not apply
-int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int nrhs) {
+// LDS bytes of one buffer of the streamed solve (k_solve_s), or -1 if it does not apply
+int streamed_bufbytes(const cn_bb& h, const SolveIO& io) {
+    if (!h.streamed) return -1;
     if (!(io.mode == 0 || (io.mode == 1 && !io.rows && !io.cols))) return -1;
-    if (!(nrhs == 1 || nrhs == 2 || nrhs == 4)) return -1;
     if (h.KLP + 1 > SP || h.WU > 16) return -1;
-    const int C = (64 / nrhs) / IL;
-    const int CE = IL * (int)h.esz;
-    const int fchunk = SP * h.KLP * CE + h.r * SP * CE + SP * IL * 4;
-    const int bchunk = SP * h.US * CE + h.r * SP * CE;
-    const int buf = (std::max(C * fchunk, C * bchunk) + 15) / 16 * 16;
-    if (2 * buf > 64 * 1024) return -1;
-    return buf;
+    const int e = (int)h.esz;
+    const int buf = std::max({fwd_bytes(h.KLP, e), bwd_bytes(h.US, h.r, e), cor_bytes(h.r, e)});
+    return 2 * buf <= LDS_CAP ? buf : -1;
 }
 
 void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
@@ -1260,23 +1437,28 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
                                 " right-hand sides not reserved before graph capture (call cn_bb_reserve)");
         reserve_impl(h, nrhs);
     }
-    const int bufbytes = h.streamed ? streamed_bufbytes(h, io, nrhs) : -1;
-    if (bufbytes > 0) {
-        SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nrhs) : solve_s_fn<double>(h.KLP, h.WU, nrhs);
-        if (fs) {
-            const int NS = 64 / nrhs;
-            const void* bin = io.mode == 0 ? io.x : io.rhs;
-            fs(dim3((h.B + NS - 1) / NS), dim3(64), 2 * bufbytes, s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p),
-               h.Wt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, bin,
-               io.x, io.mode == 0 ? io.rhs_stride : 0, h.B, h.n, h.r, h.US, bufbytes);
+    const int bufbytes = streamed_bufbytes(h, io);
+    if (bufbytes > 0 && solve_s_fn<double>(h.KLP, h.WU, 1)) {
+        // groups of up to 4 right-hand sides, one launch each (in place: rhs k at k * rhs_stride)
+        char* xb = static_cast<char*>(io.x);
+        const char* bb = static_cast<const char*>(io.mode == 0 ? io.x : io.rhs);
+        const int64_t rs = io.mode == 0 ? io.rhs_stride : 0;
+        for (int k0 = 0; k0 < nrhs;) {
+            const int nr = nr_for(nrhs - k0);
+            SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nr) : solve_s_fn<double>(h.KLP, h.WU, nr);
+            const size_t off = (size_t)k0 * (size_t)rs * h.esz;
+            fs(dim3((h.B + IL - 1) / IL), dim3(64), 2 * (size_t)bufbytes, s, h.Uf.p, h.Lf.p,
+               static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p),
+               static_cast<const int*>(h.perm.p), h.ws.p, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
             hip_check(hipGetLastError(), "k_solve_s launch");
-            return;
+            k0 += nr;
         }
+        return;
     }
     SolveFn f = (h.dtype == CN_C128) ? solve_fn<zc>(h.KLP, h.WU) : solve_fn<double>(h.KLP, h.WU);
     if (!f) fail(CN_ELOGIC, "no solve kernel for (kl, wu) = (" + std::to_string(h.KLP) + ", " + std::to_string(h.WU) + ")");
     const int lanes = h.B * nrhs;
-    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Wt.p, h.W.p,
+    f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p,
       h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, io, h.B, h.n, h.r, h.US,
       nrhs);
     hip_check(hipGetLastError(), "k_solve launch");
@@ -1348,7 +1530,6 @@ int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n, const int32_t* const
 int cn_bb_factor(cn_bb_t h, void* stream, int check) {
     return guarded([&] {
         if (!h) fail(CN_EINVAL, "cn_bb_factor: null handle");
-        h->tr_ready = false;   // the transposed data follow the factors
         factor_and_check(*h, static_cast<hipStream_t>(stream), check != 0);
     });
 }
@@ -1416,7 +1597,7 @@ int cn_bb_solve_split(cn_bb_t h, const double* re, const int64_t* rows_c, const 
 int cn_bb_prepare_transpose(cn_bb_t h) {
     return guarded([&] {
         if (!h) fail(CN_EINVAL, "cn_bb_prepare_transpose: null handle");
-        prepare_t(*h);
+        prepare_t(*h, nullptr);
     });
 }
 

@@ -122,17 +122,22 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
            "wall_ms_per_step": wall * 1e3 / steps, "mode_solves_per_s": pb.K * 1e3 / ms,
            "gate_relative_difference_20_steps": gate, "graph": True}
     if with_cpu:
+        # blocks of 100 steps from the initial state (the flow stays resolved), factorizations of
+        # both startup orders built before the timing
         o = RB2DOracle(pb, "SBDF2", 1e-4)
-        Xc = o.run(X0, 3)                  # factorizations and FFT plans outside the timing
+        o.run(X0, 3)
         n_done, t_cpu, tc0 = 0, 0.0, time.perf_counter()
         while time.perf_counter() - tc0 < cpu_seconds:
+            o.MXh, o.Fh = [], []
             ts = time.perf_counter()
-            Xc = o.run(Xc, 10)
+            Xc = o.run(X0, 100)
             t_cpu += time.perf_counter() - ts
-            n_done += 10
+            n_done += 100
+        if not np.all(np.isfinite(Xc)):
+            raise RuntimeError("rb2d CPU baseline state not finite")
         out["cpu_baseline"] = {"value": n_done / t_cpu, "unit": "steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{n_done} SBDF2 steps of the same problem after 3 startup steps "
-                                         f"({t_cpu:.1f} s, numpy/pocketfft + LAPACK zgbtrs)"}
+                               "sample": f"{n_done} SBDF2 steps of the same problem in blocks of 100 from the "
+                                         f"initial state ({t_cpu:.1f} s, numpy/pocketfft + LAPACK zgbtrs)"}
     return out
 
 

@@ -30,7 +30,8 @@ class Props(ctypes.Structure):
                 ("pivot", ctypes.c_int32), ("kl_kernel", ctypes.c_int32), ("wu_kernel", ctypes.c_int32),
                 ("lanes", ctypes.c_int32), ("nrhs_reserved", ctypes.c_int32),
                 ("transpose_ready", ctypes.c_int32), ("cond_max", ctypes.c_double),
-                ("factor_ms", ctypes.c_double), ("factor_bytes", ctypes.c_int64)]
+                ("factor_ms", ctypes.c_double), ("factor_bytes", ctypes.c_int64),
+                ("solve_mode", ctypes.c_int32)]
 
 
 _lock = threading.Lock()
@@ -68,9 +69,9 @@ def _declare(lib):
         "cn_rk_rhs": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, ctypes.POINTER(_vp),
                                      ctypes.POINTER(_vp), _dp, _dp, ctypes.c_int, _vp, _vp]),
         "cn_pointwise": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
-        "cn_rb2d_explicit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+        "cn_rb2d_explicit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                            ctypes.c_int, ctypes.c_double, _vp]),
+                                            ctypes.c_double, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -87,19 +87,19 @@ template <class T, int KL>
 __global__ void __launch_bounds__(256)
 k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int* __restrict__ piv,
           T* __restrict__ Wf, int* __restrict__ info, int B, int n, int M, int r, int G, int US,
-          int dopivot) {
+          int dopivot, int il, int ucol) {
     const int t = threadIdx.x;
     const int g = t / G;
     const int c = t - g * G;
     const int m = blockIdx.x * (256 / G) + g;
     const bool live = m < B;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    const T* A0 = Ab + ch * ((size_t)n * M * IL) + ml;
-    T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
-    T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
-    int* P0 = piv + ch * ((size_t)n * IL) + ml;
-    T* W0 = Wf + ch * ((size_t)r * n * IL) + ml;
+    const size_t ch = m / il;
+    const int ml = m % il;
+    const T* A0 = Ab + ch * ((size_t)n * M * il) + ml;
+    T* U0 = Uf + ch * ((size_t)n * US * il) + ml;
+    T* L0 = Lf + ch * ((size_t)n * KL * il) + ml;
+    int* P0 = piv + ch * ((size_t)n * il) + ml;
+    T* W0 = Wf + ch * ((size_t)r * n * il) + ml;
     const bool band = live && c < M;
     const bool wl = live && c >= M && c < M + r;
     const int k = c - M;
@@ -111,7 +111,7 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
         T v = cn::zero<T>();
         if (band && i < n) {
             const int d = c - i + KL;
-            if (d >= 0 && d < M) v = A0[((size_t)i * M + d) * IL];
+            if (d >= 0 && d < M) v = A0[((size_t)i * M + d) * il];
         }
         if (wl) v = (i == k) ? cn::one<T>() : cn::zero<T>();
         win[i] = v;
@@ -145,13 +145,14 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
         if (band) {
             T ov = win[0];
             if (is_owner) ov = rc;
-            U0[((size_t)j * US + d) * IL] = ov;
+            if (!ucol) U0[((size_t)j * US + d) * il] = ov;          // row j of U: U(j, j+d)
+            else if (j + d < n) U0[((size_t)(j + d) * US + d) * il] = ov;   // column j+d: U(j, j+d)
         }
-        if (wl) W0[((size_t)k * n + j) * IL] = win[0];
+        if (wl) W0[((size_t)k * n + j) * il] = win[0];
         if (band && is_owner) {
 #pragma unroll
-            for (int i = 0; i < KL; ++i) L0[((size_t)j * KL + i) * IL] = l[i];
-            P0[(size_t)j * IL] = ip;
+            for (int i = 0; i < KL; ++i) L0[((size_t)j * KL + i) * il] = l[i];
+            P0[(size_t)j * il] = ip;
         }
         if ((band && !is_owner) || wl) {
 #pragma unroll
@@ -166,7 +167,7 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
                 for (int i = 1; i <= KL; ++i) win[i] = cn::zero<T>();
             }
             const int dA = is_owner ? (M - 1) : (d - 1);
-            if (jn < n) v = A0[((size_t)jn * M + dA) * IL];
+            if (jn < n) v = A0[((size_t)jn * M + dA) * il];
         }
         if (wl && jn == k) v = cn::one<T>();
 #pragma unroll
@@ -761,6 +762,433 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
 }
 
 // ------------------------------------------------------------------------------------------
+// Wave mode: one wavefront per (system, right-hand side), for few or wide-band systems.
+// Factors are stored per system contiguously (no interleave) and U by columns, scaled by the
+// column's pivot after the factorization: Uc[j][0] = 1 / U(j, j), Uc[j][d] = U(j-d, j) / U(j, j)
+// (d >= 1).  The right-hand side, the forward result and the solution stay in LDS (n entries).
+// The sweeps keep the band window in the low lanes (lane 0 = the row being finished) and the
+// next rows in the lanes above; a step is one broadcast of lane 0 (v_readlane), one
+// multiply-add per window lane and one wave rotation (DPP wave_rol:1), which brings the next row
+// into the window and parks the finished row in lane 63:
+//   forward (w = kl+1 lanes): the gbtrs column order, pivot exchange of lanes 0 and p;
+//   backward (w = kl+ku+1 lanes): the column-oriented order of LAPACK's banded triangular solve,
+//   with the division by U(j, j) applied when the finished rows are flushed.
+// The L rows + pivots (forward) and U columns (backward) of the next block of SPW steps are
+// copied HBM -> LDS while the current block is computed.  t = V^T y, z = S^-1 t and
+// x = y - W z are parallel over the entries of x.
+// ------------------------------------------------------------------------------------------
+constexpr int SPW = 32;   // steps per staged block in wave mode
+
+// Ordering point of a one-wavefront workgroup: LDS operations of a wavefront execute in order,
+// and the staging copies are waited for explicitly (vm_wait_le), so only the compiler has to be
+// kept from moving memory operations across it (a workgroup barrier would also drain the copies
+// in flight).
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// yv[j] = src[j * st] for j < n (batched loads)
+template <class T>
+__device__ __forceinline__ void wave_load(T* yv, const T* __restrict__ src, size_t st, int n, int lane) {
+    for (int j0 = 0; j0 < n; j0 += 256) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = src[(size_t)min(j0 + 64 * u + lane, n - 1) * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + 64 * u + lane < n) yv[j0 + 64 * u + lane] = v[u];
+    }
+}
+
+
+__device__ __forceinline__ double bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ zc bcast(zc v, int src) { return zc{bcast(v.re, src), bcast(v.im, src)}; }
+
+__device__ __forceinline__ double xor_shfl(double v, int mask) {
+    const int lo = __shfl_xor(__double2loint(v), mask, 64);
+    const int hi = __shfl_xor(__double2hiint(v), mask, 64);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ zc xor_shfl(zc v, int mask) { return zc{xor_shfl(v.re, mask), xor_shfl(v.im, mask)}; }
+__device__ __forceinline__ double addv(double a, double b) { return a + b; }
+__device__ __forceinline__ zc addv(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
+
+// contiguous copy of nbytes (multiple of 4) into LDS, 256 bytes per instruction
+__device__ __forceinline__ void dma4(const void* src_, char* lds_dst, int nbytes, int lane) {
+    const char* src = static_cast<const char*>(src_);
+    for (int off = 0; off < nbytes; off += 256) {
+        const int my = off + lane * 4;
+        if (my < nbytes)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my), (lds_ptr)(lds_dst + off), 4, 0, 0);
+    }
+}
+// entries of T: 16-byte copies for complex (always 16-byte aligned), dwords for real
+template <class T>
+__device__ __forceinline__ void dmaT(const T* src, char* dst, int count, int lane) {
+    if constexpr (sizeof(T) == 16) dma16(src, dst, count * 16, lane);
+    else dma4(src, dst, count * (int)sizeof(T), lane);
+}
+
+__host__ __device__ constexpr int align16(int b) { return (b + 15) / 16 * 16; }
+// LDS bytes of one staging buffer of the wave-mode sweeps
+__host__ __device__ constexpr int wave_buf_bytes(int KL, int US, int esz) {
+    return align16(SPW * KL * esz + SPW * 4) > align16(SPW * US * esz) ? align16(SPW * KL * esz + SPW * 4)
+                                                                        : align16(SPW * US * esz);
+}
+
+// number of copy instructions dmaT / dma4 issue for `count` entries / bytes
+template <class T>
+__device__ __forceinline__ int dmaT_count(int count) {
+    return sizeof(T) == 16 ? (count * 16 + 1023) / 1024 : (count * (int)sizeof(T) + 255) / 256;
+}
+__device__ __forceinline__ int dma4_count(int nbytes) { return (nbytes + 255) / 256; }
+
+// s_waitcnt vmcnt(k) for a wave-uniform k in [0, 63]: wait until at most k vector memory
+// operations of this wave are outstanding (they complete in order: only loads are in flight).
+__device__ __forceinline__ void vm_wait_le(int k) {
+    switch (k) {
+#define CN_VMW(K) case K: asm volatile("s_waitcnt vmcnt(" #K ")" ::: "memory"); break;
+        CN_VMW(0) CN_VMW(1) CN_VMW(2) CN_VMW(3) CN_VMW(4) CN_VMW(5) CN_VMW(6) CN_VMW(7) CN_VMW(8) CN_VMW(9)
+        CN_VMW(10) CN_VMW(11) CN_VMW(12) CN_VMW(13) CN_VMW(14) CN_VMW(15) CN_VMW(16) CN_VMW(17) CN_VMW(18)
+        CN_VMW(19) CN_VMW(20) CN_VMW(21) CN_VMW(22) CN_VMW(23) CN_VMW(24) CN_VMW(25) CN_VMW(26) CN_VMW(27)
+        CN_VMW(28) CN_VMW(29) CN_VMW(30) CN_VMW(31) CN_VMW(32) CN_VMW(33) CN_VMW(34) CN_VMW(35) CN_VMW(36)
+        CN_VMW(37) CN_VMW(38) CN_VMW(39) CN_VMW(40) CN_VMW(41) CN_VMW(42) CN_VMW(43) CN_VMW(44) CN_VMW(45)
+        CN_VMW(46) CN_VMW(47) CN_VMW(48) CN_VMW(49) CN_VMW(50) CN_VMW(51) CN_VMW(52) CN_VMW(53) CN_VMW(54)
+        CN_VMW(55) CN_VMW(56) CN_VMW(57) CN_VMW(58) CN_VMW(59) CN_VMW(60) CN_VMW(61) CN_VMW(62)
+#undef CN_VMW
+        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
+// Staging copies of one block (returns the number of copy instructions issued)
+template <class T>
+__device__ __forceinline__ int wave_stage_fwd(const T* __restrict__ L0, const int* __restrict__ P0, int n, int KL,
+                                              int blk, char* buf, int lane) {
+    const int j0 = blk * SPW, cnt = min(SPW, n - j0);
+    dmaT<T>(L0 + (size_t)j0 * KL, buf, cnt * KL, lane);
+    dma4(P0 + j0, buf + SPW * KL * (int)sizeof(T), cnt * 4, lane);
+    return dmaT_count<T>(cnt * KL) + dma4_count(cnt * 4);
+}
+template <class T>
+__device__ __forceinline__ int wave_stage_bwd(const T* __restrict__ Uc0, int n, int US, int blk, char* buf, int lane) {
+    const int jhi = n - 1 - blk * SPW, jlo = max(0, jhi - SPW + 1);
+    dmaT<T>(Uc0 + (size_t)jlo * US, buf, (jhi - jlo + 1) * US, lane);
+    return dmaT_count<T>((jhi - jlo + 1) * US);
+}
+
+// One-lane rotation toward lane 0 across the whole wavefront (DPP wave_rol:1): lane i receives
+// lane i+1, lane 63 receives lane 0.
+__device__ __forceinline__ int rol1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xf, 0xf, false); }
+__device__ __forceinline__ double rol1(double v) {
+    return __hiloint2double(rol1(__double2hiint(v)), rol1(__double2loint(v)));
+}
+__device__ __forceinline__ zc rol1(zc v) { return zc{rol1(v.re), rol1(v.im)}; }
+
+// Forward sweep in place on yv (n entries): y_c = L^-1 P_B c.
+// Lanes 0 .. kl hold the window rows j .. j+kl (lane 0: pivot row), the lanes above hold the next
+// rows of c in order.  A step: pivot exchange of lanes 0 and p (selects, a no-op for p = 0),
+// broadcast of the pivot row, one multiply-add per lane (multiplier 0 outside the window: those
+// lanes read a zero slot), and one wave rotation, which moves the next row into the window and the
+// finished row to lane 63.  Steps run in branch-free groups of SB <= 64 - w; after a group the
+// lanes 64-SB .. 63 hold its finished rows: they are stored, and the lanes above the window are
+// reloaded with the next rows of c.  The multipliers of a block of SPW steps are read from LDS
+// into registers before the block.  A partial last block runs step by step (SB = 0: every block).
+template <class T>
+__device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
+    const T v0 = bcast(y, 0), vp = bcast(y, p);
+    y = cn::sel(lane == p, v0, y);
+    y = cn::sel(lane == 0, vp, y);
+    y = cn::fnms(l, vp, y);
+    y = rol1(y);
+}
+
+template <class T, int SB>
+__device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int* __restrict__ P0, T* yv,
+                                             const char* zslot, char* bufs, int bufbytes, int n, int KL, int lane) {
+    constexpr int ES = (int)sizeof(T);
+    const int w = KL + 1;
+    const int R = 64 - w;
+    const int nblk = (n + SPW - 1) / SPW;
+    const bool upd = lane >= 1 && lane <= KL;
+    const bool above = lane >= w;
+    T y = lane < n ? yv[lane] : cn::zero<T>();
+    int t = 0, jf = 0;            // steps since the last flush, first row finished since then
+    const int Poff = SPW * KL * ES;
+    // three staging buffers, copies issued two blocks ahead
+    wave_stage_fwd<T>(L0, P0, n, KL, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_fwd<T>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
+    vm_wait_le(c1);
+    wave_sync();
+    for (int b = 0; b < nblk; ++b) {
+        char* cur = bufs + (b % 3) * bufbytes;
+        const int c2 = b + 2 < nblk ? wave_stage_fwd<T>(L0, P0, n, KL, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int j0 = b * SPW, cnt = min(SPW, n - j0);
+        const int pv = lane < cnt ? lds_at<int>(cur + Poff + lane * 4) : 0;   // pivot of step lane
+        const char* lb = upd ? cur + (lane - 1) * ES : zslot;
+        const int lst = upd ? KL * ES : 0;
+        if (SB > 0 && cnt == SPW) {
+            T lv[SPW];
+#pragma unroll
+            for (int sidx = 0; sidx < SPW; ++sidx) lv[sidx] = lds_at<T>(lb + sidx * lst);
+#pragma unroll
+            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+#pragma unroll
+                for (int u = 0; u < SB; ++u) fwd_step<T>(y, lv[q * SB + u], __builtin_amdgcn_readlane(pv, q * SB + u), lane);
+                const int jn = j0 + (q + 1) * SB;     // first row of the new window
+                if (lane >= 64 - SB) yv[jf + lane - (64 - SB)] = y;
+                if (above) y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                jf += SB;
+            }
+        } else {
+            for (int sidx = 0; sidx < cnt; ++sidx) {
+                fwd_step<T>(y, lds_at<T>(lb + sidx * lst), __builtin_amdgcn_readlane(pv, sidx), lane);
+                if (++t == R) {       // lane w + i: finished row jf + i
+                    const int jn = j0 + sidx + 1;
+                    if (above) {
+                        yv[jf + lane - w] = y;
+                        y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                    }
+                    jf += R;
+                    t = 0;
+                }
+            }
+        }
+        vm_wait_le(c2);           // block b+1 has landed
+        wave_sync();
+    }
+    if (lane >= 64 - t) yv[jf + lane - (64 - t)] = y;   // the last t finished rows
+    wave_sync();
+}
+
+// Backward sweep in place on yv: y = U^-1 y_c with U stored by columns and scaled by the pivot of
+// the column, Uc[j][d] = U(j-d, j) / U(j, j) (d >= 1), Uc[j][0] = 1 / U(j, j).  Lanes 0 .. w-1
+// hold rows j, j-1, .., j-w+1, the lanes above the next rows of y_c.  A step: broadcast of the
+// finished row value y_j (lane 0), every window lane subtracts Uc[j][d] y_j, one wave rotation
+// (y_j moves to lane 63).  Groups, flushes and the partial last block as in the forward sweep;
+// the division x_j = y_j / U(j, j) is applied to the finished rows when they are stored, from the
+// diagonal dv (n entries in LDS).
+template <class T>
+__device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
+    const T yj = bcast(y, 0);
+    y = cn::fnms(u, yj, y);
+    y = rol1(y);
+}
+
+template <class T, int SB>
+__device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot, char* bufs,
+                                              int bufbytes, int n, int w, int US, int lane) {
+    constexpr int ES = (int)sizeof(T);
+    const int R = 64 - w;
+    const int nblk = (n + SPW - 1) / SPW;
+    const bool upd = lane >= 1 && lane < w;
+    const bool above = lane >= w;
+    wave_load<T>(dv, Uc0, US, n, lane);
+    T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
+    int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
+    wave_stage_bwd<T>(Uc0, n, US, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_bwd<T>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
+    vm_wait_le(c1);
+    wave_sync();
+    for (int b = 0; b < nblk; ++b) {
+        char* cur = bufs + (b % 3) * bufbytes;
+        const int c2 = b + 2 < nblk ? wave_stage_bwd<T>(Uc0, n, US, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int jhi = n - 1 - b * SPW, jlo = max(0, jhi - SPW + 1);
+        const int cnt = jhi - jlo + 1;
+        const char* ub = upd ? cur + lane * ES : zslot;
+        const int ust = upd ? US * ES : 0;
+        if (SB > 0 && cnt == SPW) {
+            T uv[SPW];
+#pragma unroll
+            for (int sidx = 0; sidx < SPW; ++sidx) uv[sidx] = lds_at<T>(ub + (SPW - 1 - sidx) * ust);
+#pragma unroll
+            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+#pragma unroll
+                for (int u = 0; u < SB; ++u) bwd_step<T>(y, uv[q * SB + u], lane);
+                const int jn = jhi - (q + 1) * SB;    // row of lane 0 in the new window
+                if (lane >= 64 - SB) {
+                    const int row = jf - (lane - (64 - SB));
+                    yv[row] = cn::mul(y, dv[row]);
+                }
+                if (above) y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                jf -= SB;
+            }
+        } else {
+            for (int sidx = 0; sidx < cnt; ++sidx) {
+                bwd_step<T>(y, lds_at<T>(ub + (cnt - 1 - sidx) * ust), lane);
+                if (++t == R) {       // lane w + i: finished row jf - i
+                    const int jn = jhi - sidx - 1;
+                    if (above) {
+                        const int row = jf - (lane - w);
+                        yv[row] = cn::mul(y, dv[row]);
+                        y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                    }
+                    jf -= R;
+                    t = 0;
+                }
+            }
+        }
+        vm_wait_le(c2);           // block b+1 has landed
+        wave_sync();
+    }
+    if (lane >= 64 - t) {
+        const int row = jf - (lane - (64 - t));
+        yv[row] = cn::mul(y, dv[row]);
+    }
+    wave_sync();
+}
+
+// group size of the backward sweep for a window of w lanes (0: step by step)
+__host__ __device__ constexpr int bwd_group(int w) { return w <= 32 ? 32 : w <= 48 ? 16 : w <= 56 ? 8 : 0; }
+
+template <class T>
+__device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot,
+                                                  char* bufs, int bufbytes, int n, int w, int US, int lane) {
+    switch (bwd_group(w)) {
+        case 32: wave_backward<T, 32>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 16: wave_backward<T, 16>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 8: wave_backward<T, 8>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        default: wave_backward<T, 0>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+    }
+}
+
+// Loads of the parallel phases are batched (every load of a chunk of 128 entries issued before
+// the first use), so that a phase costs a few memory latencies, not one per entry.
+constexpr int WCH = 2;   // 64-entry groups per chunk
+
+// t[a] = sum_j V^T(a, j) y_j over the wave (every lane ends with the totals)
+template <class T>
+__device__ __forceinline__ void wave_vty(const T* __restrict__ V0, const T* yv, int n, int r, int lane, T (&t)[MAXR]) {
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    for (int j0 = 0; j0 < n; j0 += 64 * WCH) {
+        T vv[WCH][MAXR];
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = min(j0 + 64 * u + lane, n - 1);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a) vv[u][a] = V0[(size_t)min(a, max(r - 1, 0)) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = j0 + 64 * u + lane;
+            const T yj = j < n ? yv[j] : cn::zero<T>();
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(vv[u][a], yj, t[a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) {
+        if (a < r) {
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) t[a] = addv(t[a], xor_shfl(t[a], mask));
+        }
+    }
+}
+
+// Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][dv: n entries][zero
+// slot][3 staging buffers].
+template <class T>
+__global__ void __launch_bounds__(64)
+k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, SolveIO io, int n, int r, int KL, int w2,
+          int US, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x, k = blockIdx.y;
+    const size_t mn = (size_t)m * n;
+    T* yv = reinterpret_cast<T*>(lds);
+    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
+    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    char* bufs = zslot + 16;
+    for (int j0 = 0; j0 < n; j0 += 256) {   // c = P b (batched loads)
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = io_load_b<T>(io, mn, k, perm[min(j0 + 64 * u + lane, n - 1)]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + 64 * u + lane < n) yv[j0 + 64 * u + lane] = v[u];
+    }
+    if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    wave_sync();
+    wave_forward<T, 32>(Lf + mn * KL, piv + mn, yv, zslot, bufs, bufbytes, n, KL, lane);   // kl + 1 <= 17 lanes
+    wave_backward_any<T>(Uc + mn * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    if (r == 0) {
+        for (int j = lane; j < n; j += 64) io_store<T>(io, mn, k, j, yv[j]);
+        return;
+    }
+    const T* V0 = Vt + (size_t)m * r * n;
+    const T* W0 = W + (size_t)m * r * n;
+    T t[MAXR];
+    wave_vty<T>(V0, yv, n, r, lane, t);
+    cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    for (int j0 = 0; j0 < n; j0 += 64 * WCH) {   // x = y - W z (batched loads)
+        T wv[WCH][MAXR];
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = min(j0 + 64 * u + lane, n - 1);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a) wv[u][a] = W0[(size_t)min(a, max(r - 1, 0)) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = j0 + 64 * u + lane;
+            if (j < n) {
+                T v = yv[j];
+#pragma unroll
+                for (int a = 0; a < MAXR; ++a)
+                    if (a < r) v = cn::fnms(wv[u][a], t[a], v);
+                io_store<T>(io, mn, k, j, v);
+            }
+        }
+    }
+}
+
+// Border completion in wave mode: grid (B, r); W row a = U^-1 (its forward half), in place, and
+// column a of S = I + V^T W.
+template <class T>
+__global__ void __launch_bounds__(64)
+k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W, T* __restrict__ S, int n, int r,
+           int w2, int US, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x, a = blockIdx.y;
+    T* yv = reinterpret_cast<T*>(lds);
+    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
+    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    char* bufs = zslot + 16;
+    T* Wa = W + ((size_t)m * r + a) * n;
+    wave_load<T>(yv, Wa, 1, n, lane);
+    if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    wave_sync();
+    wave_backward_any<T>(Uc + (size_t)m * n * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    for (int j = lane; j < n; j += 64) Wa[j] = yv[j];
+    T t[MAXR];
+    wave_vty<T>(Vt + (size_t)m * r * n, yv, n, r, lane, t);
+    if (lane == 0) {
+        T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
+    }
+}
+
+// Wave mode, after the band LU: the columns of U scaled by their pivots, Uc[j][d] *= Uc[j][0]
+// (= 1 / U(j, j)) for d >= 1.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_scale_ucol(T* __restrict__ Uc, size_t total, int US) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int d = (int)(e % US);
+    if (d == 0) return;
+    Uc[e] = cn::mul(Uc[e], Uc[e - d]);
+}
+
+// ------------------------------------------------------------------------------------------
 // Transposed solve A^T x = b (plain transpose), the discrete adjoint path (banded.hpp:346-372).
 // A^T = (B^T + V U^T) P, so y = P x solves (B^T + V U^T) y = b:
 //   y0 = B^-T b = P_B L^-T U^-T b,  z = St^-1 y0[:r],  y = y0 - Wt z,  x[perm[i]] = y[i],
@@ -776,26 +1204,27 @@ __global__ void __launch_bounds__(256)
 k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, T* __restrict__ Wt, const T* __restrict__ St,
           const int* __restrict__ stpiv, const int* __restrict__ perm, T* __restrict__ ws,
-          T* __restrict__ x, int64_t rhs_stride, int B, int n, int r, int US, int nrhs, int prep) {
+          T* __restrict__ x, int64_t rhs_stride, int B, int n, int r, int US, int nrhs, int prep, int il,
+          int ucol) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gid / nrhs;
     const int k = gid - m * nrhs;
     if (m >= B) return;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
-    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
-    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    const size_t ch = m / il;
+    const int ml = m % il;
+    const T* U0 = Uf + ch * ((size_t)n * US * il) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * il) + ml;
+    const int* P0 = piv + ch * ((size_t)n * il) + ml;
     // input row, work row (stride between consecutive entries)
     const T* in;
     size_t in_st;
     T* w;
     size_t w_st;
     if (prep) {
-        in = Vt + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
-        in_st = IL;
-        w = Wt + ch * ((size_t)r * n * IL) + ml + (size_t)k * n * IL;
-        w_st = IL;
+        in = Vt + ch * ((size_t)r * n * il) + ml + (size_t)k * n * il;
+        in_st = il;
+        w = Wt + ch * ((size_t)r * n * il) + ml + (size_t)k * n * il;
+        w_st = il;
     } else {
         in = x + k * rhs_stride + (size_t)m * n;
         in_st = 1;
@@ -807,11 +1236,20 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 #pragma unroll
     for (int d = 0; d < WU; ++d) uw[d] = cn::zero<T>();   // uw[d-1] = w_{j-d}
     for (int j = 0; j < n; ++j) {
-        T acc = in[(size_t)j * in_st];
+        T wj;
+        if (!ucol) {               // row j-d entry d = U(j-d, j)
+            T acc = in[(size_t)j * in_st];
 #pragma unroll
-        for (int d = 1; d <= WU; ++d)
-            if (j - d >= 0) acc = cn::fnms(U0[((size_t)(j - d) * US + d) * IL], uw[d - 1], acc);
-        const T wj = cn::mul(acc, U0[(size_t)j * US * IL]);
+            for (int d = 1; d <= WU; ++d)
+                if (j - d >= 0) acc = cn::fnms(U0[((size_t)(j - d) * US + d) * il], uw[d - 1], acc);
+            wj = cn::mul(acc, U0[(size_t)j * US * il]);
+        } else {                   // column j entry d = U(j-d, j) / U(j, j)
+            T acc = cn::mul(in[(size_t)j * in_st], U0[(size_t)j * US * il]);
+#pragma unroll
+            for (int d = 1; d <= WU; ++d)
+                if (j - d >= 0) acc = cn::fnms(U0[((size_t)j * US + d) * il], uw[d - 1], acc);
+            wj = acc;
+        }
         w[(size_t)j * w_st] = wj;
 #pragma unroll
         for (int d = WU - 1; d > 0; --d) uw[d] = uw[d - 1];
@@ -827,11 +1265,11 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 #pragma unroll
         for (int i = KL; i > 0; --i) y[i] = y[i - 1];
         T v = w[(size_t)j * w_st];
-        const T* Lr = L0 + (size_t)j * KL * IL;
+        const T* Lr = L0 + (size_t)j * KL * il;
 #pragma unroll
-        for (int i = 1; i <= KL; ++i) v = cn::fnms(Lr[(i - 1) * IL], y[i], v);
+        for (int i = 1; i <= KL; ++i) v = cn::fnms(Lr[(i - 1) * il], y[i], v);
         y[0] = v;
-        const int p = P0[(size_t)j * IL];
+        const int p = P0[(size_t)j * il];
         if (p != 0) cn::swap_first<T, KL>(y, p);
     }
 #pragma unroll
@@ -873,13 +1311,13 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             }
         }
     }
-    const T* W0 = Wt + ch * ((size_t)r * n * IL) + ml;
+    const T* W0 = Wt + ch * ((size_t)r * n * il) + ml;
     T* xo = x + k * rhs_stride + (size_t)m * n;
     for (int i = 0; i < n; ++i) {
         T v = w[(size_t)i * w_st];
 #pragma unroll
         for (int a = 0; a < MAXR; ++a)
-            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * IL], z[a], v);
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * il], z[a], v);
         xo[perm[i]] = v;
     }
 }
@@ -887,16 +1325,16 @@ k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 // St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
 template <class T>
 __global__ void __launch_bounds__(256)
-k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int* __restrict__ info_t, int B, int n, int r) {
+k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int* __restrict__ info_t, int B, int n, int r, int il) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = gid / (r * r);
     const int q = gid - m * r * r;
     if (m >= B) return;
     if (q == 0) info_t[m] = 0;
     const int b = q / r, a = q - b * r;
-    const size_t ch = m / IL;
-    const int ml = m % IL;
-    T v = Wt[(ch * ((size_t)r * n) + (size_t)b * n + a) * IL + ml];
+    const size_t ch = m / il;
+    const int ml = m % il;
+    T v = Wt[(ch * ((size_t)r * n) + (size_t)b * n + a) * il + ml];
     if (a == b) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);
     St[(size_t)m * r * r + (size_t)b * r + a] = v;
 }
@@ -917,15 +1355,15 @@ int pick_wu(int w) {
 }
 
 using FactorFn = void (*)(dim3, dim3, hipStream_t, const void*, void*, void*, int*, void*, int*, int, int,
-                          int, int, int, int, int);
+                          int, int, int, int, int, int, int);
 using SolveFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, const int*, const void*, const void*,
                          const void*, const int*, const int*, void*, SolveIO, int, int, int, int, int);
 
 template <class T, int KL>
 void launch_factor(dim3 g, dim3 b, hipStream_t s, const void* Ab, void* Uf, void* Lf, int* piv, void* Wf,
-                   int* info, int B, int n, int M, int r, int G, int US, int pivot) {
+                   int* info, int B, int n, int M, int r, int G, int US, int pivot, int il, int ucol) {
     hipLaunchKernelGGL((k_band_lu<T, KL>), g, b, 0, s, static_cast<const T*>(Ab), static_cast<T*>(Uf),
-                       static_cast<T*>(Lf), piv, static_cast<T*>(Wf), info, B, n, M, r, G, US, pivot);
+                       static_cast<T*>(Lf), piv, static_cast<T*>(Wf), info, B, n, M, r, G, US, pivot, il, ucol);
 }
 template <class T, int KL, int WU>
 void launch_solve(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf, const int* piv, const void* Vt,
@@ -937,15 +1375,16 @@ void launch_solve(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf,
 }
 
 using SolveTFn = void (*)(dim3, dim3, hipStream_t, const void*, const void*, const int*, const void*, void*,
-                          const void*, const int*, const int*, void*, void*, int64_t, int, int, int, int, int, int);
+                          const void*, const int*, const int*, void*, void*, int64_t, int, int, int, int, int, int,
+                          int, int);
 
 template <class T, int KL, int WU>
 void launch_solve_t(dim3 g, dim3 b, hipStream_t s, const void* Uf, const void* Lf, const int* piv, const void* Vt,
                     void* Wt, const void* St, const int* stpiv, const int* perm, void* ws, void* x, int64_t rs, int B,
-                    int n, int r, int US, int nrhs, int prep) {
+                    int n, int r, int US, int nrhs, int prep, int il, int ucol) {
     hipLaunchKernelGGL((k_solve_t<T, KL, WU>), g, b, 0, s, static_cast<const T*>(Uf), static_cast<const T*>(Lf),
                        piv, static_cast<const T*>(Vt), static_cast<T*>(Wt), static_cast<const T*>(St), stpiv, perm,
-                       static_cast<T*>(ws), static_cast<T*>(x), rs, B, n, r, US, nrhs, prep);
+                       static_cast<T*>(ws), static_cast<T*>(x), rs, B, n, r, US, nrhs, prep, il, ucol);
 }
 
 template <class T, int KL>
@@ -1132,6 +1571,8 @@ struct cn_bb {
     int dtype = CN_C128, B = 0, Bp = 0, n = 0, r = 0, kl = 0, ku = 0, KLP = 0, M = 0, WU = 0, US = 0, G = 0;
     int pivot = 1;
     int streamed = 1;   // 0: always use the direct solve kernel (CNHIP_SOLVE=direct)
+    int mode = 0;       // 0: lane mode (one lane per system and rhs), 1: wave mode (one wavefront each)
+    int il = 16;        // interleave of the factor arrays: 16 systems (lane mode) or 1 (wave mode)
     int nrhs_cap = 0;
     size_t esz = 16;
     double cond_max = 1.0, factor_ms = 0.0;
@@ -1149,9 +1590,9 @@ template <class T>
 void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, const T* dv,
                 const std::vector<int>& perm, const std::vector<int>& pos, std::vector<T>& Ab, std::vector<T>& Vt) {
     const int n = h.n, r = h.r, M = h.M, KLP = h.KLP;
-    const size_t ch = m / IL, ml = m % IL;
-    auto AB = [&](int i, int d) -> T& { return Ab[(ch * ((size_t)n * M) + (size_t)i * M + d) * IL + ml]; };
-    auto VT = [&](int a, int j) -> T& { return Vt[(ch * ((size_t)r * n) + (size_t)a * n + j) * IL + ml]; };
+    const size_t il = h.il, ch = m / il, ml = m % il;
+    auto AB = [&](int i, int d) -> T& { return Ab[(ch * ((size_t)n * M) + (size_t)i * M + d) * il + ml]; };
+    auto VT = [&](int a, int j) -> T& { return Vt[(ch * ((size_t)r * n) + (size_t)a * n + j) * il + ml]; };
     for (int i = 0; i < n; ++i) {
         const int o = perm[i];
         if (i < r) {
@@ -1174,6 +1615,10 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
     (void)pos;
 }
 
+// wave mode: one staging buffer, and the whole LDS of a wavefront ([x: n entries][2 buffers])
+int wave_buf(const cn_bb& h) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz); }
+size_t wave_lds_bytes(const cn_bb& h) { return 2 * (size_t)align16(h.n * (int)h.esz) + 16 + 3 * (size_t)wave_buf(h); }
+
 // LDS bytes of one buffer of the streamed border completion, or -1 if it does not apply
 int border_bufbytes(const cn_bb& h) {
     if (!h.streamed || h.WU > 16) return -1;
@@ -1191,14 +1636,35 @@ void run_factor(cn_bb& h, hipStream_t s) {
     if (!f) fail(CN_ELOGIC, "no factor kernel for kl " + std::to_string(h.KLP));
     // band LU, with the forward half L^-1 P_B U of W in the r extra lanes
     f(fg, fb, s, h.Ab.p, h.Uf.p, h.Lf.p, static_cast<int*>(h.piv.p), h.W.p, static_cast<int*>(h.info.p), h.B, h.n,
-      h.M, h.r, h.G, h.US, h.pivot);
+      h.M, h.r, h.G, h.US, h.pivot, h.il, h.mode);
     hip_check(hipGetLastError(), "k_band_lu launch");
+    if (h.mode == 1) {
+        const size_t tot = (size_t)h.B * h.n * h.US;
+        hipLaunchKernelGGL((k_scale_ucol<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                           static_cast<T*>(h.Uf.p), tot, h.US);
+        hip_check(hipGetLastError(), "k_scale_ucol launch");
+    }
     h.tr_ready = false;   // the transposed Woodbury data follow the factors (built on demand)
     if (h.r == 0) return;
     // W = U^-1 (L^-1 P_B U) in place and S = I + V^T W, then the LU of S
-    const int bb = border_bufbytes(h);
+    if (h.mode == 1) {
+        const size_t lds = wave_lds_bytes(h);
+        if (h.dtype == CN_C128)
+            hipLaunchKernelGGL((k_border_w<zc>), dim3(h.B, h.r), dim3(64), lds, s, static_cast<const zc*>(h.Uf.p),
+                               static_cast<const zc*>(h.Vt.p), static_cast<zc*>(h.W.p), static_cast<zc*>(h.S.p), h.n,
+                               h.r, h.M, h.US, wave_buf(h));
+        else
+            hipLaunchKernelGGL((k_border_w<double>), dim3(h.B, h.r), dim3(64), lds, s,
+                               static_cast<const double*>(h.Uf.p), static_cast<const double*>(h.Vt.p),
+                               static_cast<double*>(h.W.p), static_cast<double*>(h.S.p), h.n, h.r, h.M, h.US,
+                               wave_buf(h));
+        hip_check(hipGetLastError(), "k_border_w launch");
+    }
+    const int bb = h.mode == 1 ? -1 : border_bufbytes(h);
     BorderSFn fs = bb > 0 ? border_s_fn<T>(h.WU, nr_for(h.r)) : nullptr;
-    if (fs) {
+    if (h.mode == 1) {
+        // done above
+    } else if (fs) {
         const int nr = nr_for(h.r);
         fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(64), 2 * (size_t)bb, s, h.Uf.p, h.Vt.p, h.W.p, h.S.p,
            h.B, h.n, h.r, h.US, bb);
@@ -1225,11 +1691,11 @@ void run_prepare_t(cn_bb& h, hipStream_t s) {
         const int lanes = h.B * h.r;
         ft(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.Wt.p,
            h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), nullptr, nullptr, 0, h.B, h.n,
-           h.r, h.US, h.r, 1);
+           h.r, h.US, h.r, 1, h.il, h.mode);
         hip_check(hipGetLastError(), "k_solve_t (Wt) launch");
         const int l2 = h.B * h.r * h.r;
         hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
-                           static_cast<T*>(h.St.p), static_cast<int*>(h.info_t.p), h.B, h.n, h.r);
+                           static_cast<T*>(h.St.p), static_cast<int*>(h.info_t.p), h.B, h.n, h.r, h.il);
         hip_check(hipGetLastError(), "k_build_st launch");
         hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.St.p),
                            static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
@@ -1350,6 +1816,22 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     if (h.WU < 0) fail(CN_EUNSUPPORTED, "BandedBatch: kl+ku = " + std::to_string(h.KLP + ku) + " exceeds 32");
     h.US = h.WU + 1;
     h.Bp = (B + 63) / 64 * 64;   // whole 64-system workgroups of the streamed solve stay in bounds
+    // kernel mode: lane mode (one lane per system and right-hand side, streamed sweeps) for many
+    // narrow-band systems; wave mode (one wavefront per system and right-hand side) for few or
+    // wide-band systems.  CNHIP_MODE=lane|wave forces one where it applies.
+    {
+        const int e = (int)sizeof(T);
+        const int lane_buf = std::max({fwd_bytes(h.KLP, e), bwd_bytes(h.US, r, e), cor_bytes(r, e)});
+        const bool lane_fast = h.KLP + 1 <= SP && h.WU <= 16 && 2 * lane_buf <= LDS_CAP;
+        const bool wave_ok =
+            2 * (size_t)align16(n * e) + 16 + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e) <= (size_t)LDS_CAP;
+        const char* em = std::getenv("CNHIP_MODE");
+        const std::string want = em ? em : "";
+        if (want == "wave") h.mode = wave_ok ? 1 : 0;
+        else if (want == "lane") h.mode = 0;
+        else h.mode = (lane_fast && B >= 1024) || !wave_ok ? 0 : 1;
+        h.il = h.mode == 1 ? 1 : IL;
+    }
     const size_t es = sizeof(T);
     const size_t nAb = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.M});
     const size_t nU = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.US});
@@ -1414,7 +1896,7 @@ void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
     const int lanes = h.B * nrhs;
     f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.Wt.p,
       h.St.p, static_cast<const int*>(h.stpiv.p), static_cast<const int*>(h.perm.p), h.ws.p, x, rs, h.B, h.n, h.r,
-      h.US, nrhs, 0);
+      h.US, nrhs, 0, h.il, h.mode);
     hip_check(hipGetLastError(), "k_solve_t launch");
 }
 
@@ -1436,6 +1918,24 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
             fail(CN_EINVAL, "BandedBatch solve: work space for " + std::to_string(nrhs) +
                                 " right-hand sides not reserved before graph capture (call cn_bb_reserve)");
         reserve_impl(h, nrhs);
+    }
+    if (h.mode == 1) {
+        const size_t lds = wave_lds_bytes(h);
+        if (h.dtype == CN_C128)
+            hipLaunchKernelGGL((k_solve_w<zc>), dim3(h.B, nrhs), dim3(64), lds, s, static_cast<const zc*>(h.Uf.p),
+                               static_cast<const zc*>(h.Lf.p), static_cast<const int*>(h.piv.p),
+                               static_cast<const zc*>(h.Vt.p), static_cast<const zc*>(h.W.p),
+                               static_cast<const zc*>(h.S.p), static_cast<const int*>(h.spiv.p),
+                               static_cast<const int*>(h.perm.p), io, h.n, h.r, h.KLP, h.M, h.US, wave_buf(h));
+        else
+            hipLaunchKernelGGL((k_solve_w<double>), dim3(h.B, nrhs), dim3(64), lds, s,
+                               static_cast<const double*>(h.Uf.p), static_cast<const double*>(h.Lf.p),
+                               static_cast<const int*>(h.piv.p), static_cast<const double*>(h.Vt.p),
+                               static_cast<const double*>(h.W.p), static_cast<const double*>(h.S.p),
+                               static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), io, h.n, h.r,
+                               h.KLP, h.M, h.US, wave_buf(h));
+        hip_check(hipGetLastError(), "k_solve_w launch");
+        return;
     }
     const int bufbytes = streamed_bufbytes(h, io);
     if (bufbytes > 0 && solve_s_fn<double>(h.KLP, h.WU, 1)) {
@@ -1618,6 +2118,7 @@ int cn_bb_props_get(cn_bb_t h, cn_bb_props* p) {
         p->transpose_ready = h->tr_ready ? 1 : 0;
         p->cond_max = h->cond_max;
         p->factor_ms = h->factor_ms;
+        p->solve_mode = h->mode;
         p->factor_bytes = (int64_t)(h->Uf.bytes + h->Lf.bytes + h->piv.bytes + h->Vt.bytes + h->W.bytes + h->S.bytes +
                                     h->spiv.bytes);
     });

@@ -4,15 +4,16 @@
 // 3 outputs, derivative chains, backward transforms, products, forward transforms, C^(lam)
 // conversion into the stacked rows).
 //
-// Four launches, each an HBM/L2-streaming pass with one thread per output entry and the
-// transform expressed as a small dense matrix applied along one axis (so no transpose pass):
-//   A  z-backward: Z[q][k][j] = sum_n Bz_v[n][j] X[k][pos_f(n)]     (q = (f, v): u, uz, w, wz, T, Tz;
-//      the z-derivative is folded into the matrix: Bz_1 = T_n'(z_j))
-//   B  x-backward + products, per grid point (i, j): the 8 factors u, ux, uz, w, wx, wz, Tx, Tz
-//      are accumulated over k in registers and only the 3 products are written
-//   C  x-forward: Gh[o][k][j] = sum_i Fx[k][i] G[o][i][j]             (keeps k < K: dealiasing)
-//   D  z-forward + conversion + scatter: F[k][R + 4n + e] = sum_j Az[j][n] Gh[e][k][j],
-//      Az = (S_{0->2} Fz)^T  (keeps n < N)
+// Two launches:
+//   A  one workgroup per grid line z_j (Mz workgroups), everything in LDS:
+//      z-backward of the 6 factor columns of the modes (u, u_z, w, w_z, T, T_z; the z-derivative
+//      is folded into the matrix Bz_1 = T_n'(z_j)), x-backward of the 8 factors u, u_x, u_z, w,
+//      w_x, w_z, T_x, T_z at the Mx points of the line (x-derivative i k kappa on the modes), the 3
+//      products, and the x-forward transform of the products (modes k < K: dealiasing).  The
+//      Fourier weights come from one table of Mx twiddles W_m = exp(2 pi i m / Mx)
+//      (e^{i k kappa x_i} = W_{k i mod Mx}).  Output Gh[o][k][j].
+//   B  one workgroup per mode k: z-forward + S_{0->2} conversion of the 3 products,
+//      F[k][R + 4n + e] = sum_j Az[j][n] Gh[e][k][j], Az = (S_{0->2} Fz)^T (keeps n < Nz).
 // Grids: Mx = floor(3/2 Nx), Mz = floor(3/2 Nz) (exact dealiasing of quadratic products).
 #include <hip/hip_runtime.h>
 
@@ -27,110 +28,120 @@ using cn::zc;
 namespace {
 
 __global__ void __launch_bounds__(256)
-k_fc_zback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz, zc* __restrict__ Z,
-           int K, int n, int N, int Mz) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = 6 * K * Mz;
-    if (t >= total) return;
-    const int j = t % Mz;
-    const int k = (t / Mz) % K;
-    const int q = t / (Mz * K);
-    const int f = q >> 1, v = q & 1;
-    const zc* Xk = X + (size_t)k * n;
-    const int* pos = fidx + (size_t)f * N;
-    const double* B = Bz + (size_t)v * N * Mz + j;
-    double re = 0.0, im = 0.0;
-    for (int nn = 0; nn < N; ++nn) {
-        const double b = B[(size_t)nn * Mz];
-        const zc x = Xk[pos[nn]];
-        re = fma(b, x.re, re);
-        im = fma(b, x.im, im);
+k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
+          const zc* __restrict__ Wt, zc* __restrict__ Gh, int K, int n, int N, int Mz, int Mx, double kappa) {
+    extern __shared__ __align__(16) char lds[];
+    zc* Zj = reinterpret_cast<zc*>(lds);            // [6][K]: (u, uz, w, wz, T, Tz) at z_j per mode
+    zc* W = Zj + 6 * K;                              // [Mx] twiddles
+    double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx] products on the line
+    double* b = G + 3 * Mx;                          // [2][N]: T_n(z_j), T_n'(z_j)
+    const int j = blockIdx.x, tid = threadIdx.x;
+    for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wt[q];
+    for (int q = tid; q < 2 * N; q += blockDim.x) {
+        const int v = q / N, nn = q - v * N;
+        b[q] = Bz[((size_t)v * N + nn) * Mz + j];
     }
-    Z[t] = zc{re, im};
+    __syncthreads();
+    // z-backward: Zj[2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)]
+    for (int p = tid; p < 3 * K; p += blockDim.x) {
+        const int f = p / K, k = p - f * K;
+        const zc* Xk = X + (size_t)k * n;
+        const int* pos = fidx + (size_t)f * N;
+        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+        for (int nn = 0; nn < N; ++nn) {
+            const zc x = Xk[pos[nn]];
+            const double b0 = b[nn], b1 = b[N + nn];
+            r0 = fma(b0, x.re, r0);
+            i0 = fma(b0, x.im, i0);
+            r1 = fma(b1, x.re, r1);
+            i1 = fma(b1, x.im, i1);
+        }
+        Zj[(2 * f) * K + k] = zc{r0, i0};
+        Zj[(2 * f + 1) * K + k] = zc{r1, i1};
+    }
+    __syncthreads();
+    // x-backward of the 8 factors at x_i and the products
+    for (int i = tid; i < Mx; i += blockDim.x) {
+        double u = 0, ux = 0, uz = 0, w = 0, wx = 0, wz = 0, Tx = 0, Tz = 0;
+        int m = 0;                                   // k i mod Mx
+        for (int k = 0; k < K; ++k) {
+            const zc t = W[m];
+            const double wk = k == 0 ? 1.0 : 2.0;
+            const zc e{wk * t.re, wk * t.im};
+            const double kk = k * kappa;
+            const zc zu = Zj[k], zuz = Zj[K + k], zw = Zj[2 * K + k], zwz = Zj[3 * K + k], zT = Zj[4 * K + k],
+                     zTz = Zj[5 * K + k];
+            // p = e z: Re(p) is the value, -k kappa Im(p) the x-derivative
+            const double pu_r = fma(e.re, zu.re, -(e.im * zu.im)), pu_i = fma(e.re, zu.im, e.im * zu.re);
+            const double pw_r = fma(e.re, zw.re, -(e.im * zw.im)), pw_i = fma(e.re, zw.im, e.im * zw.re);
+            const double pT_i = fma(e.re, zT.im, e.im * zT.re);
+            u += pu_r;
+            ux = fma(-kk, pu_i, ux);
+            uz += fma(e.re, zuz.re, -(e.im * zuz.im));
+            w += pw_r;
+            wx = fma(-kk, pw_i, wx);
+            wz += fma(e.re, zwz.re, -(e.im * zwz.im));
+            Tx = fma(-kk, pT_i, Tx);
+            Tz += fma(e.re, zTz.re, -(e.im * zTz.im));
+            m += i;
+            if (m >= Mx) m -= Mx;
+        }
+        G[i] = -fma(u, ux, w * uz);
+        G[Mx + i] = -fma(u, wx, w * wz);
+        G[2 * Mx + i] = -fma(u, Tx, w * Tz);
+    }
+    __syncthreads();
+    // x-forward: Gh[o][k][j] = (1/Mx) sum_i conj(W_{k i mod Mx}) G[o][i]
+    const double inv = 1.0 / Mx;
+    for (int p = tid; p < 3 * K; p += blockDim.x) {
+        const int o = p / K, k = p - o * K;
+        const double* g = G + (size_t)o * Mx;
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int i = 0; i < Mx; ++i) {
+            const zc t = W[m];
+            const double gi = g[i] * inv;
+            re = fma(t.re, gi, re);
+            im = fma(-t.im, gi, im);
+            m += k;
+            if (m >= Mx) m -= Mx;
+        }
+        Gh[((size_t)o * K + k) * Mz + j] = zc{re, im};
+    }
 }
 
 __global__ void __launch_bounds__(256)
-k_fc_xback_products(const zc* __restrict__ Z, const zc* __restrict__ Ex, double* __restrict__ G, int K, int Mz,
-                    int Mx, double kappa) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Mx * Mz) return;
-    const int j = t % Mz;
-    const int i = t / Mz;
-    const size_t qs = (size_t)K * Mz;   // stride between the 6 z-backward arrays
-    double u = 0, ux = 0, uz = 0, w = 0, wx = 0, wz = 0, Tx = 0, Tz = 0;
-    const zc* Er = Ex + (size_t)i * K;
-    for (int k = 0; k < K; ++k) {
-        const zc e = Er[k];
-        const double kk = k * kappa;
-        const zc* Zk = Z + (size_t)k * Mz + j;
-        const zc zu = Zk[0], zuz = Zk[qs], zw = Zk[2 * qs], zwz = Zk[3 * qs], zT = Zk[4 * qs], zTz = Zk[5 * qs];
-        // p = e z: Re(p) is the value, -k kappa Im(p) the x-derivative
-        const double pu_r = fma(e.re, zu.re, -(e.im * zu.im)), pu_i = fma(e.re, zu.im, e.im * zu.re);
-        const double pw_r = fma(e.re, zw.re, -(e.im * zw.im)), pw_i = fma(e.re, zw.im, e.im * zw.re);
-        const double pT_i = fma(e.re, zT.im, e.im * zT.re);
-        u += pu_r;
-        ux = fma(-kk, pu_i, ux);
-        uz += fma(e.re, zuz.re, -(e.im * zuz.im));
-        w += pw_r;
-        wx = fma(-kk, pw_i, wx);
-        wz += fma(e.re, zwz.re, -(e.im * zwz.im));
-        Tx = fma(-kk, pT_i, Tx);
-        Tz += fma(e.re, zTz.re, -(e.im * zTz.im));
-    }
-    const size_t P = (size_t)Mx * Mz;
-    G[t] = -fma(u, ux, w * uz);
-    G[P + t] = -fma(u, wx, w * wz);
-    G[2 * P + t] = -fma(u, Tx, w * Tz);
-}
-
-__global__ void __launch_bounds__(256)
-k_fc_xfwd(const double* __restrict__ G, const zc* __restrict__ Fx, zc* __restrict__ Gh, int K, int Mz, int Mx) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * K * Mz) return;
-    const int j = t % Mz;
-    const int k = (t / Mz) % K;
-    const int o = t / (Mz * K);
-    const double* Go = G + (size_t)o * Mx * Mz + j;
-    const zc* F = Fx + (size_t)k * Mx;
-    double re = 0.0, im = 0.0;
-    for (int i = 0; i < Mx; ++i) {
-        const double g = Go[(size_t)i * Mz];
-        const zc f = F[i];
-        re = fma(f.re, g, re);
-        im = fma(f.im, g, im);
-    }
-    Gh[t] = zc{re, im};
-}
-
-__global__ void __launch_bounds__(256)
-k_fc_zfwd_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __restrict__ F, int K, int Mz, int N,
+k_fc_mode_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __restrict__ F, int K, int Mz, int N,
                int n, int R) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * K * N) return;
-    const int nn = t % N;
-    const int k = (t / N) % K;
-    const int e = t / (N * K);
-    const zc* g = Gh + ((size_t)e * K + k) * Mz;
-    double re = 0.0, im = 0.0;
-    for (int j = 0; j < Mz; ++j) {
-        const double a = Az[(size_t)j * N + nn];
-        const zc v = g[j];
-        re = fma(a, v.re, re);
-        im = fma(a, v.im, im);
+    extern __shared__ __align__(16) char lds[];
+    zc* g = reinterpret_cast<zc*>(lds);              // [3][Mz]: Gh[e][k][:]
+    const int k = blockIdx.x, tid = threadIdx.x;
+    for (int q = tid; q < 3 * Mz; q += blockDim.x) {
+        const int e = q / Mz, jj = q - e * Mz;
+        g[q] = Gh[((size_t)e * K + k) * Mz + jj];
     }
-    F[(size_t)k * n + R + 4 * nn + e] = zc{re, im};
+    __syncthreads();
+    for (int p = tid; p < 3 * N; p += blockDim.x) {
+        const int e = p / N, nn = p - e * N;
+        const zc* ge = g + (size_t)e * Mz;
+        double re = 0.0, im = 0.0;
+        for (int jj = 0; jj < Mz; ++jj) {
+            const double a = Az[(size_t)jj * N + nn];
+            const zc v = ge[jj];
+            re = fma(a, v.re, re);
+            im = fma(a, v.im, im);
+        }
+        F[(size_t)k * n + R + 4 * nn + e] = zc{re, im};
+    }
 }
-
-dim3 g256(size_t N) { return dim3((unsigned)((N + 255) / 256)); }
 
 }  // namespace
 
 extern "C" {
 
-int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Z, void* G, void* Gh, const double* Bz,
-                     const void* Ex, const void* Fx, const double* Az, int K, int Nz, int Mx, int Mz, int n, int R,
-                     double kappa, void* stream) {
-    if (!X || !fidx || !F || !Z || !G || !Gh || !Bz || !Ex || !Fx || !Az) {
+int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
+                     const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream) {
+    if (!X || !fidx || !F || !Gh || !Bz || !Wx || !Az) {
         cn::set_error("cn_rb2d_explicit: null pointer");
         return CN_EINVAL;
     }
@@ -138,14 +149,16 @@ int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Z, void* G, 
         cn::set_error("cn_rb2d_explicit: inconsistent sizes");
         return CN_EINVAL;
     }
+    const size_t lds_a = (size_t)(6 * K + Mx) * sizeof(zc) + (size_t)(3 * Mx + 2 * Nz) * sizeof(double);
+    const size_t lds_b = (size_t)3 * Mz * sizeof(zc);
+    if (lds_a > 64 * 1024 || lds_b > 64 * 1024) {
+        cn::set_error("cn_rb2d_explicit: grid too large for the line kernels (LDS)");
+        return CN_EUNSUPPORTED;
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_fc_zback, g256((size_t)6 * K * Mz), dim3(256), 0, s, static_cast<const zc*>(X), fidx, Bz,
-                       static_cast<zc*>(Z), K, n, Nz, Mz);
-    hipLaunchKernelGGL(k_fc_xback_products, g256((size_t)Mx * Mz), dim3(256), 0, s, static_cast<const zc*>(Z),
-                       static_cast<const zc*>(Ex), static_cast<double*>(G), K, Mz, Mx, kappa);
-    hipLaunchKernelGGL(k_fc_xfwd, g256((size_t)3 * K * Mz), dim3(256), 0, s, static_cast<const double*>(G),
-                       static_cast<const zc*>(Fx), static_cast<zc*>(Gh), K, Mz, Mx);
-    hipLaunchKernelGGL(k_fc_zfwd_rows, g256((size_t)3 * K * Nz), dim3(256), 0, s, static_cast<const zc*>(Gh), Az,
+    hipLaunchKernelGGL(k_fc_line, dim3(Mz), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
+                       static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
+    hipLaunchKernelGGL(k_fc_mode_rows, dim3(K), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
                        static_cast<zc*>(F), K, Mz, Nz, n, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -90,6 +90,7 @@ class BandedBatch:
         self.capacitance_cond_max = float(p.cond_max)
         self.col_border = bool(col_border)
         self.kernel = (p.kl_kernel, p.wu_kernel, p.lanes)
+        self.solve_mode = "wave" if p.solve_mode == 1 else "lane"
         self.factor_bytes = int(p.factor_bytes)
 
     level_scheduled = False
@@ -214,5 +215,7 @@ class BandedBatch:
         return (f"hip.BandedBatch(batch={self.batch}, n={self.n}, common band kl={self.kl} ku={self.ku}, "
                 f"border rank={self.rank}, pivot={'partial (in band)' if self.pivot else 'none (declared)'}, "
                 f"capacitance cond_1 max={self.capacitance_cond_max:.3g}, factor {self.factor_ms:.3g} ms); "
-                f"gfx950: factor {lanes} lanes per system (kernel kl={kl}), solve one lane per "
-                f"(system, rhs) over 16-system interleaved factors (window {wu})")
+                f"gfx950: factor {lanes} lanes per system (kernel kl={kl}), "
+                + ("solve one lane per (system, rhs) over 16-system interleaved factors (window "
+                   f"{wu}), streamed through LDS" if self.solve_mode == "lane" else
+                   "solve one wavefront per (system, rhs), band window across lanes, U stored by columns"))

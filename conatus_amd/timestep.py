@@ -36,12 +36,9 @@ class RB2DExplicit:
         dev = "cuda"
         Bz = np.stack([S.cheb_backward_matrix(N, Mz, 0, pb.Lz).T, S.cheb_backward_matrix(N, Mz, 1, pb.Lz).T])
         self.Bz = torch.from_numpy(np.ascontiguousarray(Bz)).to(dev)                       # (2, N, Mz)
-        self.Ex = torch.from_numpy(np.ascontiguousarray(S.fourier_backward_matrix(K, Mx, 0, pb.Lx))).to(dev)
-        self.Fx = torch.from_numpy(np.ascontiguousarray(S.fourier_forward_matrix(K, Mx))).to(dev)
+        self.Wx = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mx) / Mx)).to(dev)        # twiddles
         Az = (pb.S02 @ S.cheb_forward_matrix(N, Mz)).T
         self.Az = torch.from_numpy(np.ascontiguousarray(Az)).to(dev)                       # (Mz, N)
-        self.Z = torch.empty(6 * K * Mz, dtype=torch.complex128, device=dev)
-        self.G = torch.empty(3 * Mx * Mz, dtype=torch.float64, device=dev)
         self.Gh = torch.empty(3 * K * Mz, dtype=torch.complex128, device=dev)
         self._base = torch.from_numpy(pb.b0().ravel()).to(dev)
         fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
@@ -54,9 +51,9 @@ class RB2DExplicit:
         pb = self.pb
         v = ctypes.c_void_p
         p = lambda t: v(t.data_ptr())   # noqa: E731
-        st = _lib.lib().cn_rb2d_explicit(p(x), p(self.fidx), p(out), p(self.Z), p(self.G), p(self.Gh), p(self.Bz), p(self.Ex),
-                                         p(self.Fx), p(self.Az), pb.K, pb.N, pb.Mx, pb.Mz, pb.n, NBORDER,
-                                         float(pb.kappa), v(_torch().cuda.current_stream().cuda_stream))
+        st = _lib.lib().cn_rb2d_explicit(p(x), p(self.fidx), p(out), p(self.Gh), p(self.Bz), p(self.Wx), p(self.Az),
+                                         pb.K, pb.N, pb.Mx, pb.Mz, pb.n, NBORDER, float(pb.kappa),
+                                         v(_torch().cuda.current_stream().cuda_stream))
         _lib.check(st, "RB2DExplicit")
 
 

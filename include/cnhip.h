@@ -53,6 +53,7 @@ typedef struct cn_bb_props {
     double cond_max;    /* max over the batch of cond_1 of the capacitance matrix */
     double factor_ms;   /* device time of the last factorization */
     int64_t factor_bytes; /* device bytes held by the factors */
+    int32_t solve_mode;   /* 0: one lane per (system, rhs), streamed sweeps; 1: one wavefront per (system, rhs) */
 } cn_bb_props;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -159,12 +160,11 @@ int cn_pointwise(int dtype, int64_t N, void* out, const void* x, const void* g, 
  * pipeline FourierChebyshev2DRHS.evaluate (python/conatus/timestep/fc2d.py:567-643) and the fused
  * product evaluation (python/conatus/timestep/fused.py:994-1085).  Layout: coefficient m of field f (u, w, T) of
  * mode k at X[k*n + fidx[f*Nz + m]] (fidx: 3 x Nz device ints), equation e rows at F[k*n + R + 4*m + e]; other rows of F are left
- * untouched.  Scratch: Z (6 K Mz complex), G (3 Mx Mz real), Gh (3 K Mz complex).  Matrices:
- * Bz (2 x Nz x Mz: T_n, T_n' at the nodes), Ex (Mx x K complex backward Fourier weights),
- * Fx (K x Mx complex forward), Az (Mz x Nz: (S02 Fz)^T). */
-int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Z, void* G, void* Gh, const double* Bz,
-                     const void* Ex, const void* Fx, const double* Az, int K, int Nz, int Mx, int Mz, int n, int R,
-                     double kappa, void* stream);
+ * untouched.  Scratch: Gh (3 K Mz complex).  Tables: Bz (2 x Nz x Mz: T_n, T_n' at the nodes),
+ * Wx (Mx complex twiddles exp(2 pi i m / Mx)), Az (Mz x Nz: (S02 Fz)^T).  Needs
+ * (6 K + Mx) 16 + (3 Mx + 2 Nz) 8 <= 65536 bytes of LDS (CN_EUNSUPPORTED otherwise). */
+int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
+                     const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream);
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,10 @@
 Bar: normwise backward error <= 1e-13 (north-star tolerance for the solve), the residual within
 10x the oracle's own residual plus n eps (the reference's cross-backend rule,
 tests/cross_backend/test_banded_level.py:71-76), solutions within a conditioning-scaled distance of
-the oracle's, fused gather/scatter and split-complex maps exact, bitwise determinism."""
+the oracle's, fused gather/scatter and split-complex maps exact, bitwise determinism.  Every test
+runs in both solve modes of the library (CNHIP_MODE=lane: one lane per system and right-hand side
+over 16-system interleaved factors; CNHIP_MODE=wave: one wavefront per system and right-hand side,
+U stored by columns)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -15,6 +18,13 @@ from tests.banded_cases import (almost_banded, backward_error, bordered_case, co
 pytestmark = pytest.mark.gpu
 
 BE_TOL = 1e-13
+
+
+@pytest.fixture(autouse=True, params=["lane", "wave"])
+def solve_mode(request, monkeypatch):
+    monkeypatch.setenv("CNHIP_MODE", request.param)
+    print(f"solve mode {request.param}", flush=True)
+    return request.param
 
 
 def _torch():
@@ -82,7 +92,7 @@ def test_batch_matches_oracle(dtype, r, rng):
 
 @pytest.mark.parametrize("kl,ku", [(1, 0), (2, 7), (3, 3), (5, 9), (7, 1), (10, 14), (14, 17), (16, 16)])
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
-def test_band_widths_cover_every_kernel(kl, ku, dtype, rng):
+def test_band_widths_cover_every_kernel(kl, ku, dtype, rng, solve_mode):
     """Every compiled (kl, wu) bucket, with pivoting exercised (no diagonal dominance); the
     reference's relative rule (tests/cross_backend/test_banded_level.py:71-76)."""
     from conatus_amd.runtime.hip import BandedBatch, to_device
@@ -94,6 +104,7 @@ def test_band_widths_cover_every_kernel(kl, ku, dtype, rng):
         A[np.arange(r, n), np.arange(r, n)] += 0.5          # pivots still swap rows
         mats.append(sp.csr_matrix(A))
     bb = BandedBatch(mats, list(range(r)))
+    assert bb.solve_mode == solve_mode
     b = _rhs(rng, (B, n), dtype)
     d = to_device(b)
     bb.solve(d)
@@ -335,3 +346,17 @@ def test_core_shim_signatures(rng):
     assert np.array_equal(xs[0], x[0]) and s.rank == 2
     with pytest.raises(ValueError):
         bb.solve(b[:3])
+
+
+def test_auto_mode_selection(rng, monkeypatch, solve_mode):
+    """Without CNHIP_MODE: wave mode for a few systems, lane mode for >= 1024 narrow-band ones."""
+    if solve_mode != "lane":
+        pytest.skip("mode selection is checked once")
+    monkeypatch.delenv("CNHIP_MODE")
+    from conatus_amd.runtime.hip import BandedBatch
+    few = BandedBatch([conditioned(rng, 64, 2, 3, 2, dtype=complex) for _ in range(3)], [0, 1])
+    assert few.solve_mode == "wave"
+    A = conditioned(rng, 32, 2, 2, 2, dtype=complex)
+    many = BandedBatch([A] * 1024, [0, 1])
+    assert many.solve_mode == "lane"
+    assert "lane per (system, rhs)" in many.explain() and "wavefront per (system, rhs)" in few.explain()

@@ -6,7 +6,9 @@ systems, N = 256, band kl = ku = 4 (N(0,1) + i N(0,1) entries, diagonal + 8), tw
 rows (T_n(1) = 1 and T_n(-1) = (-1)^n), 4 seeded N(0,1) complex right-hand sides per system.
 A step is one batched solve of the 4 x 4096 right-hand sides (the solve kernel, factors and data
 resident in HBM); the factorization is timed separately.  Accuracy gate before timing: every
-system's normwise backward error <= 1e-13.
+system's normwise backward error <= 1e-13.  About 0.3 s of untimed solves precede the warm-up
+steps so that the timed steps run at the GPU's working clock.  A second leg reports IMEX steps/s
+of 2D Rayleigh-Benard at 128 x 64 (imex_rb2d).
 
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
@@ -102,8 +104,17 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     if not gate <= 1e-11:
         raise RuntimeError(f"rb2d accuracy gate failed ({gate:.3e})")
     s = IVPSolver(pb, "SBDF2", 1e-4)
+    # warm-up: replays from the initial state for about half a second (the GPU clock ramps up
+    # over the first ~0.1 s of work), both ring phases captured
+    tw = time.perf_counter()
+    while True:
+        s.set_state(X0)
+        s.run_graph(max(warmup, 100))
+        torch.cuda.synchronize()
+        if time.perf_counter() - tw > 0.5:
+            break
     s.set_state(X0)
-    s.run_graph(max(warmup, 4))          # startup order 1, then both ring phases captured
+    s.run_graph(4)                       # startup order 1, then full-order graph replays
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -197,7 +208,14 @@ def main() -> int:
         log("accuracy gate FAILED: no timing reported")
         return 1
 
-    # factor timing (k_band_lu + k_border + k_cap_lu from the resident layout)
+    # clock warm-up (about 0.3 s of solves: the GPU clock ramps up over the first ~0.1 s of work)
+    tw = time.perf_counter()
+    while time.perf_counter() - tw < 0.3:
+        for _ in range(20):
+            h.solve_ptr(d.data_ptr(), NRHS, B * N, False, sptr)
+        torch.cuda.synchronize()
+
+    # factor timing (k_band_lu, border completion k_border_s, k_cap_lu from the resident layout)
     for _ in range(args.warmup):
         h.factor(sptr, check=False)
     torch.cuda.synchronize()

@@ -90,6 +90,12 @@ def lib():
                                        where=str(LIB_PATH),
                                        fix="python -m conatus_amd.build (or __graft_entry__.build())")
                 try:
+                    # torch ships its own HIP runtime; loading it first makes libcnhip bind to the same
+                    # runtime (same soname), so that both see the device and share streams.
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
+                try:
                     l = ctypes.CDLL(str(LIB_PATH))
                 except OSError as e:
                     raise BackendError(f"cannot load {LIB_PATH}: {e}") from e

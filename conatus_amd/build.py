@@ -39,16 +39,33 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile the library (always: a stale .so must not survive a source change)."""
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, sources()), "-o", str(tmp)]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode}) building {LIB.name}")
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd[-3:])}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile the library (always: a stale .so must not survive a source change).  Every source is
+    compiled to an object in parallel (jobs, default: the usable cores), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    srcs = sources()
+    objs = [objdir / (p.stem + ".o") for p in srcs]
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    cmds = [[hipcc(), *flags, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+            for src, obj in zip(srcs, objs)]
+    njobs = jobs or max(1, min(len(cmds), len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(max_workers=njobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,1345 @@
+// libcnhip device code of the bordered-banded solver (kernels and device helpers), shared by
+// banded.hip and the lane-mode translation units banded_lane_*.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cn_common.h"
+
+namespace cnk {
+
+using cn::zc;
+
+constexpr int IL = 16;     // systems per interleave chunk
+constexpr int MAXR = 8;    // largest border rank handled by the kernels
+constexpr int LDS_CAP = 160 * 1024;   // LDS of one workgroup on gfx950 (163840 bytes)
+
+using SolveSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, const int*, const void*,
+                          const void*, const void*, const int*, const int*, void*, const void*, void*, int64_t, int,
+                          int, int, int, int);
+using BorderSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, void*, void*, int, int, int, int,
+                           int);
+
+// launch tables of the streamed (lane mode) kernels, one translation unit per (type, block length)
+SolveSFn lane_solve_zc16(int KL, int WU, int nr);
+SolveSFn lane_solve_zc8(int KL, int WU, int nr);
+SolveSFn lane_solve_d16(int KL, int WU, int nr);
+SolveSFn lane_solve_d8(int KL, int WU, int nr);
+BorderSFn lane_border_zc16(int WU, int nr);
+BorderSFn lane_border_zc8(int WU, int nr);
+BorderSFn lane_border_d16(int WU, int nr);
+BorderSFn lane_border_d8(int WU, int nr);
+
+// ------------------------------------------------------------------------------------------
+// Factor kernel: G lanes per system, 256/G systems per workgroup of 256 lanes.
+// Ab: unfactored band rows, row i holds A(i, i-KL+d), d = 0..M-1 (row stride M).
+// Uf: U rows, row j holds U(j, j+d), d = 0..US-1, d = 0 holding 1/U(j, j).
+// Lf: multipliers of step j, Lf(j, i) = l(j+1+i, j), i < KL.  piv: pivot offset 0..KL of step j.
+// Wf: forward half of W (L^-1 P_B e_k), rows k < r of length n.
+// ------------------------------------------------------------------------------------------
+template <class T, int KL>
+__global__ void __launch_bounds__(256)
+k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int* __restrict__ piv,
+          T* __restrict__ Wf, int* __restrict__ info, int B, int n, int M, int r, int G, int US,
+          int dopivot, int il, int ucol) {
+    const int t = threadIdx.x;
+    const int g = t / G;
+    const int c = t - g * G;
+    const int m = blockIdx.x * (256 / G) + g;
+    const bool live = m < B;
+    const size_t ch = m / il;
+    const int ml = m % il;
+    const T* A0 = Ab + ch * ((size_t)n * M * il) + ml;
+    T* U0 = Uf + ch * ((size_t)n * US * il) + ml;
+    T* L0 = Lf + ch * ((size_t)n * KL * il) + ml;
+    int* P0 = piv + ch * ((size_t)n * il) + ml;
+    T* W0 = Wf + ch * ((size_t)r * n * il) + ml;
+    const bool band = live && c < M;
+    const bool wl = live && c >= M && c < M + r;
+    const int k = c - M;
+    const int gbase = (t & 63) - c;   // wavefront lane of the group's lane 0 (G divides 64)
+
+    T win[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) {
+        T v = cn::zero<T>();
+        if (band && i < n) {
+            const int d = c - i + KL;
+            if (d >= 0 && d < M) v = A0[((size_t)i * M + d) * il];
+        }
+        if (wl) v = (i == k) ? cn::one<T>() : cn::zero<T>();
+        win[i] = v;
+    }
+    int first_singular = 0;
+    for (int j = 0; j < n; ++j) {
+        // win[i] holds row j+i of this lane's column
+        const int owner = j % M;
+        const int src = gbase + owner;
+        const bool is_owner = (c == owner);
+        // pivot search on the owner's column (first maximum of |re|+|im|, as i?amax)
+        double best = cn::cabs1(win[0]);
+        int ip = 0;
+        if (dopivot) {
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) {
+                const double v = cn::cabs1(win[i]);
+                if (v > best) { best = v; ip = i; }
+            }
+        }
+        ip = __shfl(ip, src, 64);
+        best = __shfl(best, src, 64);
+        if (ip != 0) cn::swap_first<T, KL>(win, ip);   // row interchange in every column of the window and in W
+        const bool singular = !(best != 0.0);
+        T rc = cn::recip(win[0]);
+        if (singular) rc = cn::zero<T>();
+        T l[KL];
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) l[i - 1] = cn::shfl(cn::mul(win[i], rc), src);
+        const int d = (c - owner + M) % M;   // column offset of this lane's column from j
+        if (band) {
+            T ov = win[0];
+            if (is_owner) ov = rc;
+            if (!ucol) U0[((size_t)j * US + d) * il] = ov;          // row j of U: U(j, j+d)
+            else if (j + d < n) U0[((size_t)(j + d) * US + d) * il] = ov;   // column j+d: U(j, j+d)
+        }
+        if (wl) W0[((size_t)k * n + j) * il] = win[0];
+        if (band && is_owner) {
+#pragma unroll
+            for (int i = 0; i < KL; ++i) L0[((size_t)j * KL + i) * il] = l[i];
+            P0[(size_t)j * il] = ip;
+        }
+        if ((band && !is_owner) || wl) {
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) win[i] = cn::fnms(l[i - 1], win[0], win[i]);
+        }
+        // row j leaves the window; row j+KL+1 enters at the bottom
+        const int jn = j + KL + 1;
+        T v = cn::zero<T>();
+        if (band) {
+            if (is_owner) {   // the owner takes column j+M: rows j+1..j+KL are zero there
+#pragma unroll
+                for (int i = 1; i <= KL; ++i) win[i] = cn::zero<T>();
+            }
+            const int dA = is_owner ? (M - 1) : (d - 1);
+            if (jn < n) v = A0[((size_t)jn * M + dA) * il];
+        }
+        if (wl && jn == k) v = cn::one<T>();
+#pragma unroll
+        for (int i = 0; i < KL; ++i) win[i] = win[i + 1];
+        win[KL] = v;
+        if (singular && first_singular == 0) first_singular = j + 1;
+    }
+    if (live && c == 0) info[m] = first_singular;
+}
+
+__device__ __forceinline__ double absv(double a) { return fabs(a); }
+__device__ __forceinline__ double absv(zc a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ zc neg(zc a) { return zc{-a.re, -a.im}; }
+__device__ __forceinline__ double neg(double a) { return -a; }
+
+// One lane per system: LU of S in place (column-major, partial pivoting by |re|+|im|), and
+// cond_1(S) = |S|_1 |S^-1|_1 with the inverse from the factors (banded.hpp:403-432).
+template <class T>
+__global__ void __launch_bounds__(256)
+k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, int* __restrict__ info,
+         int B, int r) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= B) return;
+    T a[MAXR * MAXR];
+    T* Sm = S + (size_t)m * r * r;
+    for (int q = 0; q < r * r; ++q) a[q] = Sm[q];
+    double nS = 0.0;
+    for (int b = 0; b < r; ++b) {
+        double s = 0.0;
+        for (int i = 0; i < r; ++i) s += absv(a[b * r + i]);
+        if (!(s <= nS)) nS = s;
+    }
+    int bad = 0;
+    int p[MAXR];
+    for (int jc = 0; jc < r; ++jc) {
+        int ip = jc;
+        double best = cn::cabs1(a[jc * r + jc]);
+        for (int i = jc + 1; i < r; ++i) {
+            const double v = cn::cabs1(a[jc * r + i]);
+            if (v > best) { best = v; ip = i; }
+        }
+        p[jc] = ip;
+        if (best == 0.0) { bad = 1; continue; }
+        if (ip != jc)
+            for (int q = 0; q < r; ++q) { T tmp = a[q * r + jc]; a[q * r + jc] = a[q * r + ip]; a[q * r + ip] = tmp; }
+        const T rc = cn::recip(a[jc * r + jc]);
+        for (int i = jc + 1; i < r; ++i) a[jc * r + i] = cn::mul(a[jc * r + i], rc);
+        for (int q = jc + 1; q < r; ++q)
+            for (int i = jc + 1; i < r; ++i) a[q * r + i] = cn::fnms(a[jc * r + i], a[q * r + jc], a[q * r + i]);
+    }
+    for (int q = 0; q < r * r; ++q) Sm[q] = a[q];
+    for (int q = 0; q < r; ++q) spiv[(size_t)m * r + q] = p[q];
+    double c = 1.0;
+    if (bad) {
+        if (info[m] == 0) info[m] = -1;
+        c = INFINITY;
+    } else {
+        // inverse column by column: solve S x = e_b with the factors
+        double ninv = 0.0;
+        for (int b = 0; b < r; ++b) {
+            T x[MAXR];
+            for (int i = 0; i < r; ++i) x[i] = (i == b) ? cn::one<T>() : cn::zero<T>();
+            for (int i = 0; i < r; ++i)
+                if (p[i] != i) { T tmp = x[i]; x[i] = x[p[i]]; x[p[i]] = tmp; }
+            for (int i = 0; i < r; ++i)
+                for (int q = 0; q < i; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
+            for (int i = r - 1; i >= 0; --i) {
+                for (int q = i + 1; q < r; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
+                x[i] = cn::mul(x[i], cn::recip(a[i * r + i]));
+            }
+            double s = 0.0;
+            for (int i = 0; i < r; ++i) s += absv(x[i]);
+            if (!(s <= ninv)) ninv = s;
+        }
+        c = nS * ninv;
+    }
+    cond[m] = c;
+}
+
+// ------------------------------------------------------------------------------------------
+// Solve kernel.  IO modes: 0 plain in place (x holds b), 1 mapped (rows/cols), 2 split complex.
+// ------------------------------------------------------------------------------------------
+struct SolveIO {
+    int mode;
+    void* x;               // mode 0: in place blocks; mode 1/2: output
+    const void* rhs;       // mode 1: rhs; mode 2: re
+    const int64_t* rows;   // mode 1: rows (or null); mode 2: rows_c
+    const int64_t* rows2;  // mode 2: rows_s
+    const int64_t* cols;   // mode 1: cols (or null); mode 2: cols_c
+    const int64_t* cols2;  // mode 2: cols_s
+    int64_t rhs_stride;    // mode 0
+};
+
+template <class T>
+__device__ __forceinline__ T io_load_b(const SolveIO& io, size_t mn, int k, int i);
+template <class T>
+__device__ __forceinline__ void io_store(const SolveIO& io, size_t mn, int k, int i, T v);
+template <class T>
+__device__ __forceinline__ T io_load_x(const SolveIO& io, size_t mn, int k, int i);
+
+template <>
+__device__ __forceinline__ double io_load_b<double>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const double*>(io.x)[k * io.rhs_stride + mn + i];
+    const size_t q = io.rows ? (size_t)io.rows[mn + i] : mn + i;
+    return static_cast<const double*>(io.rhs)[q];
+}
+template <>
+__device__ __forceinline__ zc io_load_b<zc>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const zc*>(io.x)[k * io.rhs_stride + mn + i];
+    if (io.mode == 1) {
+        const size_t q = io.rows ? (size_t)io.rows[mn + i] : mn + i;
+        return static_cast<const zc*>(io.rhs)[q];
+    }
+    const double* re = static_cast<const double*>(io.rhs);
+    return zc{re[io.rows[mn + i]], re[io.rows2[mn + i]]};
+}
+template <>
+__device__ __forceinline__ double io_load_x<double>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const double*>(io.x)[k * io.rhs_stride + mn + i];
+    const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+    return static_cast<const double*>(io.x)[q];
+}
+template <>
+__device__ __forceinline__ zc io_load_x<zc>(const SolveIO& io, size_t mn, int k, int i) {
+    if (io.mode == 0) return static_cast<const zc*>(io.x)[k * io.rhs_stride + mn + i];
+    if (io.mode == 1) {
+        const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+        return static_cast<const zc*>(io.x)[q];
+    }
+    const double* x = static_cast<const double*>(io.x);
+    return zc{x[io.cols[mn + i]], x[io.cols2[mn + i]]};
+}
+template <>
+__device__ __forceinline__ void io_store<double>(const SolveIO& io, size_t mn, int k, int i, double v) {
+    if (io.mode == 0) { static_cast<double*>(io.x)[k * io.rhs_stride + mn + i] = v; return; }
+    const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+    static_cast<double*>(io.x)[q] = v;
+}
+template <>
+__device__ __forceinline__ void io_store<zc>(const SolveIO& io, size_t mn, int k, int i, zc v) {
+    if (io.mode == 0) { static_cast<zc*>(io.x)[k * io.rhs_stride + mn + i] = v; return; }
+    if (io.mode == 1) {
+        const size_t q = io.cols ? (size_t)io.cols[mn + i] : mn + i;
+        static_cast<zc*>(io.x)[q] = v;
+        return;
+    }
+    double* x = static_cast<double*>(io.x);
+    x[io.cols[mn + i]] = v.re;
+    x[io.cols2[mn + i]] = v.im;
+}
+
+// z = S^-1 t in registers (S LU-factored column-major with pivots sp; r <= MAXR)
+template <class T>
+__device__ __forceinline__ void cap_solve(T (&t)[MAXR], const T* __restrict__ Sm, const int* __restrict__ sp, int r) {
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i >= r) break;
+        const int pi = sp[i];
+#pragma unroll
+        for (int q = i + 1; q < MAXR; ++q) {
+            const bool c = (q == pi);
+            const T ti = t[i], tq = t[q];
+            t[i] = cn::sel(c, tq, ti);
+            t[q] = cn::sel(c, ti, tq);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i >= r) break;
+#pragma unroll
+        for (int q = 0; q < i; ++q) t[i] = cn::fnms(Sm[q * r + i], t[q], t[i]);
+    }
+#pragma unroll
+    for (int i = MAXR - 1; i >= 0; --i) {
+        if (i < r) {
+#pragma unroll
+            for (int q = i + 1; q < MAXR; ++q)
+                if (q < r) t[i] = cn::fnms(Sm[q * r + i], t[q], t[i]);
+            t[i] = cn::mul(t[i], cn::recip(Sm[i * r + i]));
+        }
+    }
+}
+
+// Direct solve (any shape and IO mode), the Woodbury order of banded.hpp:312-341:
+//   c = P b;  y = U^-1 L^-1 P_B c (forward sweep, then backward sweep);  t = V^T y accumulated
+//   as the backward sweep produces y;  z = S^-1 t;  x = y - W z (a third pass over x).
+// z is formed from the y that x is built from, so the residual of x is that of the two band
+// sweeps and of W alone (no cond(B) amplification through the border rows).
+template <class T, int KL, int WU>
+__global__ void __launch_bounds__(256)
+k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+        const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+        const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+        SolveIO io, int B, int n, int r, int US, int nrhs) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / nrhs;
+    const int k = gid - m * nrhs;
+    if (m >= B) return;
+    const size_t NL = (size_t)B * nrhs;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * IL) + ml;
+    const int* P0 = piv + ch * ((size_t)n * IL) + ml;
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    const T* W0 = W + ch * ((size_t)r * n * IL) + ml;
+    const size_t mn = (size_t)m * n;
+    T* w0 = ws + gid;
+
+    // forward: y = L^-1 P_B c, window of KL+1 rows
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = (i < n) ? io_load_b<T>(io, mn, k, perm[i]) : cn::zero<T>();
+    for (int j = 0; j < n; ++j) {
+        const int p = P0[(size_t)j * IL];
+        if (p != 0) cn::swap_first<T, KL>(y, p);
+        const T* Lr = L0 + (size_t)j * KL * IL;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(Lr[(i - 1) * IL], y[0], y[i]);
+        w0[(size_t)j * NL] = y[0];
+        const int jn = j + KL + 1;
+#pragma unroll
+        for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+        y[KL] = (jn < n) ? io_load_b<T>(io, mn, k, perm[jn]) : cn::zero<T>();
+    }
+    // backward: y = U^-1 y_c, window of the last WU solutions; t = V^T y
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        const T* Ur = U0 + (size_t)j * US * IL;
+        T acc = w0[(size_t)j * NL];
+#pragma unroll
+        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T x = cn::mul(acc, Ur[0]);
+        io_store<T>(io, mn, k, j, x);
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) t[a] = cn::fmac(V0[((size_t)a * n + j) * IL], x, t[a]);
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+        xw[0] = x;
+    }
+    if (r == 0) return;
+    cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    // x = y - W z
+    for (int j = 0; j < n; ++j) {
+        T v = io_load_x<T>(io, mn, k, j);
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + j) * IL], t[a], v);
+        io_store<T>(io, mn, k, j, v);
+    }
+}
+
+// Direct completion of W = B^-1 U from its forward half (in place) and S = I + V^T W:
+// one lane per (system, border column a), back substitution with U.
+template <class T, int WU>
+__global__ void __launch_bounds__(256)
+k_border(const T* __restrict__ Uf, const T* __restrict__ Vt, T* __restrict__ W, T* __restrict__ S, int B,
+         int n, int r, int US) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / r;
+    const int a = gid - m * r;
+    if (m >= B) return;
+    const size_t ch = m / IL;
+    const int ml = m % IL;
+    const T* U0 = Uf + ch * ((size_t)n * US * IL) + ml;
+    const T* V0 = Vt + ch * ((size_t)r * n * IL) + ml;
+    T* W0 = W + ch * ((size_t)r * n * IL) + ml + (size_t)a * n * IL;
+    T t[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        const T* Ur = U0 + (size_t)j * US * IL;
+        T acc = W0[(size_t)j * IL];
+#pragma unroll
+        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T x = cn::mul(acc, Ur[0]);
+        W0[(size_t)j * IL] = x;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q < r) t[q] = cn::fmac(V0[((size_t)q * n + j) * IL], x, t[q]);
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+        xw[0] = x;
+    }
+    // column a of S (column-major): S(q, a) = delta_qa + (V^T W)(q, a)
+    T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q)
+        if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
+}
+
+// ------------------------------------------------------------------------------------------
+// Streamed sweeps (the hot kernels for many narrow-band systems).  A workgroup is one wavefront
+// and one interleave chunk of 16 systems, with NR lanes per system (NR in {1, 2, 4} right-hand
+// sides or border columns).  Every sweep runs in blocks of SPX steps: the factor rows of block
+// b+1 (chunk-contiguous, copied with global_load_lds_dwordx4) and the per-lane operands of block
+// b+1 (gathered by the same instruction, one slot per lane) travel from HBM into the other half
+// of a double-buffered LDS area while block b is computed from LDS; the results of block b are
+// held in registers and stored at the start of block b+1, so that the one wait per block
+// (s_waitcnt vmcnt(0)) covers memory operations issued a whole block earlier.
+// ------------------------------------------------------------------------------------------
+
+using lds_ptr = __attribute__((address_space(3))) void*;
+
+// contiguous copy of nbytes (multiple of 16) from global memory into LDS, 1 KiB per instruction
+__device__ __forceinline__ void dma16(const void* src_, char* lds_dst, int nbytes, int lane) {
+    const char* src = static_cast<const char*>(src_);
+    for (int off = 0; off < nbytes; off += 1024) {
+        const int my = off + lane * 16;
+        if (my < nbytes)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my), (lds_ptr)(lds_dst + off), 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+template <class T>
+__device__ __forceinline__ T lds_at(const char* p) { return *reinterpret_cast<const T*>(p); }
+
+// One value per lane gathered from an arbitrary global address into LDS slot `slot` (lane l's
+// value at slot + l * sizeof(T); a double travels as two dwords into two 256-byte halves).
+template <class T>
+struct LaneSlot;
+template <>
+struct LaneSlot<zc> {
+    static constexpr int BYTES = 64 * 16;
+    __device__ static void load(const zc* src, char* slot) {
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src), (lds_ptr)slot, 16, 0, 0);
+    }
+    __device__ static zc read(const char* slot, int lane) { return lds_at<zc>(slot + lane * 16); }
+    __device__ static void zero(char* slot, int lane) { *reinterpret_cast<zc*>(slot + lane * 16) = zc{0.0, 0.0}; }
+};
+template <>
+struct LaneSlot<double> {
+    static constexpr int BYTES = 64 * 8;
+    __device__ static void load(const double* src, char* slot) {
+        const char* s = reinterpret_cast<const char*>(src);
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(s), (lds_ptr)slot, 4, 0, 0);
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(s + 4), (lds_ptr)(slot + 256), 4, 0, 0);
+    }
+    __device__ static double read(const char* slot, int lane) {
+        const unsigned lo = lds_at<unsigned>(slot + lane * 4);
+        const unsigned hi = lds_at<unsigned>(slot + 256 + lane * 4);
+        return __hiloint2double((int)hi, (int)lo);
+    }
+    __device__ static void zero(char* slot, int lane) {
+        *reinterpret_cast<unsigned*>(slot + lane * 4) = 0u;
+        *reinterpret_cast<unsigned*>(slot + 256 + lane * 4) = 0u;
+    }
+};
+
+// LDS bytes of one buffer of each streamed sweep (blocks of SPX steps)
+__host__ __device__ constexpr int fwd_bytes(int SPX, int KL, int esz) {
+    return SPX * KL * IL * esz + SPX * IL * 4 + SPX * 64 * esz;
+}
+__host__ __device__ constexpr int bwd_bytes(int SPX, int US, int r, int esz) {
+    return (SPX * US + r * SPX) * IL * esz + SPX * 64 * esz;
+}
+__host__ __device__ constexpr int cor_bytes(int SPX, int r, int esz) { return r * SPX * IL * esz + SPX * 64 * esz; }
+
+// backward block blk (rows jhi = n-1-blk*SPX down to jlo) into buf:
+// [U: SPX rows x US][V^T: r x SPX][in: SPX lane slots, slot s = row jhi - s]
+template <class T, int SPX>
+__device__ __forceinline__ void bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
+                                        int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
+    constexpr int CE = IL * (int)sizeof(T);
+    const int jhi = n - 1 - blk * SPX;
+    const int jlo = max(0, jhi - SPX + 1);
+    const int cnt = jhi - jlo + 1;
+    const int bU = SPX * US * CE, bV = r * SPX * CE;
+    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
+    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < SPX; ++s)
+            if (jhi - s >= jlo) LaneSlot<T>::load(in + (size_t)(jhi - s) * in_st, buf + bU + bV + s * LaneSlot<T>::BYTES);
+    }
+}
+
+// One backward step of a lane: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), t += V^T(:, j) out_j.
+// bb: the block buffer, row: j - jlo, sidx: slot of in_j.
+template <class T, int WU, int SPX>
+__device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, int US, int r, int mlx, int lane,
+                                           T (&xw)[WU], T (&t)[MAXR]) {
+    constexpr int CE = IL * (int)sizeof(T);
+    const int bU = SPX * US * CE, bV = r * SPX * CE;
+    T acc = LaneSlot<T>::read(bb + bU + bV + sidx * LaneSlot<T>::BYTES, lane);
+    const char* ur = bb + row * US * CE + mlx * (int)sizeof(T);
+#pragma unroll
+    for (int d = 1; d <= WU; ++d) acc = cn::fnms(lds_at<T>(ur + d * CE), xw[d - 1], acc);
+    const T x = cn::mul(acc, lds_at<T>(ur));
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a)
+        if (a < r) t[a] = cn::fmac(lds_at<T>(bb + bU + (a * SPX + row) * CE + mlx * (int)sizeof(T)), x, t[a]);
+#pragma unroll
+    for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+    xw[0] = x;
+    return x;
+}
+
+// Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
+// t[a] += V^T(a, j) out_j.  `in` and `out` may be the same array (in place).  Full blocks run
+// without per-step tests; a partial last block step by step.
+template <class T, int WU, int SPX>
+__device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int bufbytes,
+                                          size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
+                                          size_t out_st, bool live, int lane, int mlx, T (&t)[MAXR]) {
+    const int nblk = (n + SPX - 1) / SPX;
+    T xw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    T o[SPX];
+    bwd_dma<T, SPX>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
+    dma_wait();
+    for (int b = 0; b < nblk; ++b) {
+        const int buf = b & 1;
+        const int jhi = n - 1 - b * SPX;
+        const int jlo = max(0, jhi - SPX + 1);
+        const int cnt = jhi - jlo + 1;
+        if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = o[s];
+        }
+        if (b + 1 < nblk) bwd_dma<T, SPX>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
+        const char* bb = lds + buf * bufbytes;
+        if (cnt == SPX) {
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) o[s] = bwd_lane_step<T, WU, SPX>(bb, SPX - 1 - s, s, US, r, mlx, lane, xw, t);
+        } else {
+#pragma unroll
+            for (int s = 0; s < SPX; ++s)
+                if (s < cnt) o[s] = bwd_lane_step<T, WU, SPX>(bb, cnt - 1 - s, s, US, r, mlx, lane, xw, t);
+        }
+        dma_wait();
+    }
+    if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
+        const int jhi = n - 1 - (nblk - 1) * SPX;
+#pragma unroll
+        for (int s = 0; s < SPX; ++s)
+            if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = o[s];
+    }
+    dma_wait();
+}
+
+// One forward step of a lane (window y[0..KL] = rows j .. j+KL): optional row interchange,
+// elimination below the pivot, shift, incoming row c.
+template <class T, int KL, bool PIV>
+__device__ __forceinline__ T fwd_lane_step(T (&y)[KL + 1], const char* Lrow, int p, const T& c, int mlx) {
+    constexpr int CE = IL * (int)sizeof(T);
+    if (PIV) cn::swap_first<T, KL>(y, p);
+#pragma unroll
+    for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(lds_at<T>(Lrow + (i - 1) * CE + mlx * (int)sizeof(T)), y[0], y[i]);
+    const T out = y[0];
+#pragma unroll
+    for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+    y[KL] = c;
+    return out;
+}
+
+// Streamed solve: forward sweep (pivots, L) into the work space, backward sweep (U) into x with
+// t = V^T y, z = S^-1 t, then the streamed pass x = y - W z.  Arithmetic as k_solve.
+// IO: in place (bin == xout, right-hand side k at k * rhs_stride) or contiguous out of place.
+// Forward blocks without any row interchange in the chunk skip the (select-based) interchanges.
+template <class T, int KL, int WU, int NR, int SPX>
+__global__ void __launch_bounds__(64)
+k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
+          const T* bin, T* xout, int64_t rhs_stride, int B, int n, int r, int US, int bufbytes) {
+    static_assert(KL + 1 <= SPX, "the window lookahead must stay within the next block");
+    constexpr int CE = IL * (int)sizeof(T);
+    constexpr int GB = LaneSlot<T>::BYTES;
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int mloc = lane / NR, k = lane - mloc * NR;
+    const int mlx = mloc & (IL - 1);            // LDS column of this lane's system (dead lanes alias a live one)
+    const size_t ch = blockIdx.x;
+    const int m = (int)ch * IL + mloc;
+    const bool live = mloc < IL && m < B;
+    const size_t NL = (size_t)B * NR;
+    const T* bl = bin + k * rhs_stride + (size_t)m * n;
+    T* xl = xout + k * rhs_stride + (size_t)m * n;
+    T* wl = ws + (size_t)m * NR + k;
+    const int fL = SPX * KL * CE, fP = SPX * IL * 4;
+    const int nblk = (n + SPX - 1) / SPX;
+
+    // ---- forward sweep: y_c = L^-1 P_B P b into the work space
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
+    T o[SPX];
+    for (int b = -1; b < nblk; ++b) {
+        const int buf = b & 1;
+        if (b >= 1 && live) {   // results of block b-1
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) wl[(size_t)((b - 1) * SPX + s) * NL] = o[s];
+        }
+        if (b + 1 < nblk) {     // block b+1: [L: SPX x KL][piv: SPX x 16][c: SPX lane slots]
+            char* nb = lds + ((b + 1) & 1) * bufbytes;
+            const int j0 = (b + 1) * SPX;
+            const int cnt = min(SPX, n - j0);
+            dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
+            dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) {
+                const int jn = j0 + KL + 1 + s;
+                if (jn < n) {
+                    const int pj = perm[jn];
+                    if (live) LaneSlot<T>::load(bl + pj, nb + fL + fP + s * GB);
+                } else {
+                    LaneSlot<T>::zero(nb + fL + fP + s * GB, lane);
+                }
+            }
+        }
+        if (b >= 0) {
+            const char* fb = lds + buf * bufbytes;
+            const int j0 = b * SPX, cnt = min(SPX, n - j0);
+            if (cnt == SPX) {
+                // any row interchange in this block for the chunk's systems?
+                int any = 0;
+                for (int q = lane; q < SPX * IL; q += 64) any |= lds_at<int>(fb + fL + q * 4);
+                if (__ballot(any != 0) == 0) {
+#pragma unroll
+                    for (int s = 0; s < SPX; ++s)
+                        o[s] = fwd_lane_step<T, KL, false>(y, fb + s * KL * CE, 0, LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+                } else {
+#pragma unroll
+                    for (int s = 0; s < SPX; ++s)
+                        o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
+                                                          LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SPX; ++s)
+                    if (s < cnt)
+                        o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
+                                                          LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+            }
+        }
+        dma_wait();
+    }
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < SPX; ++s) {
+            const int j = (nblk - 1) * SPX + s;
+            if (j < n) wl[(size_t)j * NL] = o[s];
+        }
+    }
+    dma_wait();
+
+    // ---- backward sweep: y = U^-1 y_c into x, t = V^T y
+    T t[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    bwd_sweep<T, WU, SPX>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, xl, 1, live, lane, mlx, t);
+    if (r == 0) return;
+
+    // ---- z = S^-1 t, then x = y - W z: block [W: r x SPX][y: SPX lane slots]
+    if (live) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+    const int cW = r * SPX * CE;
+    for (int b = -1; b < nblk; ++b) {
+        const int buf = b & 1;
+        if (b >= 1 && live) {
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) xl[(b - 1) * SPX + s] = o[s];
+        }
+        if (b + 1 < nblk) {
+            char* nb = lds + ((b + 1) & 1) * bufbytes;
+            const int j0 = (b + 1) * SPX;
+            const int cnt = min(SPX, n - j0);
+            for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
+            if (live) {
+#pragma unroll
+                for (int s = 0; s < SPX; ++s)
+                    if (j0 + s < n) LaneSlot<T>::load(xl + j0 + s, nb + cW + s * GB);
+            }
+        }
+        if (b >= 0) {
+            const char* cb = lds + buf * bufbytes;
+#pragma unroll
+            for (int s = 0; s < SPX; ++s) {
+                T v = LaneSlot<T>::read(cb + cW + s * GB, lane);
+#pragma unroll
+                for (int a = 0; a < MAXR; ++a)
+                    if (a < r) v = cn::fnms(lds_at<T>(cb + (a * SPX + s) * CE + mlx * (int)sizeof(T)), t[a], v);
+                o[s] = v;
+            }
+        }
+        dma_wait();
+    }
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < SPX; ++s) {
+            const int j = (nblk - 1) * SPX + s;
+            if (j < n) xl[j] = o[s];
+        }
+    }
+}
+
+// Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
+// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s.
+template <class T, int WU, int NR, int SPX>
+__global__ void __launch_bounds__(64)
+k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
+           int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int mloc = lane / NR, k = lane - mloc * NR;
+    const int mlx = mloc & (IL - 1);
+    const size_t ch = blockIdx.x;
+    const int m = (int)ch * IL + mloc;
+    const int a = blockIdx.y * NR + k;
+    const bool live = mloc < IL && m < B && a < r;
+    T* wa = W + (ch * r * n + (size_t)(live ? a : 0) * n) * IL + mlx;
+    T t[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
+    bwd_sweep<T, WU, SPX>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, lane, mlx, t);
+    if (!live) return;
+    T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q)
+        if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
+}
+
+// ------------------------------------------------------------------------------------------
+// Wave mode: one wavefront per (system, right-hand side), for few or wide-band systems.
+// Factors are stored per system contiguously (no interleave) and U by columns, scaled by the
+// column's pivot after the factorization: Uc[j][0] = 1 / U(j, j), Uc[j][d] = U(j-d, j) / U(j, j)
+// (d >= 1).  The right-hand side, the forward result and the solution stay in LDS (n entries).
+// The sweeps keep the band window in the low lanes (lane 0 = the row being finished) and the
+// next rows in the lanes above; a step is one broadcast of lane 0 (v_readlane), one
+// multiply-add per window lane and one wave rotation (DPP wave_rol:1), which brings the next row
+// into the window and parks the finished row in lane 63:
+//   forward (w = kl+1 lanes): the gbtrs column order, pivot exchange of lanes 0 and p;
+//   backward (w = kl+ku+1 lanes): the column-oriented order of LAPACK's banded triangular solve,
+//   with the division by U(j, j) applied when the finished rows are flushed.
+// The L rows + pivots (forward) and U columns (backward) of the next block of SPW steps are
+// copied HBM -> LDS while the current block is computed.  t = V^T y, z = S^-1 t and
+// x = y - W z are parallel over the entries of x.
+// ------------------------------------------------------------------------------------------
+constexpr int SPW = 32;   // steps per staged block in wave mode
+
+// Ordering point of a one-wavefront workgroup: LDS operations of a wavefront execute in order,
+// and the staging copies are waited for explicitly (vm_wait_le), so only the compiler has to be
+// kept from moving memory operations across it (a workgroup barrier would also drain the copies
+// in flight).
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// yv[j] = src[j * st] for j < n (batched loads)
+template <class T>
+__device__ __forceinline__ void wave_load(T* yv, const T* __restrict__ src, size_t st, int n, int lane) {
+    for (int j0 = 0; j0 < n; j0 += 256) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = src[(size_t)min(j0 + 64 * u + lane, n - 1) * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + 64 * u + lane < n) yv[j0 + 64 * u + lane] = v[u];
+    }
+}
+
+
+__device__ __forceinline__ double bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ zc bcast(zc v, int src) { return zc{bcast(v.re, src), bcast(v.im, src)}; }
+
+__device__ __forceinline__ double xor_shfl(double v, int mask) {
+    const int lo = __shfl_xor(__double2loint(v), mask, 64);
+    const int hi = __shfl_xor(__double2hiint(v), mask, 64);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ zc xor_shfl(zc v, int mask) { return zc{xor_shfl(v.re, mask), xor_shfl(v.im, mask)}; }
+__device__ __forceinline__ double addv(double a, double b) { return a + b; }
+__device__ __forceinline__ zc addv(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
+
+// contiguous copy of nbytes (multiple of 4) into LDS, 256 bytes per instruction
+__device__ __forceinline__ void dma4(const void* src_, char* lds_dst, int nbytes, int lane) {
+    const char* src = static_cast<const char*>(src_);
+    for (int off = 0; off < nbytes; off += 256) {
+        const int my = off + lane * 4;
+        if (my < nbytes)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + my), (lds_ptr)(lds_dst + off), 4, 0, 0);
+    }
+}
+// entries of T: 16-byte copies for complex (always 16-byte aligned), dwords for real
+template <class T>
+__device__ __forceinline__ void dmaT(const T* src, char* dst, int count, int lane) {
+    if constexpr (sizeof(T) == 16) dma16(src, dst, count * 16, lane);
+    else dma4(src, dst, count * (int)sizeof(T), lane);
+}
+
+__host__ __device__ constexpr int align16(int b) { return (b + 15) / 16 * 16; }
+// LDS after the two vectors: zero slot, capacitance LU (MAXR^2 entries) and its pivots
+constexpr int WAVE_SMALL = 16 + MAXR * MAXR * 16 + MAXR * 4 + 16;
+// LDS bytes of one staging buffer of the wave-mode sweeps
+__host__ __device__ constexpr int wave_buf_bytes(int KL, int US, int esz) {
+    return align16(SPW * KL * esz + SPW * 4) > align16(SPW * US * esz) ? align16(SPW * KL * esz + SPW * 4)
+                                                                        : align16(SPW * US * esz);
+}
+
+// number of copy instructions dmaT / dma4 issue for `count` entries / bytes
+template <class T>
+__device__ __forceinline__ int dmaT_count(int count) {
+    return sizeof(T) == 16 ? (count * 16 + 1023) / 1024 : (count * (int)sizeof(T) + 255) / 256;
+}
+__device__ __forceinline__ int dma4_count(int nbytes) { return (nbytes + 255) / 256; }
+
+// s_waitcnt vmcnt(k) for a wave-uniform k in [0, 63]: wait until at most k vector memory
+// operations of this wave are outstanding (they complete in order: only loads are in flight).
+__device__ __forceinline__ void vm_wait_le(int k) {
+    switch (k) {
+#define CN_VMW(K) case K: asm volatile("s_waitcnt vmcnt(" #K ")" ::: "memory"); break;
+        CN_VMW(0) CN_VMW(1) CN_VMW(2) CN_VMW(3) CN_VMW(4) CN_VMW(5) CN_VMW(6) CN_VMW(7) CN_VMW(8) CN_VMW(9)
+        CN_VMW(10) CN_VMW(11) CN_VMW(12) CN_VMW(13) CN_VMW(14) CN_VMW(15) CN_VMW(16) CN_VMW(17) CN_VMW(18)
+        CN_VMW(19) CN_VMW(20) CN_VMW(21) CN_VMW(22) CN_VMW(23) CN_VMW(24) CN_VMW(25) CN_VMW(26) CN_VMW(27)
+        CN_VMW(28) CN_VMW(29) CN_VMW(30) CN_VMW(31) CN_VMW(32) CN_VMW(33) CN_VMW(34) CN_VMW(35) CN_VMW(36)
+        CN_VMW(37) CN_VMW(38) CN_VMW(39) CN_VMW(40) CN_VMW(41) CN_VMW(42) CN_VMW(43) CN_VMW(44) CN_VMW(45)
+        CN_VMW(46) CN_VMW(47) CN_VMW(48) CN_VMW(49) CN_VMW(50) CN_VMW(51) CN_VMW(52) CN_VMW(53) CN_VMW(54)
+        CN_VMW(55) CN_VMW(56) CN_VMW(57) CN_VMW(58) CN_VMW(59) CN_VMW(60) CN_VMW(61) CN_VMW(62)
+#undef CN_VMW
+        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
+// Staging copies of one block (returns the number of copy instructions issued)
+template <class T>
+__device__ __forceinline__ int wave_stage_fwd(const T* __restrict__ L0, const int* __restrict__ P0, int n, int KL,
+                                              int blk, char* buf, int lane) {
+    const int j0 = blk * SPW, cnt = min(SPW, n - j0);
+    dmaT<T>(L0 + (size_t)j0 * KL, buf, cnt * KL, lane);
+    dma4(P0 + j0, buf + SPW * KL * (int)sizeof(T), cnt * 4, lane);
+    return dmaT_count<T>(cnt * KL) + dma4_count(cnt * 4);
+}
+template <class T>
+__device__ __forceinline__ int wave_stage_bwd(const T* __restrict__ Uc0, int n, int US, int blk, char* buf, int lane) {
+    const int jhi = n - 1 - blk * SPW, jlo = max(0, jhi - SPW + 1);
+    dmaT<T>(Uc0 + (size_t)jlo * US, buf, (jhi - jlo + 1) * US, lane);
+    return dmaT_count<T>((jhi - jlo + 1) * US);
+}
+
+// One-lane rotation toward lane 0 across the whole wavefront (DPP wave_rol:1): lane i receives
+// lane i+1, lane 63 receives lane 0.
+__device__ __forceinline__ int rol1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xf, 0xf, false); }
+__device__ __forceinline__ double rol1(double v) {
+    return __hiloint2double(rol1(__double2hiint(v)), rol1(__double2loint(v)));
+}
+__device__ __forceinline__ zc rol1(zc v) { return zc{rol1(v.re), rol1(v.im)}; }
+
+// Forward sweep in place on yv (n entries): y_c = L^-1 P_B c.
+// Lanes 0 .. kl hold the window rows j .. j+kl (lane 0: pivot row), the lanes above hold the next
+// rows of c in order.  A step: pivot exchange of lanes 0 and p (selects, a no-op for p = 0),
+// broadcast of the pivot row, one multiply-add per lane (multiplier 0 outside the window: those
+// lanes read a zero slot), and one wave rotation, which moves the next row into the window and the
+// finished row to lane 63.  Steps run in branch-free groups of SB <= 64 - w; after a group the
+// lanes 64-SB .. 63 hold its finished rows: they are stored, and the lanes above the window are
+// reloaded with the next rows of c.  The multipliers of a block of SPW steps are read from LDS
+// into registers before the block.  A partial last block runs step by step (SB = 0: every block).
+template <class T>
+__device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
+    const T v0 = bcast(y, 0), vp = bcast(y, p);
+    y = cn::sel(lane == p, v0, y);
+    y = cn::sel(lane == 0, vp, y);
+    y = cn::fnms(l, vp, y);
+    y = rol1(y);
+}
+
+template <class T, int SB>
+__device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int* __restrict__ P0, T* yv,
+                                             const char* zslot, char* bufs, int bufbytes, int n, int KL, int lane) {
+    constexpr int ES = (int)sizeof(T);
+    const int w = KL + 1;
+    const int R = 64 - w;
+    const int nblk = (n + SPW - 1) / SPW;
+    const bool upd = lane >= 1 && lane <= KL;
+    const bool above = lane >= w;
+    T y = lane < n ? yv[lane] : cn::zero<T>();
+    int t = 0, jf = 0;            // steps since the last flush, first row finished since then
+    const int Poff = SPW * KL * ES;
+    // three staging buffers, copies issued two blocks ahead
+    wave_stage_fwd<T>(L0, P0, n, KL, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_fwd<T>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
+    vm_wait_le(c1);
+    wave_sync();
+    for (int b = 0; b < nblk; ++b) {
+        char* cur = bufs + (b % 3) * bufbytes;
+        const int c2 = b + 2 < nblk ? wave_stage_fwd<T>(L0, P0, n, KL, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int j0 = b * SPW, cnt = min(SPW, n - j0);
+        const int pv = lane < cnt ? lds_at<int>(cur + Poff + lane * 4) : 0;   // pivot of step lane
+        const char* lb = upd ? cur + (lane - 1) * ES : zslot;
+        const int lst = upd ? KL * ES : 0;
+        if (SB > 0 && cnt == SPW) {
+            T lv[SPW];
+#pragma unroll
+            for (int sidx = 0; sidx < SPW; ++sidx) lv[sidx] = lds_at<T>(lb + sidx * lst);
+#pragma unroll
+            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+#pragma unroll
+                for (int u = 0; u < SB; ++u) fwd_step<T>(y, lv[q * SB + u], __builtin_amdgcn_readlane(pv, q * SB + u), lane);
+                const int jn = j0 + (q + 1) * SB;     // first row of the new window
+                if (lane >= 64 - SB) yv[jf + lane - (64 - SB)] = y;
+                if (above) y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                jf += SB;
+            }
+        } else {
+            for (int sidx = 0; sidx < cnt; ++sidx) {
+                fwd_step<T>(y, lds_at<T>(lb + sidx * lst), __builtin_amdgcn_readlane(pv, sidx), lane);
+                if (++t == R) {       // lane w + i: finished row jf + i
+                    const int jn = j0 + sidx + 1;
+                    if (above) {
+                        yv[jf + lane - w] = y;
+                        y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                    }
+                    jf += R;
+                    t = 0;
+                }
+            }
+        }
+        vm_wait_le(c2);           // block b+1 has landed
+        wave_sync();
+    }
+    if (lane >= 64 - t) yv[jf + lane - (64 - t)] = y;   // the last t finished rows
+    wave_sync();
+}
+
+// Backward sweep in place on yv: y = U^-1 y_c with U stored by columns and scaled by the pivot of
+// the column, Uc[j][d] = U(j-d, j) / U(j, j) (d >= 1), Uc[j][0] = 1 / U(j, j).  Lanes 0 .. w-1
+// hold rows j, j-1, .., j-w+1, the lanes above the next rows of y_c.  A step: broadcast of the
+// finished row value y_j (lane 0), every window lane subtracts Uc[j][d] y_j, one wave rotation
+// (y_j moves to lane 63).  Groups, flushes and the partial last block as in the forward sweep;
+// the division x_j = y_j / U(j, j) is applied to the finished rows when they are stored, from the
+// diagonal dv (n entries in LDS).
+template <class T>
+__device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
+    const T yj = bcast(y, 0);
+    y = cn::fnms(u, yj, y);
+    y = rol1(y);
+}
+
+template <class T, int SB>
+__device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot, char* bufs,
+                                              int bufbytes, int n, int w, int US, int lane) {
+    constexpr int ES = (int)sizeof(T);
+    const int R = 64 - w;
+    const int nblk = (n + SPW - 1) / SPW;
+    const bool upd = lane >= 1 && lane < w;
+    const bool above = lane >= w;
+    wave_load<T>(dv, Uc0, US, n, lane);
+    T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
+    int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
+    wave_stage_bwd<T>(Uc0, n, US, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_bwd<T>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
+    vm_wait_le(c1);
+    wave_sync();
+    for (int b = 0; b < nblk; ++b) {
+        char* cur = bufs + (b % 3) * bufbytes;
+        const int c2 = b + 2 < nblk ? wave_stage_bwd<T>(Uc0, n, US, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int jhi = n - 1 - b * SPW, jlo = max(0, jhi - SPW + 1);
+        const int cnt = jhi - jlo + 1;
+        const char* ub = upd ? cur + lane * ES : zslot;
+        const int ust = upd ? US * ES : 0;
+        if (SB > 0 && cnt == SPW) {
+            T uv[SPW];
+#pragma unroll
+            for (int sidx = 0; sidx < SPW; ++sidx) uv[sidx] = lds_at<T>(ub + (SPW - 1 - sidx) * ust);
+#pragma unroll
+            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+#pragma unroll
+                for (int u = 0; u < SB; ++u) bwd_step<T>(y, uv[q * SB + u], lane);
+                const int jn = jhi - (q + 1) * SB;    // row of lane 0 in the new window
+                if (lane >= 64 - SB) {
+                    const int row = jf - (lane - (64 - SB));
+                    yv[row] = cn::mul(y, dv[row]);
+                }
+                if (above) y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                jf -= SB;
+            }
+        } else {
+            for (int sidx = 0; sidx < cnt; ++sidx) {
+                bwd_step<T>(y, lds_at<T>(ub + (cnt - 1 - sidx) * ust), lane);
+                if (++t == R) {       // lane w + i: finished row jf - i
+                    const int jn = jhi - sidx - 1;
+                    if (above) {
+                        const int row = jf - (lane - w);
+                        yv[row] = cn::mul(y, dv[row]);
+                        y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                    }
+                    jf -= R;
+                    t = 0;
+                }
+            }
+        }
+        vm_wait_le(c2);           // block b+1 has landed
+        wave_sync();
+    }
+    if (lane >= 64 - t) {
+        const int row = jf - (lane - (64 - t));
+        yv[row] = cn::mul(y, dv[row]);
+    }
+    wave_sync();
+}
+
+// group size of the backward sweep for a window of w lanes (0: step by step)
+__host__ __device__ constexpr int bwd_group(int w) { return w <= 32 ? 32 : w <= 48 ? 16 : w <= 56 ? 8 : 0; }
+
+template <class T>
+__device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot,
+                                                  char* bufs, int bufbytes, int n, int w, int US, int lane) {
+    switch (bwd_group(w)) {
+        case 32: wave_backward<T, 32>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 16: wave_backward<T, 16>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 8: wave_backward<T, 8>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        default: wave_backward<T, 0>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+    }
+}
+
+// Loads of the parallel phases are batched (every load of a chunk of 128 entries issued before
+// the first use), so that a phase costs a few memory latencies, not one per entry.
+constexpr int WCH = 2;   // 64-entry groups per chunk
+
+// t[a] = sum_j V^T(a, j) y_j over the wave (every lane ends with the totals)
+template <class T>
+__device__ __forceinline__ void wave_vty(const T* __restrict__ V0, const T* yv, int n, int r, int lane, T (&t)[MAXR]) {
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
+    for (int j0 = 0; j0 < n; j0 += 64 * WCH) {
+        T vv[WCH][MAXR];
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = min(j0 + 64 * u + lane, n - 1);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a) vv[u][a] = V0[(size_t)min(a, max(r - 1, 0)) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = j0 + 64 * u + lane;
+            const T yj = j < n ? yv[j] : cn::zero<T>();
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a)
+                if (a < r) t[a] = cn::fmac(vv[u][a], yj, t[a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) {
+        if (a < r) {
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) t[a] = addv(t[a], xor_shfl(t[a], mask));
+        }
+    }
+}
+
+// Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][dv: n entries][zero
+// slot, S, pivots of S][3 staging buffers].
+template <class T>
+__global__ void __launch_bounds__(64)
+k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, SolveIO io, int n, int r, int KL, int w2,
+          int US, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x, k = blockIdx.y;
+    const size_t mn = (size_t)m * n;
+    T* yv = reinterpret_cast<T*>(lds);
+    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
+    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    T* Sl = reinterpret_cast<T*>(zslot + 16);                  // capacitance LU (r x r) and pivots
+    int* spl = reinterpret_cast<int*>(Sl + MAXR * MAXR);
+    char* bufs = zslot + WAVE_SMALL;
+    if (lane < r * r) Sl[lane] = S[(size_t)m * r * r + lane];
+    if (lane < r) spl[lane] = spiv[(size_t)m * r + lane];
+    for (int j0 = 0; j0 < n; j0 += 256) {   // c = P b (batched loads)
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = io_load_b<T>(io, mn, k, perm[min(j0 + 64 * u + lane, n - 1)]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + 64 * u + lane < n) yv[j0 + 64 * u + lane] = v[u];
+    }
+    if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    wave_sync();
+    wave_forward<T, 32>(Lf + mn * KL, piv + mn, yv, zslot, bufs, bufbytes, n, KL, lane);   // kl + 1 <= 17 lanes
+    wave_backward_any<T>(Uc + mn * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    if (r == 0) {
+        for (int j = lane; j < n; j += 64) io_store<T>(io, mn, k, j, yv[j]);
+        return;
+    }
+    const T* V0 = Vt + (size_t)m * r * n;
+    const T* W0 = W + (size_t)m * r * n;
+    T t[MAXR];
+    wave_vty<T>(V0, yv, n, r, lane, t);
+    cap_solve<T>(t, Sl, spl, r);
+    for (int j0 = 0; j0 < n; j0 += 64 * WCH) {   // x = y - W z (batched loads)
+        T wv[WCH][MAXR];
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = min(j0 + 64 * u + lane, n - 1);
+#pragma unroll
+            for (int a = 0; a < MAXR; ++a) wv[u][a] = W0[(size_t)min(a, max(r - 1, 0)) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < WCH; ++u) {
+            const int j = j0 + 64 * u + lane;
+            if (j < n) {
+                T v = yv[j];
+#pragma unroll
+                for (int a = 0; a < MAXR; ++a)
+                    if (a < r) v = cn::fnms(wv[u][a], t[a], v);
+                io_store<T>(io, mn, k, j, v);
+            }
+        }
+    }
+}
+
+// Border completion in wave mode: grid (B, r); W row a = U^-1 (its forward half), in place, and
+// column a of S = I + V^T W.
+template <class T>
+__global__ void __launch_bounds__(64)
+k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W, T* __restrict__ S, int n, int r,
+           int w2, int US, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x, a = blockIdx.y;
+    T* yv = reinterpret_cast<T*>(lds);
+    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
+    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    char* bufs = zslot + WAVE_SMALL;
+    T* Wa = W + ((size_t)m * r + a) * n;
+    wave_load<T>(yv, Wa, 1, n, lane);
+    if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    wave_sync();
+    wave_backward_any<T>(Uc + (size_t)m * n * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    for (int j = lane; j < n; j += 64) Wa[j] = yv[j];
+    T t[MAXR];
+    wave_vty<T>(Vt + (size_t)m * r * n, yv, n, r, lane, t);
+    if (lane == 0) {
+        T* Sm = S + (size_t)m * r * r + (size_t)a * r;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
+    }
+}
+
+// Wave mode, after the band LU: the columns of U scaled by their pivots, Uc[j][d] *= Uc[j][0]
+// (= 1 / U(j, j)) for d >= 1.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_scale_ucol(T* __restrict__ Uc, size_t total, int US) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int d = (int)(e % US);
+    if (d == 0) return;
+    Uc[e] = cn::mul(Uc[e], Uc[e - d]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Transposed solve A^T x = b (plain transpose), the discrete adjoint path (banded.hpp:346-372).
+// A^T = (B^T + V U^T) P, so y = P x solves (B^T + V U^T) y = b:
+//   y0 = B^-T b = P_B L^-T U^-T b,  z = St^-1 y0[:r],  y = y0 - Wt z,  x[perm[i]] = y[i],
+// with Wt = B^-T V and St = I + U^T Wt built once (prepare).  Lane = (system, rhs).
+//   phase 1: U^T w = b (forward; column j of U is read as U(j-d, j) = Uf[j-d][d]);
+//   phase 2: L^T and the row interchanges, backward (gbtrs 'T' order: dot, then swap);
+//   phase 3: Woodbury correction and the scatter through perm.
+// prep = 1 computes Wt instead: lane = (system, border index a), input row a of V^T, output and
+// work space row a of Wt, no phase 3.
+// ------------------------------------------------------------------------------------------
+template <class T, int KL, int WU>
+__global__ void __launch_bounds__(256)
+k_solve_t(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, T* __restrict__ Wt, const T* __restrict__ St,
+          const int* __restrict__ stpiv, const int* __restrict__ perm, T* __restrict__ ws,
+          T* __restrict__ x, int64_t rhs_stride, int B, int n, int r, int US, int nrhs, int prep, int il,
+          int ucol) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / nrhs;
+    const int k = gid - m * nrhs;
+    if (m >= B) return;
+    const size_t ch = m / il;
+    const int ml = m % il;
+    const T* U0 = Uf + ch * ((size_t)n * US * il) + ml;
+    const T* L0 = Lf + ch * ((size_t)n * KL * il) + ml;
+    const int* P0 = piv + ch * ((size_t)n * il) + ml;
+    // input row, work row (stride between consecutive entries)
+    const T* in;
+    size_t in_st;
+    T* w;
+    size_t w_st;
+    if (prep) {
+        in = Vt + ch * ((size_t)r * n * il) + ml + (size_t)k * n * il;
+        in_st = il;
+        w = Wt + ch * ((size_t)r * n * il) + ml + (size_t)k * n * il;
+        w_st = il;
+    } else {
+        in = x + k * rhs_stride + (size_t)m * n;
+        in_st = 1;
+        w = ws + gid;
+        w_st = (size_t)B * nrhs;
+    }
+    // phase 1: U^T w = b
+    T uw[WU];
+#pragma unroll
+    for (int d = 0; d < WU; ++d) uw[d] = cn::zero<T>();   // uw[d-1] = w_{j-d}
+    for (int j = 0; j < n; ++j) {
+        T wj;
+        if (!ucol) {               // row j-d entry d = U(j-d, j)
+            T acc = in[(size_t)j * in_st];
+#pragma unroll
+            for (int d = 1; d <= WU; ++d)
+                if (j - d >= 0) acc = cn::fnms(U0[((size_t)(j - d) * US + d) * il], uw[d - 1], acc);
+            wj = cn::mul(acc, U0[(size_t)j * US * il]);
+        } else {                   // column j entry d = U(j-d, j) / U(j, j)
+            T acc = cn::mul(in[(size_t)j * in_st], U0[(size_t)j * US * il]);
+#pragma unroll
+            for (int d = 1; d <= WU; ++d)
+                if (j - d >= 0) acc = cn::fnms(U0[((size_t)j * US + d) * il], uw[d - 1], acc);
+            wj = acc;
+        }
+        w[(size_t)j * w_st] = wj;
+#pragma unroll
+        for (int d = WU - 1; d > 0; --d) uw[d] = uw[d - 1];
+        uw[0] = wj;
+    }
+    // phase 2: L^T with interchanges, j = n-1 .. 0; window y[i] = entry j+i
+    T y[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) y[i] = cn::zero<T>();
+    for (int j = n - 1; j >= 0; --j) {
+        // entry j+1+KL leaves the window: no later step touches it, it is final
+        if (j + 1 + KL <= n - 1) w[(size_t)(j + 1 + KL) * w_st] = y[KL];
+#pragma unroll
+        for (int i = KL; i > 0; --i) y[i] = y[i - 1];
+        T v = w[(size_t)j * w_st];
+        const T* Lr = L0 + (size_t)j * KL * il;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) v = cn::fnms(Lr[(i - 1) * il], y[i], v);
+        y[0] = v;
+        const int p = P0[(size_t)j * il];
+        if (p != 0) cn::swap_first<T, KL>(y, p);
+    }
+#pragma unroll
+    for (int i = 0; i <= KL; ++i)
+        if (i < n) w[(size_t)i * w_st] = y[i];
+    if (prep) return;
+    // phase 3: z = St^-1 y0[:r]; y = y0 - Wt z; x[perm[i]] = y[i]
+    T z[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) z[a] = (a < r) ? w[(size_t)a * w_st] : cn::zero<T>();
+    if (r > 0) {
+        const T* Sm = St + (size_t)m * r * r;
+        const int* sp = stpiv + (size_t)m * r;
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            if (i >= r) break;
+            const int pi = sp[i];
+#pragma unroll
+            for (int q = i + 1; q < MAXR; ++q) {
+                const bool c = (q == pi);
+                const T zi = z[i], zq = z[q];
+                z[i] = cn::sel(c, zq, zi);
+                z[q] = cn::sel(c, zi, zq);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            if (i >= r) break;
+#pragma unroll
+            for (int q = 0; q < i; ++q) z[i] = cn::fnms(Sm[q * r + i], z[q], z[i]);
+        }
+#pragma unroll
+        for (int i = MAXR - 1; i >= 0; --i) {
+            if (i < r) {
+#pragma unroll
+                for (int q = i + 1; q < MAXR; ++q)
+                    if (q < r) z[i] = cn::fnms(Sm[q * r + i], z[q], z[i]);
+                z[i] = cn::mul(z[i], cn::recip(Sm[i * r + i]));
+            }
+        }
+    }
+    const T* W0 = Wt + ch * ((size_t)r * n * il) + ml;
+    T* xo = x + k * rhs_stride + (size_t)m * n;
+    for (int i = 0; i < n; ++i) {
+        T v = w[(size_t)i * w_st];
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+            if (a < r) v = cn::fnms(W0[((size_t)a * n + i) * il], z[a], v);
+        xo[perm[i]] = v;
+    }
+}
+
+// St = I + U^T Wt (column-major r x r per system): St(a, b) = delta_ab + Wt[b][a].
+template <class T>
+__global__ void __launch_bounds__(256)
+k_build_st(const T* __restrict__ Wt, T* __restrict__ St, int* __restrict__ info_t, int B, int n, int r, int il) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = gid / (r * r);
+    const int q = gid - m * r * r;
+    if (m >= B) return;
+    if (q == 0) info_t[m] = 0;
+    const int b = q / r, a = q - b * r;
+    const size_t ch = m / il;
+    const int ml = m % il;
+    T v = Wt[(ch * ((size_t)r * n) + (size_t)b * n + a) * il + ml];
+    if (a == b) v = cn::fmac(cn::one<T>(), cn::one<T>(), v);
+    St[(size_t)m * r * r + (size_t)b * r + a] = v;
+}
+
+
+}  // namespace cnk

@@ -7,6 +7,8 @@ the oracle's, fused gather/scatter and split-complex maps exact, bitwise determi
 runs in both solve modes of the library (CNHIP_MODE=lane: one lane per system and right-hand side
 over 16-system interleaved factors; CNHIP_MODE=wave: one wavefront per system and right-hand side,
 U stored by columns)."""
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -20,11 +22,15 @@ pytestmark = pytest.mark.gpu
 BE_TOL = 1e-13
 
 
-@pytest.fixture(autouse=True, params=["lane", "wave"])
+@pytest.fixture(autouse=True, params=["lane", "lane8", "wave"])
 def solve_mode(request, monkeypatch):
-    monkeypatch.setenv("CNHIP_MODE", request.param)
+    """lane: streamed sweeps in blocks of 16 steps where they fit in LDS; lane8: blocks of 8."""
+    mode = request.param.rstrip("8")
+    monkeypatch.setenv("CNHIP_MODE", mode)
+    if request.param == "lane8":
+        monkeypatch.setenv("CNHIP_SPX", "8")
     print(f"solve mode {request.param}", flush=True)
-    return request.param
+    return mode
 
 
 def _torch():
@@ -350,7 +356,7 @@ def test_core_shim_signatures(rng):
 
 def test_auto_mode_selection(rng, monkeypatch, solve_mode):
     """Without CNHIP_MODE: wave mode for a few systems, lane mode for >= 1024 narrow-band ones."""
-    if solve_mode != "lane":
+    if solve_mode != "lane" or os.environ.get("CNHIP_SPX"):
         pytest.skip("mode selection is checked once")
     monkeypatch.delenv("CNHIP_MODE")
     from conatus_amd.runtime.hip import BandedBatch

@@ -202,6 +202,30 @@ k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, i
     cond[m] = c;
 }
 
+// acc0 - sum_{d=1..WU} u(d) xw[d-1], the row step of the back substitution with U.  The terms
+// d >= 2 use solutions of earlier steps and go into up to four independent partial sums, so that
+// the dependent chain of one step is the d = 1 term alone (one complex fnms instead of WU).  Every
+// lane-mode back substitution sums in this order, so that all of them agree bit for bit.
+template <class T, int WU, class F>
+__device__ __forceinline__ T band_dot(T acc0, const T (&xw)[WU], F u) {
+    static_assert(WU >= 2, "band_dot needs at least two terms");
+    constexpr int NP = WU - 1 < 4 ? WU - 1 : 4;
+    T s[NP];
+#pragma unroll
+    for (int d = WU; d >= 2; --d) {
+        const int q = (WU - d) % NP;
+        if (WU - d < NP)
+            s[q] = cn::mul(u(d), xw[d - 1]);
+        else
+            s[q] = cn::fmac(u(d), xw[d - 1], s[q]);
+    }
+    T tot = s[0];
+    if constexpr (NP == 4) tot = cn::add(cn::add(s[0], s[1]), cn::add(s[2], s[3]));
+    else if constexpr (NP == 3) tot = cn::add(cn::add(s[0], s[1]), s[2]);
+    else if constexpr (NP == 2) tot = cn::add(s[0], s[1]);
+    return cn::fnms(u(1), xw[0], cn::sub(acc0, tot));
+}
+
 // ------------------------------------------------------------------------------------------
 // Solve kernel.  IO modes: 0 plain in place (x holds b), 1 mapped (rows/cols), 2 split complex.
 // ------------------------------------------------------------------------------------------
@@ -357,9 +381,7 @@ k_solve(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restric
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
     for (int j = n - 1; j >= 0; --j) {
         const T* Ur = U0 + (size_t)j * US * IL;
-        T acc = w0[(size_t)j * NL];
-#pragma unroll
-        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T acc = band_dot<T, WU>(w0[(size_t)j * NL], xw, [&](int d) { return Ur[d * IL]; });
         const T x = cn::mul(acc, Ur[0]);
         io_store<T>(io, mn, k, j, x);
 #pragma unroll
@@ -404,9 +426,7 @@ k_border(const T* __restrict__ Uf, const T* __restrict__ Vt, T* __restrict__ W, 
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
     for (int j = n - 1; j >= 0; --j) {
         const T* Ur = U0 + (size_t)j * US * IL;
-        T acc = W0[(size_t)j * IL];
-#pragma unroll
-        for (int d = 1; d <= WU; ++d) acc = cn::fnms(Ur[d * IL], xw[d - 1], acc);
+        const T acc = band_dot<T, WU>(W0[(size_t)j * IL], xw, [&](int d) { return Ur[d * IL]; });
         const T x = cn::mul(acc, Ur[0]);
         W0[(size_t)j * IL] = x;
 #pragma unroll
@@ -518,10 +538,9 @@ __device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, in
                                            T (&xw)[WU], T (&t)[MAXR]) {
     constexpr int CE = IL * (int)sizeof(T);
     const int bU = SPX * US * CE, bV = r * SPX * CE;
-    T acc = LaneSlot<T>::read(bb + bU + bV + sidx * LaneSlot<T>::BYTES, lane);
     const char* ur = bb + row * US * CE + mlx * (int)sizeof(T);
-#pragma unroll
-    for (int d = 1; d <= WU; ++d) acc = cn::fnms(lds_at<T>(ur + d * CE), xw[d - 1], acc);
+    const T acc = band_dot<T, WU>(LaneSlot<T>::read(bb + bU + bV + sidx * LaneSlot<T>::BYTES, lane), xw,
+                                  [&](int d) { return lds_at<T>(ur + d * CE); });
     const T x = cn::mul(acc, lds_at<T>(ur));
 #pragma unroll
     for (int a = 0; a < MAXR; ++a)

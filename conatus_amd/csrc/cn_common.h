@@ -34,6 +34,14 @@ __device__ __forceinline__ zc mul(zc a, zc b) {
     return zc{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
 
+// a + b
+__device__ __forceinline__ double add(double a, double b) { return a + b; }
+__device__ __forceinline__ zc add(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
+
+// a - b
+__device__ __forceinline__ double sub(double a, double b) { return a - b; }
+__device__ __forceinline__ zc sub(zc a, zc b) { return zc{a.re - b.re, a.im - b.im}; }
+
 // a - l * u
 __device__ __forceinline__ double fnms(double l, double u, double a) { return fma(-l, u, a); }
 __device__ __forceinline__ zc fnms(zc l, zc u, zc a) {

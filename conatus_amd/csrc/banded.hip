@@ -197,15 +197,32 @@ BorderFn border_fn(int WU) {
     return nullptr;
 }
 
+// streamed kernels for r border rows (compiled for rb_for(r) rows) and blocks of spx steps
 template <class T>
-SolveSFn solve_s_fn(int KL, int WU, int nr, int spx) {
-    if constexpr (sizeof(T) == 16) return spx == 16 ? lane_solve_zc16(KL, WU, nr) : spx == 8 ? lane_solve_zc8(KL, WU, nr) : nullptr;
-    else return spx == 16 ? lane_solve_d16(KL, WU, nr) : spx == 8 ? lane_solve_d8(KL, WU, nr) : nullptr;
+SolveSFn solve_s_fn(int KL, int WU, int nr, int spx, int r) {
+    const bool r2 = rb_for(r) == 2;
+    if (r > MAXR) return nullptr;
+    if constexpr (sizeof(T) == 16) {
+        if (spx == 16) return r2 ? lane_solve_zc16_r2(KL, WU, nr) : lane_solve_zc16_r8(KL, WU, nr);
+        if (spx == 8) return r2 ? lane_solve_zc8_r2(KL, WU, nr) : lane_solve_zc8_r8(KL, WU, nr);
+    } else {
+        if (spx == 16) return r2 ? lane_solve_d16_r2(KL, WU, nr) : lane_solve_d16_r8(KL, WU, nr);
+        if (spx == 8) return r2 ? lane_solve_d8_r2(KL, WU, nr) : lane_solve_d8_r8(KL, WU, nr);
+    }
+    return nullptr;
 }
 template <class T>
-BorderSFn border_s_fn(int WU, int nr, int spx) {
-    if constexpr (sizeof(T) == 16) return spx == 16 ? lane_border_zc16(WU, nr) : spx == 8 ? lane_border_zc8(WU, nr) : nullptr;
-    else return spx == 16 ? lane_border_d16(WU, nr) : spx == 8 ? lane_border_d8(WU, nr) : nullptr;
+BorderSFn border_s_fn(int WU, int nr, int spx, int r) {
+    const bool r2 = rb_for(r) == 2;
+    if (r > MAXR) return nullptr;
+    if constexpr (sizeof(T) == 16) {
+        if (spx == 16) return r2 ? lane_border_zc16_r2(WU, nr) : lane_border_zc16_r8(WU, nr);
+        if (spx == 8) return r2 ? lane_border_zc8_r2(WU, nr) : lane_border_zc8_r8(WU, nr);
+    } else {
+        if (spx == 16) return r2 ? lane_border_d16_r2(WU, nr) : lane_border_d16_r8(WU, nr);
+        if (spx == 8) return r2 ? lane_border_d8_r2(WU, nr) : lane_border_d8_r8(WU, nr);
+    }
+    return nullptr;
 }
 
 template <class T>
@@ -300,11 +317,13 @@ size_t wave_lds_bytes(const cn_bb& h) {
 
 // Block length of the streamed sweeps: 16 steps when two buffers fit in LDS, else 8 (0: none fits);
 // CNHIP_SPX=8 forces the short blocks (tests of both variants).
-int pick_spx(int buf16, int buf8) {
+// block length of a streamed sweep with double buffers of buf16 / buf8 bytes and extra16 /
+// extra8 further bytes of LDS
+int pick_spx(int buf16, int buf8, int extra16 = 0, int extra8 = 0) {
     const char* e = std::getenv("CNHIP_SPX");
     const bool force8 = e && e[0] == '8' && e[1] == 0;
-    if (!force8 && 2 * buf16 <= LDS_CAP) return 16;
-    return 2 * buf8 <= LDS_CAP ? 8 : 0;
+    if (!force8 && 2 * buf16 + extra16 <= LDS_CAP) return 16;
+    return 2 * buf8 + extra8 <= LDS_CAP ? 8 : 0;
 }
 
 // LDS bytes of one buffer of the streamed border completion (block length in spx), or -1
@@ -312,8 +331,9 @@ int border_bufbytes(const cn_bb& h, int& spx) {
     spx = 0;
     if (!h.streamed || h.WU > 16) return -1;
     const int e = (int)h.esz;
-    spx = pick_spx(bwd_bytes(16, h.US, h.r, e), bwd_bytes(8, h.US, h.r, e));
-    return spx ? bwd_bytes(spx, h.US, h.r, e) : -1;
+    const int rb = rb_for(h.r);
+    spx = pick_spx(bwd_bytes(16, h.US, rb, e), bwd_bytes(8, h.US, rb, e));
+    return spx ? bwd_bytes(spx, h.US, rb, e) : -1;
 }
 
 int nr_for(int r) { return r >= 4 ? 4 : (r >= 2 ? 2 : 1); }
@@ -352,7 +372,7 @@ void run_factor(cn_bb& h, hipStream_t s) {
     }
     int spx = 0;
     const int bb = h.mode == 1 ? -1 : border_bufbytes(h, spx);
-    BorderSFn fs = bb > 0 ? border_s_fn<T>(h.WU, nr_for(h.r), spx) : nullptr;
+    BorderSFn fs = bb > 0 ? border_s_fn<T>(h.WU, nr_for(h.r), spx, h.r) : nullptr;
     if (h.mode == 1) {
         // done above
     } else if (fs) {
@@ -512,8 +532,9 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     // wide-band systems.  CNHIP_MODE=lane|wave forces one where it applies.
     {
         const int e = (int)sizeof(T);
-        const int lane_buf = std::max({fwd_bytes(8, h.KLP, e), bwd_bytes(8, h.US, r, e), cor_bytes(8, r, e)});
-        const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf <= LDS_CAP;
+        const int rb = rb_for(r);
+        const int lane_buf = std::max({fwd_bytes(8, h.KLP, e), bwd_bytes(8, h.US, rb, e), cor_bytes(8, rb, e)});
+        const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf + seg_bytes(8, e) <= LDS_CAP;
         const bool wave_ok =
             2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e) <= (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
@@ -598,8 +619,9 @@ int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int& spx) {
     if (!(io.mode == 0 || (io.mode == 1 && !io.rows && !io.cols))) return -1;
     if (h.KLP + 1 > 8 || h.WU > 16) return -1;
     const int e = (int)h.esz;
-    auto buf = [&](int x) { return std::max({fwd_bytes(x, h.KLP, e), bwd_bytes(x, h.US, h.r, e), cor_bytes(x, h.r, e)}); };
-    spx = pick_spx(buf(16), buf(8));
+    const int rb = rb_for(h.r);
+    auto buf = [&](int x) { return std::max({fwd_bytes(x, h.KLP, e), bwd_bytes(x, h.US, rb, e), cor_bytes(x, rb, e)}); };
+    spx = pick_spx(buf(16), buf(8), seg_bytes(16, e), seg_bytes(8, e));
     return spx ? buf(spx) : -1;
 }
 
@@ -632,17 +654,17 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
     }
     int spx = 0;
     const int bufbytes = streamed_bufbytes(h, io, spx);
-    if (bufbytes > 0 && solve_s_fn<double>(h.KLP, h.WU, 1, spx)) {
+    if (bufbytes > 0 && solve_s_fn<double>(h.KLP, h.WU, 1, spx, h.r)) {
         // groups of up to 4 right-hand sides, one launch each (in place: rhs k at k * rhs_stride)
         char* xb = static_cast<char*>(io.x);
         const char* bb = static_cast<const char*>(io.mode == 0 ? io.x : io.rhs);
         const int64_t rs = io.mode == 0 ? io.rhs_stride : 0;
         for (int k0 = 0; k0 < nrhs;) {
             const int nr = nr_for(nrhs - k0);
-            SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nr, spx)
-                                               : solve_s_fn<double>(h.KLP, h.WU, nr, spx);
+            SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nr, spx, h.r)
+                                               : solve_s_fn<double>(h.KLP, h.WU, nr, spx, h.r);
             const size_t off = (size_t)k0 * (size_t)rs * h.esz;
-            fs(dim3((h.B + IL - 1) / IL), dim3(64), 2 * (size_t)bufbytes, s, h.Uf.p, h.Lf.p,
+            fs(dim3((h.B + IL - 1) / IL), dim3(64), 2 * (size_t)bufbytes + seg_bytes(spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
                static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p),
                static_cast<const int*>(h.perm.p), h.ws.p, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
             hip_check(hipGetLastError(), "k_solve_s launch");

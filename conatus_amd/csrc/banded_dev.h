@@ -21,15 +21,20 @@ using SolveSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const vo
 using BorderSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, void*, void*, int, int, int, int,
                            int);
 
-// launch tables of the streamed (lane mode) kernels, one translation unit per (type, block length)
-SolveSFn lane_solve_zc16(int KL, int WU, int nr);
-SolveSFn lane_solve_zc8(int KL, int WU, int nr);
-SolveSFn lane_solve_d16(int KL, int WU, int nr);
-SolveSFn lane_solve_d8(int KL, int WU, int nr);
-BorderSFn lane_border_zc16(int WU, int nr);
-BorderSFn lane_border_zc8(int WU, int nr);
-BorderSFn lane_border_d16(int WU, int nr);
-BorderSFn lane_border_d8(int WU, int nr);
+// launch tables of the streamed (lane mode) kernels, one translation unit per (type, block
+// length, border rows compiled for)
+#define CN_LANE_TABLES(tag)                                  \
+    SolveSFn lane_solve_##tag(int KL, int WU, int nr);       \
+    BorderSFn lane_border_##tag(int WU, int nr);
+CN_LANE_TABLES(zc16_r2)
+CN_LANE_TABLES(zc16_r8)
+CN_LANE_TABLES(zc8_r2)
+CN_LANE_TABLES(zc8_r8)
+CN_LANE_TABLES(d16_r2)
+CN_LANE_TABLES(d16_r8)
+CN_LANE_TABLES(d8_r2)
+CN_LANE_TABLES(d8_r8)
+#undef CN_LANE_TABLES
 
 // ------------------------------------------------------------------------------------------
 // Factor kernel: G lanes per system, 256/G systems per workgroup of 256 lanes.
@@ -483,6 +488,7 @@ struct LaneSlot<zc> {
     }
     __device__ static zc read(const char* slot, int lane) { return lds_at<zc>(slot + lane * 16); }
     __device__ static void zero(char* slot, int lane) { *reinterpret_cast<zc*>(slot + lane * 16) = zc{0.0, 0.0}; }
+    __device__ static void write(char* slot, int lane, zc v) { *reinterpret_cast<zc*>(slot + lane * 16) = v; }
 };
 template <>
 struct LaneSlot<double> {
@@ -501,27 +507,64 @@ struct LaneSlot<double> {
         *reinterpret_cast<unsigned*>(slot + lane * 4) = 0u;
         *reinterpret_cast<unsigned*>(slot + 256 + lane * 4) = 0u;
     }
+    __device__ static void write(char* slot, int lane, double v) {
+        *reinterpret_cast<unsigned*>(slot + lane * 4) = (unsigned)__double2loint(v);
+        *reinterpret_cast<unsigned*>(slot + 256 + lane * 4) = (unsigned)__double2hiint(v);
+    }
 };
+
+// Lane segments: a block of SPX consecutive entries of each lane's own vector (its segment) moves
+// between HBM and LDS as whole segments.  DMA instruction i carries the segments q = SPI*i + g
+// (g = L / SPX, SPI = 64 / SPX segments per instruction): lane L carries entry
+// e = (L mod SPX - g) mod SPX of segment q, so that one instruction touches SPI runs of SPX entries
+// instead of 64 scattered entries.  In LDS instruction i's slot sits at i * PITCH and entry e of
+// segment q at lane position g * SPX + (e + g) mod SPX; the rotation by g and the padding of the
+// pitch spread the simultaneous reads of one entry by all 64 lanes over the banks.
+template <class T, int SPX>
+struct Seg {
+    static constexpr int SPI = 64 / SPX;
+    static constexpr int PITCH = LaneSlot<T>::BYTES + 64 * (int)sizeof(T) / SPX;
+    static constexpr int BYTES = SPX * PITCH;                  // 64 segments
+    // entry of its segment that lane L carries in a DMA instruction
+    __device__ static int entry(int L) { return (L - L / SPX) & (SPX - 1); }
+    // slot offset and lane position of entry e of segment q
+    __device__ static int slot(int q) { return (q / SPI) * PITCH; }
+    __device__ static int pos(int q, int e) {
+        const int g = q % SPI;
+        return g * SPX + ((e + g) & (SPX - 1));
+    }
+};
+__host__ __device__ constexpr int seg_bytes(int SPX, int esz) {
+    return SPX * (64 * esz + 64 * esz / SPX);
+}
+
+// compiler ordering point for LDS data exchanged between the lanes of one wavefront (the LDS
+// operations of a wavefront execute in program order)
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 
 // LDS bytes of one buffer of each streamed sweep (blocks of SPX steps)
 __host__ __device__ constexpr int fwd_bytes(int SPX, int KL, int esz) {
-    return SPX * KL * IL * esz + SPX * IL * 4 + SPX * 64 * esz;
+    return SPX * KL * IL * esz + SPX * IL * 4 + seg_bytes(SPX, esz);
 }
-__host__ __device__ constexpr int bwd_bytes(int SPX, int US, int r, int esz) {
-    return (SPX * US + r * SPX) * IL * esz + SPX * 64 * esz;
+// (rb: the border rows the kernel is compiled for, rb >= r; rows r .. rb-1 of V^T and W are
+// zero pads in LDS, so that the steps carry no test on r)
+__host__ __device__ constexpr int bwd_bytes(int SPX, int US, int rb, int esz) {
+    return (SPX * US + rb * SPX) * IL * esz + SPX * 64 * esz;
 }
-__host__ __device__ constexpr int cor_bytes(int SPX, int r, int esz) { return r * SPX * IL * esz + SPX * 64 * esz; }
+__host__ __device__ constexpr int cor_bytes(int SPX, int rb, int esz) { return rb * SPX * IL * esz + SPX * 64 * esz; }
+// border rows a kernel is compiled for: 2 (r <= 2) or MAXR
+__host__ __device__ constexpr int rb_for(int r) { return r <= 2 ? 2 : MAXR; }
 
 // backward block blk (rows jhi = n-1-blk*SPX down to jlo) into buf:
-// [U: SPX rows x US][V^T: r x SPX][in: SPX lane slots, slot s = row jhi - s]
-template <class T, int SPX>
+// [U: SPX rows x US][V^T: RB x SPX (rows >= r: pads)][in: SPX lane slots, slot s = row jhi - s]
+template <class T, int SPX, int RB>
 __device__ __forceinline__ void bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
                                         int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
     constexpr int CE = IL * (int)sizeof(T);
     const int jhi = n - 1 - blk * SPX;
     const int jlo = max(0, jhi - SPX + 1);
     const int cnt = jhi - jlo + 1;
-    const int bU = SPX * US * CE, bV = r * SPX * CE;
+    const int bU = SPX * US * CE, bV = RB * SPX * CE;
     dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
     for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
     if (live) {
@@ -533,18 +576,17 @@ __device__ __forceinline__ void bwd_dma(const T* __restrict__ Uf, const T* __res
 
 // One backward step of a lane: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), t += V^T(:, j) out_j.
 // bb: the block buffer, row: j - jlo, sidx: slot of in_j.
-template <class T, int WU, int SPX>
-__device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, int US, int r, int mlx, int lane,
-                                           T (&xw)[WU], T (&t)[MAXR]) {
+template <class T, int WU, int SPX, int RB>
+__device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, int US, int mlx, int lane,
+                                           T (&xw)[WU], T (&t)[RB]) {
     constexpr int CE = IL * (int)sizeof(T);
-    const int bU = SPX * US * CE, bV = r * SPX * CE;
+    const int bU = SPX * US * CE, bV = RB * SPX * CE;
     const char* ur = bb + row * US * CE + mlx * (int)sizeof(T);
     const T acc = band_dot<T, WU>(LaneSlot<T>::read(bb + bU + bV + sidx * LaneSlot<T>::BYTES, lane), xw,
                                   [&](int d) { return lds_at<T>(ur + d * CE); });
     const T x = cn::mul(acc, lds_at<T>(ur));
 #pragma unroll
-    for (int a = 0; a < MAXR; ++a)
-        if (a < r) t[a] = cn::fmac(lds_at<T>(bb + bU + (a * SPX + row) * CE + mlx * (int)sizeof(T)), x, t[a]);
+    for (int a = 0; a < RB; ++a) t[a] = cn::fmac(lds_at<T>(bb + bU + (a * SPX + row) * CE + mlx * (int)sizeof(T)), x, t[a]);
 #pragma unroll
     for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
     xw[0] = x;
@@ -552,18 +594,19 @@ __device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, in
 }
 
 // Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
-// t[a] += V^T(a, j) out_j.  `in` and `out` may be the same array (in place).  Full blocks run
-// without per-step tests; a partial last block step by step.
-template <class T, int WU, int SPX>
+// t[a] += V^T(a, j) out_j (a < RB; the entries a >= r accumulate the LDS pads and are not used).
+// `in` and `out` may be the same array (in place).  Full blocks run without per-step tests; a
+// partial last block step by step.
+template <class T, int WU, int SPX, int RB>
 __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int bufbytes,
                                           size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
-                                          size_t out_st, bool live, int lane, int mlx, T (&t)[MAXR]) {
+                                          size_t out_st, bool live, int lane, int mlx, T (&t)[RB]) {
     const int nblk = (n + SPX - 1) / SPX;
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
     T o[SPX];
-    bwd_dma<T, SPX>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
+    bwd_dma<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
     dma_wait();
     for (int b = 0; b < nblk; ++b) {
         const int buf = b & 1;
@@ -574,15 +617,15 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
 #pragma unroll
             for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = o[s];
         }
-        if (b + 1 < nblk) bwd_dma<T, SPX>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
+        if (b + 1 < nblk) bwd_dma<T, SPX, RB>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
         const char* bb = lds + buf * bufbytes;
         if (cnt == SPX) {
 #pragma unroll
-            for (int s = 0; s < SPX; ++s) o[s] = bwd_lane_step<T, WU, SPX>(bb, SPX - 1 - s, s, US, r, mlx, lane, xw, t);
+            for (int s = 0; s < SPX; ++s) o[s] = bwd_lane_step<T, WU, SPX, RB>(bb, SPX - 1 - s, s, US, mlx, lane, xw, t);
         } else {
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
-                if (s < cnt) o[s] = bwd_lane_step<T, WU, SPX>(bb, cnt - 1 - s, s, US, r, mlx, lane, xw, t);
+                if (s < cnt) o[s] = bwd_lane_step<T, WU, SPX, RB>(bb, cnt - 1 - s, s, US, mlx, lane, xw, t);
         }
         dma_wait();
     }
@@ -610,19 +653,25 @@ __device__ __forceinline__ T fwd_lane_step(T (&y)[KL + 1], const char* Lrow, int
     return out;
 }
 
-// Streamed solve: forward sweep (pivots, L) into the work space, backward sweep (U) into x with
-// t = V^T y, z = S^-1 t, then the streamed pass x = y - W z.  Arithmetic as k_solve.
-// IO: in place (bin == xout, right-hand side k at k * rhs_stride) or contiguous out of place.
-// Forward blocks without any row interchange in the chunk skip the (select-based) interchanges.
-template <class T, int KL, int WU, int NR, int SPX>
+// Streamed solve: forward sweep (pivots, L) into the work space, backward sweep (U) in place in
+// the work space with t = V^T y, z = S^-1 t, then the streamed pass x = y - W z into x.
+// Arithmetic as k_solve.  IO: in place (bin == xout, right-hand side k at k * rhs_stride) or
+// contiguous out of place.  The work space holds entry j of every (system, rhs) lane of the
+// chunk in one contiguous row (coalesced); the right-hand side enters and x leaves by lane
+// segments (Seg): per block, each lane's SPX entries travel as one run.  Forward blocks without
+// any row interchange in the chunk skip the (select-based) interchanges.
+template <class T, int KL, int WU, int NR, int SPX, int RB>
 __global__ void __launch_bounds__(64)
 k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
           const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
           const T* bin, T* xout, int64_t rhs_stride, int B, int n, int r, int US, int bufbytes) {
     static_assert(KL + 1 <= SPX, "the window lookahead must stay within the next block");
+    using SG = Seg<T, SPX>;
     constexpr int CE = IL * (int)sizeof(T);
     constexpr int GB = LaneSlot<T>::BYTES;
+    constexpr int NI = IL * NR / SG::SPI;       // DMA instructions per block of segments
+    static_assert(IL * NR % SG::SPI == 0, "segments fill whole DMA instructions");
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x;
     const int mloc = lane / NR, k = lane - mloc * NR;
@@ -631,16 +680,24 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int m = (int)ch * IL + mloc;
     const bool live = mloc < IL && m < B;
     const size_t NL = (size_t)B * NR;
-    const T* bl = bin + k * rhs_stride + (size_t)m * n;
-    T* xl = xout + k * rhs_stride + (size_t)m * n;
     T* wl = ws + (size_t)m * NR + k;
     const int fL = SPX * KL * CE, fP = SPX * IL * 4;
     const int nblk = (n + SPX - 1) / SPX;
+    // this lane's part of the segment DMA: entry se of segment SPI*i + sg in instruction i, i.e.
+    // system ch*IL + (SPI/NR)*i + sg/NR, right-hand side sg % NR
+    const int sg = lane / SPX, se = SG::entry(lane);
+    const size_t seg_off = (size_t)(sg % NR) * rhs_stride + ((size_t)ch * IL + sg / NR) * n;
+    const size_t seg_step = (size_t)(SG::SPI / NR) * n;
+    const int seg_live = B - (int)ch * IL - sg / NR;   // instruction i is live if (SPI/NR)*i < seg_live
+    const int my_slot = SG::slot(lane);                // where this lane's own segment is read/written
 
     // ---- forward sweep: y_c = L^-1 P_B P b into the work space
+    const T* bl = bin + k * rhs_stride + (size_t)m * n;
     T y[KL + 1];
 #pragma unroll
     for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
+    // row of the right-hand side behind entry se of the next block (prefetched one block ahead)
+    int pc = (KL + 1 + se < n) ? perm[KL + 1 + se] : -1;
     T o[SPX];
     for (int b = -1; b < nblk; ++b) {
         const int buf = b & 1;
@@ -648,25 +705,28 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 #pragma unroll
             for (int s = 0; s < SPX; ++s) wl[(size_t)((b - 1) * SPX + s) * NL] = o[s];
         }
-        if (b + 1 < nblk) {     // block b+1: [L: SPX x KL][piv: SPX x 16][c: SPX lane slots]
+        if (b + 1 < nblk) {     // block b+1: [L: SPX x KL][piv: SPX x 16][c: segments]
             char* nb = lds + ((b + 1) & 1) * bufbytes;
             const int j0 = (b + 1) * SPX;
             const int cnt = min(SPX, n - j0);
             dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
             dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
+            char* cs = nb + fL + fP;
+            if (pc >= 0) {
+                const T* src = bin + seg_off + pc;
 #pragma unroll
-            for (int s = 0; s < SPX; ++s) {
-                const int jn = j0 + KL + 1 + s;
-                if (jn < n) {
-                    const int pj = perm[jn];
-                    if (live) LaneSlot<T>::load(bl + pj, nb + fL + fP + s * GB);
-                } else {
-                    LaneSlot<T>::zero(nb + fL + fP + s * GB, lane);
-                }
+                for (int i = 0; i < NI; ++i)
+                    if ((SG::SPI / NR) * i < seg_live) LaneSlot<T>::load(src + i * seg_step, cs + i * SG::PITCH);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NI; ++i) LaneSlot<T>::zero(cs + i * SG::PITCH, lane);
             }
+            const int jn = j0 + SPX + KL + 1 + se;    // entry se of block b+2
+            pc = jn < n ? perm[jn] : -1;
         }
         if (b >= 0) {
             const char* fb = lds + buf * bufbytes;
+            const char* cb = fb + fL + fP + my_slot;
             const int j0 = b * SPX, cnt = min(SPX, n - j0);
             if (cnt == SPX) {
                 // any row interchange in this block for the chunk's systems?
@@ -675,19 +735,20 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
                 if (__ballot(any != 0) == 0) {
 #pragma unroll
                     for (int s = 0; s < SPX; ++s)
-                        o[s] = fwd_lane_step<T, KL, false>(y, fb + s * KL * CE, 0, LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+                        o[s] = fwd_lane_step<T, KL, false>(y, fb + s * KL * CE, 0,
+                                                           LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
                 } else {
 #pragma unroll
                     for (int s = 0; s < SPX; ++s)
                         o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
-                                                          LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+                                                          LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
                 }
             } else {
 #pragma unroll
                 for (int s = 0; s < SPX; ++s)
                     if (s < cnt)
                         o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
-                                                          LaneSlot<T>::read(fb + fL + fP + s * GB, lane), mlx);
+                                                          LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
             }
         }
         dma_wait();
@@ -701,22 +762,47 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     }
     dma_wait();
 
-    // ---- backward sweep: y = U^-1 y_c into x, t = V^T y
-    T t[MAXR];
+    // ---- backward sweep: y = U^-1 y_c in place, t = V^T y
+    T t[RB];
 #pragma unroll
-    for (int a = 0; a < MAXR; ++a) t[a] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, xl, 1, live, lane, mlx, t);
-    if (r == 0) return;
+    for (int a = 0; a < RB; ++a) t[a] = cn::zero<T>();
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, wl, NL, live, lane, mlx, t);
 
-    // ---- z = S^-1 t, then x = y - W z: block [W: r x SPX][y: SPX lane slots]
-    if (live) cap_solve<T>(t, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
-    const int cW = r * SPX * CE;
+    // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
+    // slots]; x leaves by segments through the staging area behind the two buffers
+    {
+        T z[MAXR];   // the pad entries t[a >= r] are dropped: W z must not see them
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a) z[a] = cn::zero<T>();
+#pragma unroll
+        for (int a = 0; a < RB; ++a) z[a] = cn::sel(a < r, t[a], cn::zero<T>());
+        if (live && r > 0) cap_solve<T>(z, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+#pragma unroll
+        for (int a = 0; a < RB; ++a) t[a] = z[a];
+    }
+    for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in both buffers
+        *reinterpret_cast<T*>(lds + q * (int)sizeof(T)) = cn::zero<T>();
+        *reinterpret_cast<T*>(lds + bufbytes + q * (int)sizeof(T)) = cn::zero<T>();
+    }
+    const int cW = RB * SPX * CE;
+    char* stg = lds + 2 * bufbytes;
+    T* xseg = xout + seg_off + se;
+    // x entries j0 .. j0+cnt-1 of every lane from o[]: park them by segments, then store whole runs
+    auto store_block = [&](int j0, int cnt) {
+#pragma unroll
+        for (int s = 0; s < SPX; ++s)
+            if (s < cnt) LaneSlot<T>::write(stg + my_slot, SG::pos(lane, s), o[s]);
+        lds_order();
+        if (se < cnt) {
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                if ((SG::SPI / NR) * i < seg_live) xseg[i * seg_step + j0] = LaneSlot<T>::read(stg + i * SG::PITCH, lane);
+        }
+        lds_order();
+    };
     for (int b = -1; b < nblk; ++b) {
         const int buf = b & 1;
-        if (b >= 1 && live) {
-#pragma unroll
-            for (int s = 0; s < SPX; ++s) xl[(b - 1) * SPX + s] = o[s];
-        }
+        if (b >= 1) store_block((b - 1) * SPX, SPX);
         if (b + 1 < nblk) {
             char* nb = lds + ((b + 1) & 1) * bufbytes;
             const int j0 = (b + 1) * SPX;
@@ -725,7 +811,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             if (live) {
 #pragma unroll
                 for (int s = 0; s < SPX; ++s)
-                    if (j0 + s < n) LaneSlot<T>::load(xl + j0 + s, nb + cW + s * GB);
+                    if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
             }
         }
         if (b >= 0) {
@@ -734,25 +820,19 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             for (int s = 0; s < SPX; ++s) {
                 T v = LaneSlot<T>::read(cb + cW + s * GB, lane);
 #pragma unroll
-                for (int a = 0; a < MAXR; ++a)
-                    if (a < r) v = cn::fnms(lds_at<T>(cb + (a * SPX + s) * CE + mlx * (int)sizeof(T)), t[a], v);
+                for (int a = 0; a < RB; ++a) v = cn::fnms(lds_at<T>(cb + (a * SPX + s) * CE + mlx * (int)sizeof(T)), t[a], v);
                 o[s] = v;
             }
         }
         dma_wait();
     }
-    if (live) {
-#pragma unroll
-        for (int s = 0; s < SPX; ++s) {
-            const int j = (nblk - 1) * SPX + s;
-            if (j < n) xl[j] = o[s];
-        }
-    }
+    store_block((nblk - 1) * SPX, n - (nblk - 1) * SPX);
+    dma_wait();
 }
 
 // Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
 // (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s.
-template <class T, int WU, int NR, int SPX>
+template <class T, int WU, int NR, int SPX, int RB>
 __global__ void __launch_bounds__(64)
 k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
            int bufbytes) {
@@ -765,14 +845,14 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
     const int a = blockIdx.y * NR + k;
     const bool live = mloc < IL && m < B && a < r;
     T* wa = W + (ch * r * n + (size_t)(live ? a : 0) * n) * IL + mlx;
-    T t[MAXR];
+    T t[RB];
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, lane, mlx, t);
+    for (int q = 0; q < RB; ++q) t[q] = cn::zero<T>();
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, lane, mlx, t);
     if (!live) return;
     T* Sm = S + (size_t)m * r * r + (size_t)a * r;
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q)
+    for (int q = 0; q < RB; ++q)
         if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
 }
 

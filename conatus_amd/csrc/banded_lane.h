@@ -1,6 +1,7 @@
-// Launch tables of the lane-mode streamed kernels (k_solve_s, k_border_s) for one element type
-// and one block length; included by the banded_lane_*.hip translation units, which define
-// CN_LANE_T (element type), CN_LANE_SPX (steps per block) and the two table names.
+// Launch tables of the lane-mode streamed kernels (k_solve_s, k_border_s) for one element type,
+// one block length and one bound on the border rows; included by the banded_lane_*.hip
+// translation units, which define CN_LANE_T (element type), CN_LANE_SPX (steps per block),
+// CN_LANE_RB (border rows compiled for) and the two table names.
 #pragma once
 #include "banded_dev.h"
 
@@ -9,12 +10,13 @@ namespace {   // internal to each translation unit: the helpers below differ per
 
 using T = CN_LANE_T;
 constexpr int SPX = CN_LANE_SPX;
+constexpr int RB = CN_LANE_RB;
 
 template <int KL, int WU, int NR>
 void launch_solve_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Lf, const int* piv,
                     const void* Vt, const void* W, const void* S, const int* spiv, const int* perm, void* ws,
                     const void* bin, void* xout, int64_t rs, int B, int n, int r, int US, int bufbytes) {
-    hipLaunchKernelGGL((k_solve_s<T, KL, WU, NR, SPX>), g, b, lds, s, static_cast<const T*>(Uf),
+    hipLaunchKernelGGL((k_solve_s<T, KL, WU, NR, SPX, RB>), g, b, lds, s, static_cast<const T*>(Uf),
                        static_cast<const T*>(Lf), piv, static_cast<const T*>(Vt), static_cast<const T*>(W),
                        static_cast<const T*>(S), spiv, perm, static_cast<T*>(ws), static_cast<const T*>(bin),
                        static_cast<T*>(xout), rs, B, n, r, US, bufbytes);
@@ -41,7 +43,7 @@ SolveSFn solve_wu(int WU, int nr) {
 template <int WU, int NR>
 void launch_border_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Vt, void* W, void* S, int B,
                      int n, int r, int US, int bufbytes) {
-    hipLaunchKernelGGL((k_border_s<T, WU, NR, SPX>), g, b, lds, s, static_cast<const T*>(Uf),
+    hipLaunchKernelGGL((k_border_s<T, WU, NR, SPX, RB>), g, b, lds, s, static_cast<const T*>(Uf),
                        static_cast<const T*>(Vt), static_cast<T*>(W), static_cast<T*>(S), B, n, r, US, bufbytes);
 }
 template <int WU>

@@ -593,6 +593,41 @@ __device__ __forceinline__ T bwd_lane_step(const char* bb, int row, int sidx, in
     return x;
 }
 
+// LDS operands of one backward step in registers: U row (u[0] = 1 / U(j, j)), V^T column, in_j
+template <class T, int WU, int RB>
+struct BwdOps {
+    T u[WU + 1];
+    T v[RB];
+    T in;
+};
+template <class T, int WU, int SPX, int RB>
+__device__ __forceinline__ void bwd_ops_load(BwdOps<T, WU, RB>& op, const char* bb, int row, int sidx, int US, int mlx,
+                                             int lane) {
+    constexpr int CE = IL * (int)sizeof(T);
+    const int bU = SPX * US * CE, bV = RB * SPX * CE;
+    const char* ur = bb + row * US * CE + mlx * (int)sizeof(T);
+#pragma unroll
+    for (int d = 0; d <= WU; ++d) op.u[d] = lds_at<T>(ur + d * CE);
+#pragma unroll
+    for (int a = 0; a < RB; ++a) op.v[a] = lds_at<T>(bb + bU + (a * SPX + row) * CE + mlx * (int)sizeof(T));
+    op.in = LaneSlot<T>::read(bb + bU + bV + sidx * LaneSlot<T>::BYTES, lane);
+}
+// the arithmetic of bwd_lane_step on operands in registers
+template <class T, int WU, int RB>
+__device__ __forceinline__ T bwd_ops_step(const BwdOps<T, WU, RB>& op, T (&xw)[WU], T (&t)[RB]) {
+    const T acc = band_dot<T, WU>(op.in, xw, [&](int d) { return op.u[d]; });
+    const T x = cn::mul(acc, op.u[0]);
+#pragma unroll
+    for (int a = 0; a < RB; ++a) t[a] = cn::fmac(op.v[a], x, t[a]);
+#pragma unroll
+    for (int d = WU - 1; d > 0; --d) xw[d] = xw[d - 1];
+    xw[0] = x;
+    return x;
+}
+
+// scheduling fence: the LDS operands of the next step are read before this step's arithmetic
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 // Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
 // t[a] += V^T(a, j) out_j (a < RB; the entries a >= r accumulate the LDS pads and are not used).
 // `in` and `out` may be the same array (in place).  Full blocks run without per-step tests; a
@@ -619,9 +654,16 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         }
         if (b + 1 < nblk) bwd_dma<T, SPX, RB>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
         const char* bb = lds + buf * bufbytes;
-        if (cnt == SPX) {
+        if (cnt == SPX) {   // operands of step s+1 in flight during step s
+            BwdOps<T, WU, RB> op[2];
+            bwd_ops_load<T, WU, SPX, RB>(op[0], bb, SPX - 1, 0, US, mlx, lane);
 #pragma unroll
-            for (int s = 0; s < SPX; ++s) o[s] = bwd_lane_step<T, WU, SPX, RB>(bb, SPX - 1 - s, s, US, mlx, lane, xw, t);
+            for (int s = 0; s < SPX; ++s) {
+                if (s + 1 < SPX) bwd_ops_load<T, WU, SPX, RB>(op[(s + 1) & 1], bb, SPX - 2 - s, s + 1, US, mlx, lane);
+                sched_fence();
+                o[s] = bwd_ops_step<T, WU, RB>(op[s & 1], xw, t);
+                sched_fence();
+            }
         } else {
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
@@ -653,6 +695,50 @@ __device__ __forceinline__ T fwd_lane_step(T (&y)[KL + 1], const char* Lrow, int
     return out;
 }
 
+// LDS operands of one forward step in registers: multipliers, pivot offset, incoming row
+template <class T, int KL>
+struct FwdOps {
+    T l[KL];
+    T c;
+    int p;
+};
+template <class T, int KL, bool PIV>
+__device__ __forceinline__ void fwd_ops_load(FwdOps<T, KL>& op, const char* Lrow, const char* prow, const char* cslot,
+                                             int cpos, int mlx) {
+    constexpr int CE = IL * (int)sizeof(T);
+#pragma unroll
+    for (int i = 0; i < KL; ++i) op.l[i] = lds_at<T>(Lrow + i * CE + mlx * (int)sizeof(T));
+    op.c = LaneSlot<T>::read(cslot, cpos);
+    op.p = PIV ? lds_at<int>(prow + mlx * 4) : 0;
+}
+template <class T, int KL, bool PIV>
+__device__ __forceinline__ T fwd_ops_step(T (&y)[KL + 1], const FwdOps<T, KL>& op) {
+    if (PIV) cn::swap_first<T, KL>(y, op.p);
+#pragma unroll
+    for (int i = 1; i <= KL; ++i) y[i] = cn::fnms(op.l[i - 1], y[0], y[i]);
+    const T out = y[0];
+#pragma unroll
+    for (int i = 0; i < KL; ++i) y[i] = y[i + 1];
+    y[KL] = op.c;
+    return out;
+}
+// a full forward block, operands of step s+1 in flight during step s
+template <class T, int KL, int SPX, bool PIV, class CPos>
+__device__ __forceinline__ void fwd_block(T (&y)[KL + 1], T (&o)[SPX], const char* fb, int fL, const char* cb, CPos cpos,
+                                          int mlx) {
+    constexpr int CE = IL * (int)sizeof(T);
+    FwdOps<T, KL> op[2];
+    fwd_ops_load<T, KL, PIV>(op[0], fb, fb + fL, cb, cpos(0), mlx);
+#pragma unroll
+    for (int s = 0; s < SPX; ++s) {
+        if (s + 1 < SPX)
+            fwd_ops_load<T, KL, PIV>(op[(s + 1) & 1], fb + (s + 1) * KL * CE, fb + fL + (s + 1) * IL * 4, cb, cpos(s + 1), mlx);
+        sched_fence();
+        o[s] = fwd_ops_step<T, KL, PIV>(y, op[s & 1]);
+        sched_fence();
+    }
+}
+
 // Streamed solve: forward sweep (pivots, L) into the work space, backward sweep (U) in place in
 // the work space with t = V^T y, z = S^-1 t, then the streamed pass x = y - W z into x.
 // Arithmetic as k_solve.  IO: in place (bin == xout, right-hand side k at k * rhs_stride) or
@@ -661,7 +747,7 @@ __device__ __forceinline__ T fwd_lane_step(T (&y)[KL + 1], const char* Lrow, int
 // segments (Seg): per block, each lane's SPX entries travel as one run.  Forward blocks without
 // any row interchange in the chunk skip the (select-based) interchanges.
 template <class T, int KL, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
           const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
@@ -732,17 +818,11 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
                 // any row interchange in this block for the chunk's systems?
                 int any = 0;
                 for (int q = lane; q < SPX * IL; q += 64) any |= lds_at<int>(fb + fL + q * 4);
-                if (__ballot(any != 0) == 0) {
-#pragma unroll
-                    for (int s = 0; s < SPX; ++s)
-                        o[s] = fwd_lane_step<T, KL, false>(y, fb + s * KL * CE, 0,
-                                                           LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
-                } else {
-#pragma unroll
-                    for (int s = 0; s < SPX; ++s)
-                        o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
-                                                          LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
-                }
+                auto cpos = [&](int s) { return SG::pos(lane, s); };
+                if (__ballot(any != 0) == 0)
+                    fwd_block<T, KL, SPX, false>(y, o, fb, fL, cb, cpos, mlx);
+                else
+                    fwd_block<T, KL, SPX, true>(y, o, fb, fL, cb, cpos, mlx);
             } else {
 #pragma unroll
                 for (int s = 0; s < SPX; ++s)
@@ -833,7 +913,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 // Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
 // (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s.
 template <class T, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
            int bufbytes) {
     extern __shared__ __align__(16) char lds[];

@@ -315,16 +315,17 @@ size_t wave_lds_bytes(const cn_bb& h) {
     return 2 * (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h);
 }
 
-// Block length of the streamed sweeps: 16 steps when two buffers fit in LDS, else 8 (0: none fits);
-// CNHIP_SPX=8 forces the short blocks (tests of both variants).
-// block length of a streamed sweep with double buffers of buf16 / buf8 bytes and extra16 /
-// extra8 further bytes of LDS
-int pick_spx(int buf16, int buf8, int extra16 = 0, int extra8 = 0) {
+// Block length of the streamed sweeps: 16 steps when the two buffers of buf16 bytes and the two
+// stages fit in LDS, else 8 (0: none fits); CNHIP_SPX=8 forces the short blocks (tests of both
+// variants).
+int pick_spx(int buf16, int buf8, int esz) {
     const char* e = std::getenv("CNHIP_SPX");
     const bool force8 = e && e[0] == '8' && e[1] == 0;
-    if (!force8 && 2 * buf16 + extra16 <= LDS_CAP) return 16;
-    return 2 * buf8 + extra8 <= LDS_CAP ? 8 : 0;
+    if (!force8 && 2 * buf16 + 2 * stage_bytes(16, esz) <= LDS_CAP) return 16;
+    return 2 * buf8 + 2 * stage_bytes(8, esz) <= LDS_CAP ? 8 : 0;
 }
+// LDS bytes of a streamed launch: two buffers and two stages
+size_t streamed_lds(int bufbytes, int spx, int esz) { return 2 * (size_t)bufbytes + 2 * (size_t)stage_bytes(spx, esz); }
 
 // LDS bytes of one buffer of the streamed border completion (block length in spx), or -1
 int border_bufbytes(const cn_bb& h, int& spx) {
@@ -332,7 +333,7 @@ int border_bufbytes(const cn_bb& h, int& spx) {
     if (!h.streamed || h.WU > 16) return -1;
     const int e = (int)h.esz;
     const int rb = rb_for(h.r);
-    spx = pick_spx(bwd_bytes(16, h.US, rb, e), bwd_bytes(8, h.US, rb, e));
+    spx = pick_spx(bwd_bytes(16, h.US, rb, e), bwd_bytes(8, h.US, rb, e), e);
     return spx ? bwd_bytes(spx, h.US, rb, e) : -1;
 }
 
@@ -377,7 +378,8 @@ void run_factor(cn_bb& h, hipStream_t s) {
         // done above
     } else if (fs) {
         const int nr = nr_for(h.r);
-        fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(64), 2 * (size_t)bb, s, h.Uf.p, h.Vt.p, h.W.p, h.S.p,
+        fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(128), streamed_lds(bb, spx, (int)h.esz), s, h.Uf.p,
+           h.Vt.p, h.W.p, h.S.p,
            h.B, h.n, h.r, h.US, bb);
         hip_check(hipGetLastError(), "k_border_s launch");
     } else {
@@ -534,7 +536,7 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
         const int e = (int)sizeof(T);
         const int rb = rb_for(r);
         const int lane_buf = std::max({fwd_bytes(8, h.KLP, e), bwd_bytes(8, h.US, rb, e), cor_bytes(8, rb, e)});
-        const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf + seg_bytes(8, e) <= LDS_CAP;
+        const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf + 2 * stage_bytes(8, e) <= LDS_CAP;
         const bool wave_ok =
             2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e) <= (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
@@ -621,7 +623,7 @@ int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int& spx) {
     const int e = (int)h.esz;
     const int rb = rb_for(h.r);
     auto buf = [&](int x) { return std::max({fwd_bytes(x, h.KLP, e), bwd_bytes(x, h.US, rb, e), cor_bytes(x, rb, e)}); };
-    spx = pick_spx(buf(16), buf(8), seg_bytes(16, e), seg_bytes(8, e));
+    spx = pick_spx(buf(16), buf(8), e);
     return spx ? buf(spx) : -1;
 }
 
@@ -664,7 +666,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
             SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nr, spx, h.r)
                                                : solve_s_fn<double>(h.KLP, h.WU, nr, spx, h.r);
             const size_t off = (size_t)k0 * (size_t)rs * h.esz;
-            fs(dim3((h.B + IL - 1) / IL), dim3(64), 2 * (size_t)bufbytes + seg_bytes(spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
+            fs(dim3((h.B + IL - 1) / IL), dim3(128), streamed_lds(bufbytes, spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
                static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p),
                static_cast<const int*>(h.perm.p), h.ws.p, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
             hip_check(hipGetLastError(), "k_solve_s launch");

@@ -628,56 +628,83 @@ __device__ __forceinline__ T bwd_ops_step(const BwdOps<T, WU, RB>& op, T (&xw)[W
 // scheduling fence: the LDS operands of the next step are read before this step's arithmetic
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
+// ------------------------------------------------------------------------------------------
+// Two-wave workgroups of the streamed sweeps.  Wave 0 (compute) runs the recurrences from LDS
+// and leaves its results in an LDS stage; wave 1 (memory) issues every HBM access: the copies
+// of block b+1 into the other buffer and the stores of block b-1 from the other stage while
+// block b is computed.  A wave that issues memory instructions stalls while the CU's share of
+// the HBM bandwidth drains them; with the roles split, that stall no longer holds up the
+// arithmetic, and one workgroup barrier per block hands the buffers over.
+// LDS: [buffer 0][buffer 1][stage 0][stage 1], stage = STAGE_BYTES (the larger of SPX lane slots
+// and the segment layout).
+// ------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int stage_bytes(int SPX, int esz) { return seg_bytes(SPX, esz); }
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
 // t[a] += V^T(a, j) out_j (a < RB; the entries a >= r accumulate the LDS pads and are not used).
-// `in` and `out` may be the same array (in place).  Full blocks run without per-step tests; a
-// partial last block step by step.
+// `in` and `out` may be the same array (in place).  Both waves call it (t is the compute wave's).
+// Full blocks run without per-step tests; a partial last block step by step.
 template <class T, int WU, int SPX, int RB>
 __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int bufbytes,
                                           size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
-                                          size_t out_st, bool live, int lane, int mlx, T (&t)[RB]) {
+                                          size_t out_st, bool live, int wave, int lane, int mlx, T (&t)[RB]) {
+    constexpr int GB = LaneSlot<T>::BYTES;
     const int nblk = (n + SPX - 1) / SPX;
+    char* stage = lds + 2 * bufbytes;
+    const int SB = stage_bytes(SPX, (int)sizeof(T));
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    T o[SPX];
-    bwd_dma<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
-    dma_wait();
+    if (wave == 1) {
+        bwd_dma<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
+        dma_wait();
+    }
+    __syncthreads();
     for (int b = 0; b < nblk; ++b) {
         const int buf = b & 1;
         const int jhi = n - 1 - b * SPX;
         const int jlo = max(0, jhi - SPX + 1);
         const int cnt = jhi - jlo + 1;
-        if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
+        if (wave == 1) {
+            if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
+                const char* sb = stage + (buf ^ 1) * SB;
 #pragma unroll
-            for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = o[s];
-        }
-        if (b + 1 < nblk) bwd_dma<T, SPX, RB>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
-        const char* bb = lds + buf * bufbytes;
-        if (cnt == SPX) {   // operands of step s+1 in flight during step s
-            BwdOps<T, WU, RB> op[2];
-            bwd_ops_load<T, WU, SPX, RB>(op[0], bb, SPX - 1, 0, US, mlx, lane);
-#pragma unroll
-            for (int s = 0; s < SPX; ++s) {
-                if (s + 1 < SPX) bwd_ops_load<T, WU, SPX, RB>(op[(s + 1) & 1], bb, SPX - 2 - s, s + 1, US, mlx, lane);
-                sched_fence();
-                o[s] = bwd_ops_step<T, WU, RB>(op[s & 1], xw, t);
-                sched_fence();
+                for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
+            if (b + 1 < nblk) bwd_dma<T, SPX, RB>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
+            dma_wait();
         } else {
+            const char* bb = lds + buf * bufbytes;
+            char* sb = stage + buf * SB;
+            if (cnt == SPX) {   // operands of step s+1 in flight during step s
+                BwdOps<T, WU, RB> op[2];
+                bwd_ops_load<T, WU, SPX, RB>(op[0], bb, SPX - 1, 0, US, mlx, lane);
 #pragma unroll
-            for (int s = 0; s < SPX; ++s)
-                if (s < cnt) o[s] = bwd_lane_step<T, WU, SPX, RB>(bb, cnt - 1 - s, s, US, mlx, lane, xw, t);
+                for (int s = 0; s < SPX; ++s) {
+                    if (s + 1 < SPX) bwd_ops_load<T, WU, SPX, RB>(op[(s + 1) & 1], bb, SPX - 2 - s, s + 1, US, mlx, lane);
+                    sched_fence();
+                    LaneSlot<T>::write(sb + s * GB, lane, bwd_ops_step<T, WU, RB>(op[s & 1], xw, t));
+                    sched_fence();
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SPX; ++s)
+                    if (s < cnt) LaneSlot<T>::write(sb + s * GB, lane, bwd_lane_step<T, WU, SPX, RB>(bb, cnt - 1 - s, s, US, mlx, lane, xw, t));
+            }
         }
-        dma_wait();
+        __syncthreads();
     }
-    if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
+    if (wave == 1 && live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
         const int jhi = n - 1 - (nblk - 1) * SPX;
+        const char* sb = stage + ((nblk - 1) & 1) * SB;
 #pragma unroll
         for (int s = 0; s < SPX; ++s)
-            if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = o[s];
+            if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
     }
-    dma_wait();
+    if (wave == 1) dma_wait();
+    __syncthreads();
 }
 
 // One forward step of a lane (window y[0..KL] = rows j .. j+KL): optional row interchange,
@@ -722,10 +749,10 @@ __device__ __forceinline__ T fwd_ops_step(T (&y)[KL + 1], const FwdOps<T, KL>& o
     y[KL] = op.c;
     return out;
 }
-// a full forward block, operands of step s+1 in flight during step s
+// a full forward block, operands of step s+1 in flight during step s; result s into stage slot s
 template <class T, int KL, int SPX, bool PIV, class CPos>
-__device__ __forceinline__ void fwd_block(T (&y)[KL + 1], T (&o)[SPX], const char* fb, int fL, const char* cb, CPos cpos,
-                                          int mlx) {
+__device__ __forceinline__ void fwd_block(T (&y)[KL + 1], char* sb, const char* fb, int fL, const char* cb, CPos cpos,
+                                          int mlx, int lane) {
     constexpr int CE = IL * (int)sizeof(T);
     FwdOps<T, KL> op[2];
     fwd_ops_load<T, KL, PIV>(op[0], fb, fb + fL, cb, cpos(0), mlx);
@@ -734,7 +761,7 @@ __device__ __forceinline__ void fwd_block(T (&y)[KL + 1], T (&o)[SPX], const cha
         if (s + 1 < SPX)
             fwd_ops_load<T, KL, PIV>(op[(s + 1) & 1], fb + (s + 1) * KL * CE, fb + fL + (s + 1) * IL * 4, cb, cpos(s + 1), mlx);
         sched_fence();
-        o[s] = fwd_ops_step<T, KL, PIV>(y, op[s & 1]);
+        LaneSlot<T>::write(sb + s * LaneSlot<T>::BYTES, lane, fwd_ops_step<T, KL, PIV>(y, op[s & 1]));
         sched_fence();
     }
 }
@@ -745,9 +772,10 @@ __device__ __forceinline__ void fwd_block(T (&y)[KL + 1], T (&o)[SPX], const cha
 // contiguous out of place.  The work space holds entry j of every (system, rhs) lane of the
 // chunk in one contiguous row (coalesced); the right-hand side enters and x leaves by lane
 // segments (Seg): per block, each lane's SPX entries travel as one run.  Forward blocks without
-// any row interchange in the chunk skip the (select-based) interchanges.
+// any row interchange in the chunk skip the (select-based) interchanges.  Two waves: lanes
+// 0..63 compute, lanes 64..127 move the data (same lane mapping).
 template <class T, int KL, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
           const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
@@ -759,7 +787,8 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     constexpr int NI = IL * NR / SG::SPI;       // DMA instructions per block of segments
     static_assert(IL * NR % SG::SPI == 0, "segments fill whole DMA instructions");
     extern __shared__ __align__(16) char lds[];
-    const int lane = threadIdx.x;
+    const int wave = wave_id();
+    const int lane = threadIdx.x & 63;
     const int mloc = lane / NR, k = lane - mloc * NR;
     const int mlx = mloc & (IL - 1);            // LDS column of this lane's system (dead lanes alias a live one)
     const size_t ch = blockIdx.x;
@@ -769,8 +798,10 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     T* wl = ws + (size_t)m * NR + k;
     const int fL = SPX * KL * CE, fP = SPX * IL * 4;
     const int nblk = (n + SPX - 1) / SPX;
-    // this lane's part of the segment DMA: entry se of segment SPI*i + sg in instruction i, i.e.
-    // system ch*IL + (SPI/NR)*i + sg/NR, right-hand side sg % NR
+    char* stage = lds + 2 * bufbytes;
+    constexpr int SB = stage_bytes(SPX, (int)sizeof(T));
+    // the memory wave's part of the segment DMA: entry se of segment SPI*i + sg in instruction i,
+    // i.e. system ch*IL + (SPI/NR)*i + sg/NR, right-hand side sg % NR
     const int sg = lane / SPX, se = SG::entry(lane);
     const size_t seg_off = (size_t)(sg % NR) * rhs_stride + ((size_t)ch * IL + sg / NR) * n;
     const size_t seg_step = (size_t)(SG::SPI / NR) * n;
@@ -778,41 +809,54 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int my_slot = SG::slot(lane);                // where this lane's own segment is read/written
 
     // ---- forward sweep: y_c = L^-1 P_B P b into the work space
-    const T* bl = bin + k * rhs_stride + (size_t)m * n;
     T y[KL + 1];
-#pragma unroll
-    for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
-    // row of the right-hand side behind entry se of the next block (prefetched one block ahead)
+    // memory wave: copy of forward block bn into buffer nb; pc = row of the right-hand side behind
+    // entry se of block bn, on return the one of block bn+1
     int pc = (KL + 1 + se < n) ? perm[KL + 1 + se] : -1;
-    T o[SPX];
-    for (int b = -1; b < nblk; ++b) {
+    auto fwd_copy = [&](int bn, char* nb) {
+        const int j0 = bn * SPX;
+        const int cnt = min(SPX, n - j0);
+        dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
+        dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
+        char* cs = nb + fL + fP;
+        if (pc >= 0) {
+            const T* src = bin + seg_off + pc;
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                if ((SG::SPI / NR) * i < seg_live) LaneSlot<T>::load(src + i * seg_step, cs + i * SG::PITCH);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NI; ++i) LaneSlot<T>::zero(cs + i * SG::PITCH, lane);
+        }
+        const int jn = j0 + SPX + KL + 1 + se;    // entry se of block bn+1
+        pc = jn < n ? perm[jn] : -1;
+    };
+    auto fwd_store = [&](int bs) {                // forward results of block bs from its stage
+        const char* sb = stage + (bs & 1) * SB;
+        const int j0 = bs * SPX;
+#pragma unroll
+        for (int s = 0; s < SPX; ++s)
+            if (j0 + s < n) wl[(size_t)(j0 + s) * NL] = LaneSlot<T>::read(sb + s * GB, lane);
+    };
+    if (wave == 0) {
+        const T* bl = bin + k * rhs_stride + (size_t)m * n;
+#pragma unroll
+        for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
+    } else {
+        fwd_copy(0, lds);
+        dma_wait();
+    }
+    __syncthreads();
+    for (int b = 0; b < nblk; ++b) {
         const int buf = b & 1;
-        if (b >= 1 && live) {   // results of block b-1
-#pragma unroll
-            for (int s = 0; s < SPX; ++s) wl[(size_t)((b - 1) * SPX + s) * NL] = o[s];
-        }
-        if (b + 1 < nblk) {     // block b+1: [L: SPX x KL][piv: SPX x 16][c: segments]
-            char* nb = lds + ((b + 1) & 1) * bufbytes;
-            const int j0 = (b + 1) * SPX;
-            const int cnt = min(SPX, n - j0);
-            dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
-            dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
-            char* cs = nb + fL + fP;
-            if (pc >= 0) {
-                const T* src = bin + seg_off + pc;
-#pragma unroll
-                for (int i = 0; i < NI; ++i)
-                    if ((SG::SPI / NR) * i < seg_live) LaneSlot<T>::load(src + i * seg_step, cs + i * SG::PITCH);
-            } else {
-#pragma unroll
-                for (int i = 0; i < NI; ++i) LaneSlot<T>::zero(cs + i * SG::PITCH, lane);
-            }
-            const int jn = j0 + SPX + KL + 1 + se;    // entry se of block b+2
-            pc = jn < n ? perm[jn] : -1;
-        }
-        if (b >= 0) {
+        if (wave == 1) {
+            if (b >= 1 && live) fwd_store(b - 1);
+            if (b + 1 < nblk) fwd_copy(b + 1, lds + (buf ^ 1) * bufbytes);
+            dma_wait();
+        } else {
             const char* fb = lds + buf * bufbytes;
             const char* cb = fb + fL + fP + my_slot;
+            char* sb = stage + buf * SB;
             const int j0 = b * SPX, cnt = min(SPX, n - j0);
             if (cnt == SPX) {
                 // any row interchange in this block for the chunk's systems?
@@ -820,37 +864,36 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
                 for (int q = lane; q < SPX * IL; q += 64) any |= lds_at<int>(fb + fL + q * 4);
                 auto cpos = [&](int s) { return SG::pos(lane, s); };
                 if (__ballot(any != 0) == 0)
-                    fwd_block<T, KL, SPX, false>(y, o, fb, fL, cb, cpos, mlx);
+                    fwd_block<T, KL, SPX, false>(y, sb, fb, fL, cb, cpos, mlx, lane);
                 else
-                    fwd_block<T, KL, SPX, true>(y, o, fb, fL, cb, cpos, mlx);
+                    fwd_block<T, KL, SPX, true>(y, sb, fb, fL, cb, cpos, mlx, lane);
             } else {
 #pragma unroll
                 for (int s = 0; s < SPX; ++s)
                     if (s < cnt)
-                        o[s] = fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
-                                                          LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx);
+                        LaneSlot<T>::write(sb + s * GB, lane,
+                                           fwd_lane_step<T, KL, true>(y, fb + s * KL * CE, lds_at<int>(fb + fL + (s * IL + mlx) * 4),
+                                                                      LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx));
             }
         }
+        __syncthreads();
+    }
+    if (wave == 1) {
+        if (live) fwd_store(nblk - 1);
         dma_wait();
     }
-    if (live) {
-#pragma unroll
-        for (int s = 0; s < SPX; ++s) {
-            const int j = (nblk - 1) * SPX + s;
-            if (j < n) wl[(size_t)j * NL] = o[s];
-        }
-    }
-    dma_wait();
+    __syncthreads();
 
     // ---- backward sweep: y = U^-1 y_c in place, t = V^T y
     T t[RB];
 #pragma unroll
     for (int a = 0; a < RB; ++a) t[a] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, wl, NL, live, lane, mlx, t);
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t);
 
     // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
-    // slots]; x leaves by segments through the staging area behind the two buffers
-    {
+    // slots]; x leaves by segments through the stages
+    const int cW = RB * SPX * CE;
+    if (wave == 0) {
         T z[MAXR];   // the pad entries t[a >= r] are dropped: W z must not see them
 #pragma unroll
         for (int a = 0; a < MAXR; ++a) z[a] = cn::zero<T>();
@@ -859,65 +902,70 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         if (live && r > 0) cap_solve<T>(z, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
 #pragma unroll
         for (int a = 0; a < RB; ++a) t[a] = z[a];
+        for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in both buffers
+            *reinterpret_cast<T*>(lds + q * (int)sizeof(T)) = cn::zero<T>();
+            *reinterpret_cast<T*>(lds + bufbytes + q * (int)sizeof(T)) = cn::zero<T>();
+        }
     }
-    for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in both buffers
-        *reinterpret_cast<T*>(lds + q * (int)sizeof(T)) = cn::zero<T>();
-        *reinterpret_cast<T*>(lds + bufbytes + q * (int)sizeof(T)) = cn::zero<T>();
-    }
-    const int cW = RB * SPX * CE;
-    char* stg = lds + 2 * bufbytes;
     T* xseg = xout + seg_off + se;
-    // x entries j0 .. j0+cnt-1 of every lane from o[]: park them by segments, then store whole runs
-    auto store_block = [&](int j0, int cnt) {
+    auto cor_copy = [&](int bn, char* nb) {
+        const int j0 = bn * SPX;
+        const int cnt = min(SPX, n - j0);
+        for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
+        if (live) {
 #pragma unroll
-        for (int s = 0; s < SPX; ++s)
-            if (s < cnt) LaneSlot<T>::write(stg + my_slot, SG::pos(lane, s), o[s]);
-        lds_order();
-        if (se < cnt) {
+            for (int s = 0; s < SPX; ++s)
+                if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
+        }
+    };
+    auto cor_store = [&](int bs) {                // x entries of block bs from its stage, whole segment runs
+        const char* sb = stage + (bs & 1) * SB;
+        const int j0 = bs * SPX;
+        if (j0 + se < n) {
 #pragma unroll
             for (int i = 0; i < NI; ++i)
-                if ((SG::SPI / NR) * i < seg_live) xseg[i * seg_step + j0] = LaneSlot<T>::read(stg + i * SG::PITCH, lane);
+                if ((SG::SPI / NR) * i < seg_live) xseg[i * seg_step + j0] = LaneSlot<T>::read(sb + i * SG::PITCH, lane);
         }
-        lds_order();
     };
-    for (int b = -1; b < nblk; ++b) {
+    if (wave == 1) {
+        cor_copy(0, lds);
+        dma_wait();
+    }
+    __syncthreads();
+    for (int b = 0; b < nblk; ++b) {
         const int buf = b & 1;
-        if (b >= 1) store_block((b - 1) * SPX, SPX);
-        if (b + 1 < nblk) {
-            char* nb = lds + ((b + 1) & 1) * bufbytes;
-            const int j0 = (b + 1) * SPX;
-            const int cnt = min(SPX, n - j0);
-            for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
-            if (live) {
-#pragma unroll
-                for (int s = 0; s < SPX; ++s)
-                    if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
-            }
-        }
-        if (b >= 0) {
+        if (wave == 1) {
+            if (b >= 1) cor_store(b - 1);
+            if (b + 1 < nblk) cor_copy(b + 1, lds + (buf ^ 1) * bufbytes);
+            dma_wait();
+        } else {
             const char* cb = lds + buf * bufbytes;
+            char* sb = stage + buf * SB + my_slot;
 #pragma unroll
             for (int s = 0; s < SPX; ++s) {
                 T v = LaneSlot<T>::read(cb + cW + s * GB, lane);
 #pragma unroll
                 for (int a = 0; a < RB; ++a) v = cn::fnms(lds_at<T>(cb + (a * SPX + s) * CE + mlx * (int)sizeof(T)), t[a], v);
-                o[s] = v;
+                LaneSlot<T>::write(sb, SG::pos(lane, s), v);
             }
         }
+        __syncthreads();
+    }
+    if (wave == 1) {
+        cor_store(nblk - 1);
         dma_wait();
     }
-    store_block((nblk - 1) * SPX, n - (nblk - 1) * SPX);
-    dma_wait();
 }
 
 // Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
-// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s.
+// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s (two waves).
 template <class T, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
            int bufbytes) {
     extern __shared__ __align__(16) char lds[];
-    const int lane = threadIdx.x;
+    const int wave = wave_id();
+    const int lane = threadIdx.x & 63;
     const int mloc = lane / NR, k = lane - mloc * NR;
     const int mlx = mloc & (IL - 1);
     const size_t ch = blockIdx.x;
@@ -928,8 +976,8 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
     T t[RB];
 #pragma unroll
     for (int q = 0; q < RB; ++q) t[q] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, lane, mlx, t);
-    if (!live) return;
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, wave, lane, mlx, t);
+    if (wave != 0 || !live) return;
     T* Sm = S + (size_t)m * r * r + (size_t)a * r;
 #pragma unroll
     for (int q = 0; q < RB; ++q)

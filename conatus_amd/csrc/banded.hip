@@ -54,6 +54,8 @@ using namespace cnk;
 namespace {
 
 thread_local std::string g_err;
+// kernel mode of the handles this thread creates next: 0 automatic, 1 lane, 2 wave
+thread_local int g_mode_request = 0;
 
 struct CnError {
     int code;
@@ -540,7 +542,7 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
         const bool wave_ok =
             2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e) <= (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
-        const std::string want = em ? em : "";
+        const std::string want = g_mode_request == 1 ? "lane" : g_mode_request == 2 ? "wave" : em ? em : "";
         if (want == "wave") h.mode = wave_ok ? 1 : 0;
         else if (want == "lane") h.mode = 0;
         else h.mode = (lane_fast && B >= 1024) || !wave_ok ? 0 : 1;
@@ -851,5 +853,14 @@ int cn_bb_cond(cn_bb_t h, double* out) {
 }
 
 void cn_bb_destroy(cn_bb_t h) { delete h; }
+
+int cn_bb_request_mode(int mode) {
+    if (mode < 0 || mode > 2) {
+        g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane) or 2 (wave)";
+        return CN_EINVAL;
+    }
+    g_mode_request = mode;
+    return CN_OK;
+}
 
 }  // extern "C"

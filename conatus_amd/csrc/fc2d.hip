@@ -4,8 +4,8 @@
 // 3 outputs, derivative chains, backward transforms, products, forward transforms, C^(lam)
 // conversion into the stacked rows).
 //
-// Two launches:
-//   A  one workgroup per grid line z_j (Mz workgroups), everything in LDS:
+// Two launches (grid y: ensemble member, the members' states (K, n) one after another):
+//   A  one workgroup per grid line z_j (Mz workgroups per member), everything in LDS:
 //      z-backward of the 6 factor columns of the modes (u, u_z, w, w_z, T, T_z; the z-derivative
 //      is folded into the matrix Bz_1 = T_n'(z_j)), x-backward of the 8 factors u, u_x, u_z, w,
 //      w_x, w_z, T_x, T_z at the Mx points of the line (x-derivative i k kappa on the modes), the 3
@@ -36,6 +36,8 @@ k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* 
     double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx] products on the line
     double* b = G + 3 * Mx;                          // [2][N]: T_n(z_j), T_n'(z_j)
     const int j = blockIdx.x, tid = threadIdx.x;
+    X += (size_t)blockIdx.y * K * n;
+    Gh += (size_t)blockIdx.y * 3 * K * Mz;
     for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wt[q];
     for (int q = tid; q < 2 * N; q += blockDim.x) {
         const int v = q / N, nn = q - v * N;
@@ -116,6 +118,8 @@ k_fc_mode_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __r
     extern __shared__ __align__(16) char lds[];
     zc* g = reinterpret_cast<zc*>(lds);              // [3][Mz]: Gh[e][k][:]
     const int k = blockIdx.x, tid = threadIdx.x;
+    Gh += (size_t)blockIdx.y * 3 * K * Mz;
+    F += (size_t)blockIdx.y * K * n;
     for (int q = tid; q < 3 * Mz; q += blockDim.x) {
         const int e = q / Mz, jj = q - e * Mz;
         g[q] = Gh[((size_t)e * K + k) * Mz + jj];
@@ -139,13 +143,14 @@ k_fc_mode_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __r
 
 extern "C" {
 
-int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
-                     const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream) {
+int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
+                          const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, int members,
+                          void* stream) {
     if (!X || !fidx || !F || !Gh || !Bz || !Wx || !Az) {
         cn::set_error("cn_rb2d_explicit: null pointer");
         return CN_EINVAL;
     }
-    if (K < 1 || Nz < 2 || Mx < 1 || Mz < 1 || n < R + 4 * Nz) {
+    if (K < 1 || Nz < 2 || Mx < 1 || Mz < 1 || n < R + 4 * Nz || members < 1 || members > 65535) {
         cn::set_error("cn_rb2d_explicit: inconsistent sizes");
         return CN_EINVAL;
     }
@@ -156,9 +161,9 @@ int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const do
         return CN_EUNSUPPORTED;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_fc_line, dim3(Mz), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
+    hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
                        static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
-    hipLaunchKernelGGL(k_fc_mode_rows, dim3(K), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
+    hipLaunchKernelGGL(k_fc_mode_rows, dim3(K, members), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
                        static_cast<zc*>(F), K, Mz, Nz, n, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -166,6 +171,11 @@ int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const do
         return CN_EHIP;
     }
     return CN_OK;
+}
+
+int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
+                     const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream) {
+    return cn_rb2d_explicit_many(X, fidx, F, Gh, Bz, Wx, Az, K, Nz, Mx, Mz, n, R, kappa, 1, stream);
 }
 
 }  // extern "C"

@@ -118,8 +118,10 @@ class BatchHandle:
     """Owner of one ``cn_bb`` handle: a factored batch resident on the current HIP device."""
 
     def __init__(self, csrs: Sequence[tuple], border: Sequence[int], dtype, pivot: bool = True,
-                 col_border: bool = False):
+                 col_border: bool = False, mode: str | None = None):
         L = _lib.lib()
+        if mode not in (None, "lane", "wave"):
+            raise OptionError(f"BandedBatch: mode must be None, 'lane' or 'wave', got {mode!r}")
         self.dtype = np.dtype(dtype)
         B = len(csrs)
         if B == 0:
@@ -138,9 +140,13 @@ class BatchHandle:
         bord = np.ascontiguousarray(np.asarray([int(i) for i in border], dtype=np.int32))
         h = ctypes.c_void_p()
         code = _lib.CN_C128 if self.dtype == np.complex128 else _lib.CN_F64
-        st = L.cn_bb_create(ctypes.byref(h), code, B, n, ip, ix, dv, nnz,
-                            bord.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(bord.size),
-                            1 if pivot else 0, 1 if col_border else 0)
+        _lib.check(L.cn_bb_request_mode({None: 0, "lane": 1, "wave": 2}[mode]), "BandedBatch")
+        try:
+            st = L.cn_bb_create(ctypes.byref(h), code, B, n, ip, ix, dv, nnz,
+                                bord.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(bord.size),
+                                1 if pivot else 0, 1 if col_border else 0)
+        finally:
+            L.cn_bb_request_mode(0)
         self._keep = None          # the library copied what it needs
         _lib.check(st, "BandedBatch")
         self._h = h

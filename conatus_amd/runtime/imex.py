@@ -102,6 +102,19 @@ class DeviceCsr:
                 pass
 
 
+def capture_graph(fn):
+    """Capture the device work that ``fn`` enqueues into a HIP graph (torch's CUDAGraph on a side
+    stream ordered after the current one)."""
+    torch = _torch()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=s):
+        fn()
+    torch.cuda.current_stream().wait_stream(s)
+    return g
+
+
 class Imex:
     """Batched IMEX integrator on the GPU (same constructor as python/conatus/runtime/cuda.py:267-286).
 
@@ -109,7 +122,7 @@ class Imex:
     Scheme and dt are required: nothing is chosen for the user."""
 
     def __init__(self, Ms: Sequence, Ls: Sequence, border_rows: Sequence[int], scheme: str, dt: float,
-                 g=None, h=None, pivot: bool = True, dtype=None, explicit=None):
+                 g=None, h=None, pivot: bool = True, dtype=None, explicit=None, mode: str | None = None):
         torch = _torch()
         if scheme not in SCHEMES:
             raise OptionError(f"Imex: unknown scheme {scheme!r}", fix=f"one of {SCHEMES}")
@@ -126,6 +139,7 @@ class Imex:
             if A.shape != (self.n, self.n):
                 raise OptionError("Imex: all modes need the same n")
         self.scheme, self.dt = scheme, float(dt)
+        self.mode = mode              # kernel mode of the factorizations (None: automatic)
         self.border_rows = [int(i) for i in border_rows]
         self.pivot = bool(pivot)
         self._Mh = [A.astype(self.dtype) for A in Ms]
@@ -191,7 +205,7 @@ class Imex:
         if f is None:
             lhs = [csr_args((self.dtype.type(alpha) * M + self.dtype.type(beta) * L).tocsr(), self.dtype)
                    for M, L in zip(self._Mh, self._Lh)]
-            f = BatchHandle(lhs, self.border_rows, self.dtype, self.pivot)
+            f = BatchHandle(lhs, self.border_rows, self.dtype, self.pivot, mode=self.mode)
             self._factors[key] = f
         return f
 
@@ -283,6 +297,22 @@ class Imex:
         if self.order:
             self._last = (d.clone(), float(t))
 
+    def adopt_history(self, src: "Imex", rows, d, t: float) -> None:
+        """Continue the trajectories of systems ``rows`` (device index tensor) of ``src`` (same
+        scheme, dt and n) whose states are now in ``d`` at time ``t``: their SBDF history levels
+        are copied and the ring phase and step count taken over, so that the next step is the one
+        ``src`` would have made for them."""
+        if (self.scheme, self.dt, self.n) != (src.scheme, src.dt, src.n):
+            raise OptionError("Imex.adopt_history: the integrators differ in scheme, dt or n")
+        if self.order:
+            shape, sshape = (self.batch, self.n), (src.batch, src.n)
+            for (mx, f), (smx, sf) in zip(self._ring, src._ring):
+                mx.view(shape).copy_(smx.view(sshape).index_select(0, rows))
+                f.view(shape).copy_(sf.view(sshape).index_select(0, rows))
+            self._levels = list(src._levels)
+        self.iteration = src.iteration
+        self._remember(d, t)
+
     def run(self, d, t0: float, nsteps: int) -> float:
         """Advance the (batch, n) device tensor ``d`` in place by ``nsteps``; returns the time
         t0 + nsteps dt computed as the reference does (one addition per step, imex.hpp:112-116)."""
@@ -307,52 +337,63 @@ class Imex:
         torch = _torch()
         self._check_state(d)
         self._continuation(d, t0)
-        t = float(t0)
+        t, done = self._graph_startup(d, float(t0), nsteps)
+        while done < nsteps:
+            gk, fn, commit = self._graph_step(d)
+            g = self._graphs.get(gk)
+            if g is None:
+                g = capture_graph(fn)
+                self._graphs[gk] = g
+            g.replay()
+            commit()
+            t += self.dt
+            done += 1
+        self._remember(d, t)
+        return t
+
+    # -- graph steps (shared with ensemble sets that capture several integrators in one graph) --
+    def _graph_startup(self, d, t: float, nsteps: int):
+        """Eager lower-order startup steps (SBDF), and every factorization the full-order steps use
+        built before any capture; returns (t, steps done)."""
         done = 0
         if self.order:
-            while self.history_length < self.order - 1 and done < nsteps:   # lower-order startup
+            while self.history_length < self.order - 1 and done < nsteps:
                 self._sbdf_step(d)
                 t += self.dt
                 self.iteration += 1
                 done += 1
             if done < nsteps:
                 a, _ = sbdf_coefficients(self.order)
-                self._factor(a[0] / self.dt, 1.0)                            # built before capture
+                self._factor(a[0] / self.dt, 1.0)
         else:
             A, H, cc = self.tab
             for i in range(1, len(cc)):
                 self._factor(1.0, self.dt * H[i][i])
+        return t, done
+
+    def _graph_step(self, d):
+        """(key, launch, commit) of the next full-order step: the key names the ring phase, launch
+        enqueues the step's kernels, commit advances the history bookkeeping once it ran."""
         key = d.data_ptr()
-        while done < nsteps:
-            if self.order:
-                if len(self._levels) < self.order:
-                    used = set(self._levels)
-                    slot = next(i for i in range(self.order) if i not in used)
-                    levels = [slot] + self._levels
-                else:
-                    levels = [self._levels[-1]] + self._levels[:-1]
-                gk = (key, tuple(levels))
-                fn = lambda lv=levels: self._sbdf_launch(d, lv)     # noqa: E731
+        if self.order:
+            if len(self._levels) < self.order:
+                used = set(self._levels)
+                slot = next(i for i in range(self.order) if i not in used)
+                levels = [slot] + self._levels
             else:
-                gk = (key, "rk")
-                fn = lambda: self._rk_step(d)                       # noqa: E731
-            g = self._graphs.get(gk)
-            if g is None:
-                g = torch.cuda.CUDAGraph()
-                s = torch.cuda.Stream()
-                s.wait_stream(torch.cuda.current_stream())
-                with torch.cuda.graph(g, stream=s):
-                    fn()
-                torch.cuda.current_stream().wait_stream(s)
-                self._graphs[gk] = g
-            g.replay()
-            if self.order:
-                self._levels = list(gk[1])
-            t += self.dt
+                levels = [self._levels[-1]] + self._levels[:-1]
+            gk = (key, tuple(levels))
+            fn = lambda lv=levels: self._sbdf_launch(d, lv)     # noqa: E731
+        else:
+            levels = None
+            gk = (key, "rk")
+            fn = lambda: self._rk_step(d)                       # noqa: E731
+
+        def commit(lv=levels):
+            if lv is not None:
+                self._levels = list(lv)
             self.iteration += 1
-            done += 1
-        self._remember(d, t)
-        return t
+        return gk, fn, commit
 
     def explain(self) -> str:
         return (f"hip.Imex(scheme={self.scheme}, dt={self.dt:g}, batch={self.batch}, n={self.n}, "

@@ -27,11 +27,14 @@ def _torch():
 
 
 class RB2DExplicit:
-    """F(X) of 2D Rayleigh-Benard on the device (cn_rb2d_explicit)."""
+    """F(X) of 2D Rayleigh-Benard on the device (cn_rb2d_explicit_many): `members` states of one
+    discretization stored one after another (members x K x n), one launch pair for all of them.
+    F does not depend on Ra or Pr, so members with different parameters share it."""
 
-    def __init__(self, pb: RayleighBenard2D):
+    def __init__(self, pb: RayleighBenard2D, members: int = 1):
         torch = _torch()
         self.pb = pb
+        self.members = int(members)
         K, N, Mx, Mz = pb.K, pb.N, pb.Mx, pb.Mz
         dev = "cuda"
         Bz = np.stack([S.cheb_backward_matrix(N, Mz, 0, pb.Lz).T, S.cheb_backward_matrix(N, Mz, 1, pb.Lz).T])
@@ -39,8 +42,8 @@ class RB2DExplicit:
         self.Wx = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mx) / Mx)).to(dev)        # twiddles
         Az = (pb.S02 @ S.cheb_forward_matrix(N, Mz)).T
         self.Az = torch.from_numpy(np.ascontiguousarray(Az)).to(dev)                       # (Mz, N)
-        self.Gh = torch.empty(3 * K * Mz, dtype=torch.complex128, device=dev)
-        self._base = torch.from_numpy(pb.b0().ravel()).to(dev)
+        self.Gh = torch.empty(self.members * 3 * K * Mz, dtype=torch.complex128, device=dev)
+        self._base = torch.from_numpy(np.tile(pb.b0().ravel(), self.members)).to(dev)
         fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
         self.fidx = torch.from_numpy(np.ascontiguousarray(fidx)).to(dev)
 
@@ -51,23 +54,25 @@ class RB2DExplicit:
         pb = self.pb
         v = ctypes.c_void_p
         p = lambda t: v(t.data_ptr())   # noqa: E731
-        st = _lib.lib().cn_rb2d_explicit(p(x), p(self.fidx), p(out), p(self.Gh), p(self.Bz), p(self.Wx), p(self.Az),
-                                         pb.K, pb.N, pb.Mx, pb.Mz, pb.n, NBORDER, float(pb.kappa),
-                                         v(_torch().cuda.current_stream().cuda_stream))
+        st = _lib.lib().cn_rb2d_explicit_many(p(x), p(self.fidx), p(out), p(self.Gh), p(self.Bz), p(self.Wx),
+                                              p(self.Az), pb.K, pb.N, pb.Mx, pb.Mz, pb.n, NBORDER, float(pb.kappa),
+                                              self.members, v(_torch().cuda.current_stream().cuda_stream))
         _lib.check(st, "RB2DExplicit")
 
 
 class IVPSolver:
     """Device IVP solver of a Fourier x Chebyshev problem (2D Rayleigh-Benard)."""
 
-    def __init__(self, problem: RayleighBenard2D, scheme: str, dt: float, pivot: bool = True):
+    def __init__(self, problem: RayleighBenard2D, scheme: str, dt: float, pivot: bool = True,
+                 mode: str | None = None):
         torch = _torch()
         self.problem = problem
         self.explicit = RB2DExplicit(problem)
         Ms = [m[0] for m in problem.mats]
         Ls = [m[1] for m in problem.mats]
         self.imex = Imex(Ms, Ls, problem.border, scheme, dt, pivot=pivot, dtype=np.complex128,
-                         explicit=self.explicit)
+                         explicit=self.explicit, mode=mode)
+        self.pivot, self.mode = pivot, mode
         self.X = torch.zeros((problem.K, problem.n), dtype=torch.complex128, device="cuda")
         self.t = 0.0
         self.scheme, self.dt = scheme, float(dt)

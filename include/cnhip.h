@@ -123,6 +123,14 @@ int cn_bb_cond(cn_bb_t h, double* out);
 /* Release the handle and its device memory. */
 void cn_bb_destroy(cn_bb_t h);
 
+/* Kernel mode of the handles the calling thread creates from now on: 0 automatic (lane mode for
+ * many narrow-band systems, wave mode otherwise), 1 lane mode (one lane per system and
+ * right-hand side, streamed sweeps), 2 wave mode (one wavefront per system and right-hand side);
+ * a mode that does not fit the shape falls back to the other.  For the per-size-class autotuner;
+ * no counterpart in the reference (its CUDA batch picks band vs level-scheduled solves by flags,
+ * core/include/conatus/cuda.hpp:57-120). */
+int cn_bb_request_mode(int mode);
+
 /* ---- IMEX stage (core/include/conatus/imex.hpp) -------------------------------------------
  * Batched block-diagonal CSR operators on the device and the fused right-hand-side assembly of
  * M dX/dt + L X = F.  Every product and sum is rounded separately, in the reference's order, so
@@ -165,6 +173,14 @@ int cn_pointwise(int dtype, int64_t N, void* out, const void* x, const void* g, 
  * (6 K + Mx) 16 + (3 Mx + 2 Nz) 8 <= 65536 bytes of LDS (CN_EUNSUPPORTED otherwise). */
 int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
                      const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream);
+
+/* The same for `members` states of one discretization stored one after another (X and F:
+ * members x K x n, Gh: members x 3 K Mz): the explicit term of an ensemble in one launch pair
+ * (the reference evaluates an ensemble's F through BatchedRHS, python/conatus/ensembles/batched.py:116-440).
+ * 1 <= members <= 65535. */
+int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
+                          const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, int members,
+                          void* stream);
 
 #ifdef __cplusplus
 }

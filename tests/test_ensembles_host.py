@@ -1,0 +1,39 @@
+"""Host-side checks of the ensemble and serving layers (no GPU): grouping keys, footprint model,
+option errors raised before any device work."""
+import numpy as np
+import pytest
+
+from conatus_amd.problems import RayleighBenard2D
+
+
+def test_discretization_key_ignores_parameters():
+    from conatus_amd.ensembles import discretization_key
+    a = RayleighBenard2D(Nx=16, Nz=12, Lx=4.0, Ra=1e4, Pr=1.0)
+    b = RayleighBenard2D(Nx=16, Nz=12, Lx=4.0, Ra=1e6, Pr=7.0)
+    c = RayleighBenard2D(Nx=32, Nz=12, Lx=4.0, Ra=1e4, Pr=1.0)
+    assert discretization_key(a) == discretization_key(b) != discretization_key(c)
+
+
+def test_footprint_grows_with_size_and_order():
+    from conatus_amd.serving import footprint_bytes
+    small = RayleighBenard2D(Nx=64, Nz=32, Lx=4.0, Ra=1e5, Pr=1.0)
+    big = RayleighBenard2D(Nx=256, Nz=128, Lx=4.0, Ra=1e5, Pr=1.0)
+    assert footprint_bytes(big) > 8 * footprint_bytes(small)
+    assert footprint_bytes(small, 3) > footprint_bytes(small, 2)
+    # a 128 x 64 run needs well under 1 GB: thousands fit a 200 GB budget
+    assert footprint_bytes(RayleighBenard2D(Nx=128, Nz=64, Lx=4.0, Ra=1e5, Pr=1.0)) < 1e9
+
+
+def test_server_refuses_bad_requests():
+    from conatus_amd.errors import OptionError
+    from conatus_amd.serving import Request, Server
+    pb = RayleighBenard2D(Nx=16, Nz=12, Lx=4.0, Ra=1e4, Pr=1.0)
+    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave")
+    with pytest.raises(OptionError):
+        srv.submit(Request(pb, 0, pb.initial_state()))
+    with pytest.raises(OptionError):
+        srv.submit(Request(pb, 3, np.zeros((2, 2))))
+    with pytest.raises(OptionError):
+        Server("SBDF2", 1e-3, hbm_budget=10.0, autotune=False).submit(Request(pb, 3, pb.initial_state()))
+    with pytest.raises(OptionError):
+        Server(autotune=True, mode="wave")

@@ -7,8 +7,9 @@ rows (T_n(1) = 1 and T_n(-1) = (-1)^n), 4 seeded N(0,1) complex right-hand sides
 A step is one batched solve of the 4 x 4096 right-hand sides (the solve kernel, factors and data
 resident in HBM); the factorization is timed separately.  Accuracy gate before timing: every
 system's normwise backward error <= 1e-13.  About 0.3 s of untimed solves precede the warm-up
-steps so that the timed steps run at the GPU's working clock.  A second leg reports IMEX steps/s
-of 2D Rayleigh-Benard at 128 x 64 (imex_rb2d).
+steps so that the timed steps run at the GPU's working clock.  Further legs: IMEX steps/s of 2D Rayleigh-Benard at
+128 x 64 (imex_rb2d), an uneven ensemble of 256 RB 2D runs advanced together (ensemble) and a
+served stream of RB 2D requests with continuous batching, p50/p99 step latency (served).
 
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
@@ -162,6 +163,123 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     return out
 
 
+def rb2d_step_bytes(pb) -> float:
+    """Algorithmic HBM bytes of one SBDF2 step of a RB 2D run: the batched solve of its K mode
+    systems (band rows of kl + ku + 1 = 31 entries for U and 12 for L, pivots, V^T and W of r = 7
+    rows, S, the right-hand side read and x written: 16 (43 n + 2 r n + 2 n) + 4 n + 16 r^2 per
+    system), the explicit term (X read, the 3 K Mz line products written and read, F rows
+    written: 16 K (2 n + 6 Mz)) and the stage (history M X and F of two levels read, M X and the
+    right-hand side written, CSR of M with ~3 entries per row: 16 K n (4 + 2) + 12 * 3 K n)."""
+    K, n, Mz = pb.K, pb.n, pb.Mz
+    kl, ku, r = 12, 18, 7
+    solve = K * (16 * ((kl + ku + 1 + kl) * n + 2 * r * n + 2 * n) + 4 * n + 16 * r * r)
+    explicit = 16 * K * (2 * n + 6 * Mz)
+    stage = 16 * K * n * 6 + 12 * 3 * K * n
+    return float(solve + explicit + stage)
+
+
+def single_run_ms(pb, dt: float, steps: int = 40) -> float:
+    """GPU time per step of one run stepped alone (IVPSolver, graph replays, after warm-up)."""
+    import torch
+    from conatus_amd.timestep import IVPSolver
+    s = IVPSolver(pb, "SBDF2", dt)
+    X0 = pb.initial_state(seed=0)
+    s.set_state(X0)
+    s.run_graph(steps)
+    torch.cuda.synchronize()
+    quiet_heap()
+    t0 = time.perf_counter()
+    s.run_graph(steps)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def ensemble_leg(runs: int, steps: int, dt: float) -> dict:
+    """Uneven ensemble: `runs` independent RB 2D runs, Ny drawn from {32, 48, 64, 96, 128} (seeded),
+    Nx = 2 Ny, Ra log-uniform in [1e4, 1e6], Pr = 1, seeded noise initial states as the 128 x 64
+    leg, SBDF2; all runs advanced together by one EnsembleSet (one batched step per size class,
+    all classes in one captured graph, kernel mode per class by the autotuner).  Reported beside
+    per-run sequential GPU stepping: each size class's single-run step time (measured) times its
+    runs, summed."""
+    import torch
+    from conatus_amd.ensembles import EnsembleSet
+    from conatus_amd.problems import RayleighBenard2D
+    rng = np.random.default_rng(3)
+    nys = rng.choice([32, 48, 64, 96, 128], size=runs)
+    ras = 10 ** rng.uniform(4, 6, size=runs)
+    t0 = time.perf_counter()
+    pbs = [RayleighBenard2D(Nx=2 * int(ny), Nz=int(ny), Lx=4.0, Ra=float(ra), Pr=1.0) for ny, ra in zip(nys, ras)]
+    X0 = [pb.initial_state(seed=1000 + i) for i, pb in enumerate(pbs)]
+    es = EnsembleSet(pbs, "SBDF2", dt)
+    es.set_states(X0)
+    log(f"ensemble set up in {time.perf_counter() - t0:.1f} s: {es.explain()}")
+    es.run_graph(3)                        # startup, capture
+    torch.cuda.synchronize()
+    quiet_heap()
+    t0 = time.perf_counter()
+    es.run_graph(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fin = all(np.all(np.isfinite(x)) for x in es.states())
+    if not fin:
+        raise RuntimeError("ensemble state not finite after the timed steps")
+    per_class = {}
+    for ny in sorted(set(int(v) for v in nys)):
+        pb = RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0)
+        per_class[ny] = single_run_ms(pb, dt)
+    seq_ms = sum(per_class[int(ny)] for ny in nys)          # one step of every run, one run after another
+    ms = wall * 1e3 / steps
+    modes = sum(pb.K for pb in pbs)
+    nbytes = sum(rb2d_step_bytes(pb) for pb in pbs)
+    return {"workload": f"{runs} RB 2D runs, Ny in {{32,48,64,96,128}} (seeded), Nx = 2 Ny, Ra log-uniform "
+                        f"[1e4, 1e6], Pr = 1, SBDF2 dt = {dt:g}, {steps} timed steps after 3",
+            "runs": runs, "size_classes": {str(k): int((nys == k).sum()) for k in sorted(set(int(v) for v in nys))},
+            "ms_per_ensemble_step": ms, "run_steps_per_s": runs * 1e3 / ms,
+            "mode_solves_per_s": modes * 1e3 / ms,
+            "achieved_GBs": nbytes / (ms * 1e-3) / 1e9, "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_step": nbytes,
+            "sequential_single_run_ms_per_ensemble_step": seq_ms,
+            "speedup_vs_sequential_single_runs": seq_ms / ms,
+            "single_run_ms_per_step": {str(k): v for k, v in per_class.items()},
+            "tuning": es.tuning}
+
+
+def served_leg(nreq: int, dt: float) -> dict:
+    """Served stream: Poisson arrivals (seeded, mean 50 requests/s) of RB 2D runs with Ny drawn from
+    {32, 48, 64, 96, 128}, Nx = 2 Ny, Ra log-uniform in [1e4, 1e6], 50-500 steps each, admitted by
+    HBM footprint up to 200 GB and continuously batched (conatus_amd.serving).  Reports p50/p99 of
+    the per-step latency every run saw and the aggregate run-steps/s, beside per-run sequential
+    GPU stepping of the same requests (single-run step times per size class times steps)."""
+    from conatus_amd.problems import RayleighBenard2D
+    from conatus_amd.serving import Request, Server
+    rng = np.random.default_rng(4)
+    gaps = rng.exponential(1 / 50.0, size=nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    nys = rng.choice([32, 48, 64, 96, 128], size=nreq)
+    ras = 10 ** rng.uniform(4, 6, size=nreq)
+    steps = rng.integers(50, 501, size=nreq)
+    reqs = []
+    for i in range(nreq):
+        pb = RayleighBenard2D(Nx=2 * int(nys[i]), Nz=int(nys[i]), Lx=4.0, Ra=float(ras[i]), Pr=1.0)
+        reqs.append(Request(pb, int(steps[i]), pb.initial_state(seed=2000 + i), arrival=float(arrivals[i])))
+    srv = Server("SBDF2", dt, hbm_budget=200e9)
+    for ny in sorted(set(int(v) for v in nys)):          # autotune every size class before the clock starts
+        srv._mode_for(RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0))
+    quiet_heap()
+    st = srv.serve(reqs, clock="wall")
+    per_class = {ny: single_run_ms(RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0), dt)
+                 for ny in sorted(set(int(v) for v in nys))}
+    seq_s = sum(per_class[int(nys[i])] * int(steps[i]) for i in range(nreq)) * 1e-3
+    st.update({"workload": f"{nreq} requests, Poisson arrivals mean 50/s (seeded), RB 2D Ny in {{32..128}}, "
+                           f"Nx = 2 Ny, Ra log-uniform [1e4, 1e6], 50-500 SBDF2 steps each, dt = {dt:g}, "
+                           "HBM budget 200 GB",
+               "arrival_span_s": float(arrivals[-1]),
+               "sequential_single_run_steps_per_s": st["steps"] / seq_s,
+               "speedup_vs_sequential_single_runs": (st["steps"] / st["wall_s"]) / (st["steps"] / seq_s),
+               "single_run_ms_per_step": {str(k): v for k, v in per_class.items()}})
+    return st
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +289,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rb2d", action="store_true", help="skip the RB 2D IMEX steps/s leg")
     ap.add_argument("--rb2d-steps", type=int, default=200)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-ensemble leg")
+    ap.add_argument("--ensemble-runs", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--served-requests", type=int, default=120)
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain streaming kernel y = 2 x over 2^26 doubles (512 MiB read and 512 MiB "
                          "written per launch, 5 launches): the reference reading for the HBM byte counters")
@@ -302,6 +425,18 @@ def main() -> int:
         log(f"RB 2D IMEX leg ({args.rb2d_steps} graph-replayed SBDF2 steps)")
         out["imex_rb2d"] = rb2d_leg(args.rb2d_steps, args.warmup, min(args.cpu_seconds, 5.0), not args.no_cpu)
         log(f"rb2d: {out['imex_rb2d']['ms_per_step']:.4f} ms per step")
+    if not args.no_ensemble:
+        log(f"uneven ensemble leg ({args.ensemble_runs} runs)")
+        out["ensemble"] = ensemble_leg(args.ensemble_runs, args.ensemble_steps, 1e-4)
+        e = out["ensemble"]
+        log(f"ensemble: {e['ms_per_ensemble_step']:.3f} ms per step of all runs, {e['run_steps_per_s']:.4g} "
+            f"run-steps/s, {e['speedup_vs_sequential_single_runs']:.1f}x per-run sequential")
+    if not args.no_served:
+        log(f"served leg ({args.served_requests} requests)")
+        out["served"] = served_leg(args.served_requests, 1e-4)
+        v = out["served"]
+        log(f"served: p50 {v['p50_step_ms']:.3f} ms, p99 {v['p99_step_ms']:.3f} ms per step, "
+            f"{v['steps_per_s']:.4g} run-steps/s, {v['speedup_vs_sequential_single_runs']:.1f}x per-run sequential")
     print(json.dumps(out), flush=True)
     return 0
 

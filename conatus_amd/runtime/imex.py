@@ -297,20 +297,30 @@ class Imex:
         if self.order:
             self._last = (d.clone(), float(t))
 
-    def adopt_history(self, src: "Imex", rows, d, t: float) -> None:
-        """Continue the trajectories of systems ``rows`` (device index tensor) of ``src`` (same
-        scheme, dt and n) whose states are now in ``d`` at time ``t``: their SBDF history levels
-        are copied and the ring phase and step count taken over, so that the next step is the one
-        ``src`` would have made for them."""
-        if (self.scheme, self.dt, self.n) != (src.scheme, src.dt, src.n):
-            raise OptionError("Imex.adopt_history: the integrators differ in scheme, dt or n")
+    def adopt_history(self, sources: Sequence, d, t: float) -> None:
+        """Continue the trajectories of other integrators' systems: ``sources`` is a list of
+        (imex, rows) with rows a device index tensor of that integrator's systems; their states
+        are now in ``d`` one source after another, at time ``t``.  Every source has the same
+        scheme, dt and n and a full SBDF history (or none, RK); history level i (newest first)
+        of each source is copied into level i here, so that the next step of every system is the
+        one its source would have made."""
+        torch = _torch()
+        for src, _ in sources:
+            if (self.scheme, self.dt, self.n) != (src.scheme, src.dt, src.n):
+                raise OptionError("Imex.adopt_history: the integrators differ in scheme, dt or n")
+            if self.order and src.history_length != len(sources[0][0]._levels):
+                raise OptionError("Imex.adopt_history: the sources are at different startup orders")
         if self.order:
-            shape, sshape = (self.batch, self.n), (src.batch, src.n)
-            for (mx, f), (smx, sf) in zip(self._ring, src._ring):
-                mx.view(shape).copy_(smx.view(sshape).index_select(0, rows))
-                f.view(shape).copy_(sf.view(sshape).index_select(0, rows))
-            self._levels = list(src._levels)
-        self.iteration = src.iteration
+            q = sources[0][0].history_length
+            self._levels = list(range(q))
+            shape = (self.batch, self.n)
+            for i in range(q):
+                mx, f = self._ring[i]
+                mx.view(shape).copy_(torch.cat([src._ring[src._levels[i]][0].view(src.batch, src.n).index_select(0, rows)
+                                                for src, rows in sources]))
+                f.view(shape).copy_(torch.cat([src._ring[src._levels[i]][1].view(src.batch, src.n).index_select(0, rows)
+                                               for src, rows in sources]))
+        self.iteration = max(src.iteration for src, _ in sources)
         self._remember(d, t)
 
     def run(self, d, t0: float, nsteps: int) -> float:

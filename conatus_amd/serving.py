@@ -15,8 +15,12 @@ ticks; one tick is one step of every active run:
   advanced in the tick);
 * retirement: a run that reached its step count leaves its cohort with its final state copied to
   the host; the cohort is then compacted to its remaining members, whose states and SBDF history
-  levels are carried over (``Imex.adopt_history``), so that every run's arithmetic is the one of
-  its single run (tested: each result is the same alone and in the mix).
+  levels are carried over (``Imex.adopt_history``);
+* merging: cohorts of one size class that have all reached full order become one cohort (their
+  states and history levels copied, ring phases mapped), so that runs admitted at different
+  ticks share one batched step from then on.
+Compaction and merging only copy values: every run's arithmetic stays the one of its single run
+(tested: each result is the same alone and in the mix).
 
 The kernel mode of each size class is picked once by ``autotune_mode``.  There is no counterpart
 in the reference (it has no server); the per-run step is the reference's IVPSolver step
@@ -81,26 +85,49 @@ class _Cohort:
             self.ens.run_graph(1)
         self.done_steps += 1
 
+    def full_order(self) -> bool:
+        im = self.ens.imex
+        return not im.order or im.history_length == im.order
+
     def compact(self, keep: list) -> None:
         """Keep members ``keep`` (indices), carrying their states and history over."""
+        merged = _Cohort.merge([(self, keep)])
+        self.ens, self.requests = merged.ens, merged.requests
+
+    @staticmethod
+    def merge(parts: list) -> "_Cohort":
+        """One cohort of the members ``keep`` of every (cohort, keep) in ``parts`` (same size
+        class, same history length): states and SBDF history levels are copied, so that every
+        member's next step is the one its own cohort would have made."""
         torch = _torch()
-        old = self.ens
-        with torch.cuda.stream(self.stream):
-            new = Ensemble.from_problems([old.problems[b] for b in keep], old.scheme, old.dt, mode=old.mode)
-            K = old.K
-            rows = torch.cat([torch.arange(b * K, (b + 1) * K, device="cuda") for b in keep])
-            new.X.view(len(keep) * K, old.n).copy_(old.X.view(old.B * K, old.n).index_select(0, rows))
-            new.t = old.t
-            new.imex.adopt_history(old.imex, rows, new._flat(), new.t)
-        self.ens = new
-        self.requests = [self.requests[b] for b in keep]
+        c0 = parts[0][0]
+        old0 = c0.ens
+        out = _Cohort.__new__(_Cohort)
+        out.stream = c0.stream
+        out.requests = [c.requests[b] for c, keep in parts for b in keep]
+        probs = [c.ens.problems[b] for c, keep in parts for b in keep]
+        K, n = old0.K, old0.n
+        with torch.cuda.stream(out.stream):
+            for c, _ in parts:
+                out.stream.wait_stream(c.stream)
+            new = Ensemble.from_problems(probs, old0.scheme, old0.dt, mode=old0.mode)
+            sources = []
+            for c, keep in parts:
+                rows = torch.cat([torch.arange(b * K, (b + 1) * K, device="cuda") for b in keep])
+                sources.append((c.ens.imex, rows))
+            new._flat().copy_(torch.cat([c.ens._flat().index_select(0, rows) for (c, _), (_, rows) in zip(parts, sources)]))
+            new.t = old0.t
+            new.imex.adopt_history(sources, new._flat(), new.t)
+        out.ens = new
+        out.done_steps = c0.done_steps
+        return out
 
 
 class Server:
     """Continuous-batching server of 2D Rayleigh-Benard IVP requests (module doc)."""
 
     def __init__(self, scheme: str = "SBDF2", dt: float = 1e-4, hbm_budget: float = 200e9,
-                 autotune: bool = True, mode: str | None = None):
+                 autotune: bool = True, mode: str | None = None, merge: bool = True):
         if autotune and mode is not None:
             raise OptionError("Server: autotune and a fixed mode exclude each other")
         self.scheme, self.dt = scheme, float(dt)
@@ -114,6 +141,8 @@ class Server:
         self.ticks: list[float] = []
         self._order = {"SBDF1": 1, "SBDF2": 2, "SBDF3": 3, "SBDF4": 4}.get(scheme, 0)
         self._next_id = 0
+        self.merge = merge
+        self.merges = 0
 
     # ------------------------------------------------------------------ requests
     def submit(self, req: Request) -> Request:
@@ -190,9 +219,29 @@ class Server:
                 c.compact(keep)
             if keep:
                 alive.append(c)
-        self.cohorts = alive
+        self.cohorts = self._merge(alive)
         self.ticks.append(lat)
         return lat
+
+    def _merge(self, cohorts: list) -> list:
+        """Cohorts of one size class that all run at full order become one (continuous batching:
+        runs admitted at different ticks share one batched step from then on)."""
+        if not self.merge:
+            return cohorts
+        by: dict = {}
+        out = []
+        for c in cohorts:
+            if c.full_order():
+                by.setdefault(discretization_key(c.ens.problems[0]), []).append(c)
+            else:
+                out.append(c)
+        for key, cs in by.items():
+            if len(cs) == 1:
+                out.append(cs[0])
+            else:
+                out.append(_Cohort.merge([(c, list(range(len(c.requests)))) for c in cs]))
+                self.merges += 1
+        return out
 
     def serve(self, requests: list[Request], clock: str = "wall") -> dict:
         """Submit ``requests`` (their ``arrival`` times in seconds) and serve until all finished.
@@ -222,4 +271,4 @@ class Server:
                 "steps_per_s": steps / wall if wall > 0 else float("nan"),
                 "p50_step_ms": float(np.percentile(lat, 50)) if steps else float("nan"),
                 "p99_step_ms": float(np.percentile(lat, 99)) if steps else float("nan"),
-                "ticks": len(self.ticks), "modes": {f"{k[0]}x{k[1]}": v for k, v in self._modes.items()}}
+                "ticks": len(self.ticks), "merges": self.merges, "modes": {f"{k[0]}x{k[1]}": v for k, v in self._modes.items()}}

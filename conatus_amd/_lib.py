@@ -21,7 +21,8 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_bb_reserve", "cn_bb_solve", "cn_bb_solve_mapped", "cn_bb_solve_split",
            "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
            "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
-           "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode")
+           "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode",
+           "cn_bb_copy_systems")
 
 
 class Props(ctypes.Structure):
@@ -73,6 +74,7 @@ def _declare(lib):
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double, _vp]),
         "cn_bb_request_mode": (ctypes.c_int, [ctypes.c_int]),
+        "cn_bb_copy_systems": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]),
         "cn_rb2d_explicit_many": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_double, ctypes.c_int, _vp]),

@@ -854,6 +854,73 @@ int cn_bb_cond(cn_bb_t h, double* out) {
 
 void cn_bb_destroy(cn_bb_t h) { delete h; }
 
+int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, int count, void* stream) {
+    return guarded([&] {
+        if (!dst || !src) fail(CN_EINVAL, "cn_bb_copy_systems: null handle");
+        const cn_bb& S_ = *src;
+        cn_bb& D = *dst;
+        if (S_.dtype != D.dtype || S_.n != D.n || S_.r != D.r || S_.KLP != D.KLP || S_.M != D.M || S_.US != D.US ||
+            S_.WU != D.WU || S_.il != D.il || S_.mode != D.mode || S_.pivot != D.pivot || S_.device != D.device)
+            fail(CN_EINVAL, "cn_bb_copy_systems: the handles differ in type, size, band, border rank, kernel mode or device");
+        if (count < 0 || src_first < 0 || dst_first < 0 || src_first + count > S_.B || dst_first + count > D.B)
+            fail(CN_EINVAL, "cn_bb_copy_systems: system range out of bounds");
+        if (count == 0) return;
+        {   // the row permutation (border rows first) must be the same
+            std::vector<int> a(S_.n), b(D.n);
+            hip_check(hipMemcpy(a.data(), S_.perm.p, sizeof(int) * S_.n, hipMemcpyDeviceToHost), "hipMemcpy perm");
+            hip_check(hipMemcpy(b.data(), D.perm.p, sizeof(int) * D.n, hipMemcpyDeviceToHost), "hipMemcpy perm");
+            if (a != b) fail(CN_EINVAL, "cn_bb_copy_systems: the handles have different border rows");
+        }
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const size_t il = D.il, es = D.esz;
+        // interleaved arrays: entry e of system m at ((m / il) * len + e) * il + m % il
+        auto inter = [&](const DevBuf& from, DevBuf& to, size_t len, size_t esz) {
+            for (int q = 0; q < count;) {
+                const int ms = src_first + q, md = dst_first + q;
+                if (il == 1) {
+                    hip_check(hipMemcpyAsync(static_cast<char*>(to.p) + (size_t)md * len * esz,
+                                             static_cast<const char*>(from.p) + (size_t)ms * len * esz,
+                                             (size_t)count * len * esz, hipMemcpyDeviceToDevice, st),
+                              "cn_bb_copy_systems");
+                    return;
+                }
+                const char* sp = static_cast<const char*>(from.p) + (((size_t)ms / il) * len * il + ms % il) * esz;
+                char* dp = static_cast<char*>(to.p) + (((size_t)md / il) * len * il + md % il) * esz;
+                hip_check(hipMemcpy2DAsync(dp, il * esz, sp, il * esz, esz, len, hipMemcpyDeviceToDevice, st),
+                          "cn_bb_copy_systems");
+                ++q;
+            }
+        };
+        // system-major arrays: len entries per system, contiguous
+        auto major = [&](const DevBuf& from, DevBuf& to, size_t len, size_t esz) {
+            if (len == 0) return;
+            hip_check(hipMemcpyAsync(static_cast<char*>(to.p) + (size_t)dst_first * len * esz,
+                                     static_cast<const char*>(from.p) + (size_t)src_first * len * esz,
+                                     (size_t)count * len * esz, hipMemcpyDeviceToDevice, st),
+                      "cn_bb_copy_systems");
+        };
+        const size_t n = D.n, r = D.r;
+        inter(S_.Ab, D.Ab, n * D.M, es);
+        inter(S_.Uf, D.Uf, n * D.US, es);
+        inter(S_.Lf, D.Lf, n * D.KLP, es);
+        inter(S_.piv, D.piv, n, sizeof(int));
+        if (r) {
+            inter(S_.Vt, D.Vt, r * n, es);
+            inter(S_.W, D.W, r * n, es);
+        }
+        major(S_.S, D.S, r * r, es);
+        major(S_.spiv, D.spiv, r, sizeof(int));
+        major(S_.cond, D.cond, 1, sizeof(double));
+        major(S_.info, D.info, 1, sizeof(int));
+        if ((int)D.hcond.size() < D.B) D.hcond.assign(D.B, 1.0);
+        for (int q = 0; q < count; ++q)
+            D.hcond[dst_first + q] = src_first + q < (int)S_.hcond.size() ? S_.hcond[src_first + q] : 1.0;
+        D.cond_max = 1.0;
+        for (double c : D.hcond) D.cond_max = std::max(D.cond_max, c);
+        D.tr_ready = false;   // the transposed Woodbury data follow the factors (built on demand)
+    });
+}
+
 int cn_bb_request_mode(int mode) {
     if (mode < 0 || mode > 2) {
         g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane) or 2 (wave)";

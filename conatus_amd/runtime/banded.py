@@ -177,6 +177,12 @@ class BatchHandle:
                                           1 if transpose else 0, ctypes.c_void_p(stream)),
                    "BandedBatch.solve_ptr")
 
+    def copy_systems_from(self, src: "BatchHandle", src_first: int, dst_first: int, count: int, stream: int) -> None:
+        """Device copy of factored systems src_first .. of ``src`` into systems dst_first .. here
+        (cn_bb_copy_systems): a run's mode systems moved into a slot without refactorization."""
+        _lib.check(_lib.lib().cn_bb_copy_systems(self._h, int(dst_first), src._h, int(src_first), int(count),
+                                                 ctypes.c_void_p(stream)), "BandedBatch.copy_systems_from")
+
     def solve_mapped_ptr(self, rhs: int, rows: int, x: int, cols: int, stream: int) -> None:
         v = ctypes.c_void_p
         _lib.check(_lib.lib().cn_bb_solve_mapped(self._h, v(rhs), v(rows or None), v(x), v(cols or None),

@@ -286,8 +286,9 @@ class Imex:
 
     def _continuation(self, d, t0: float) -> None:
         torch = _torch()
+        # bitwise comparison (a state holding NaN still continues itself)
         if self.history_length and not (self._last is not None and self._last[1] == float(t0)
-                                         and torch.equal(self._last[0], d)):
+                                         and torch.equal(self._last[0].view(torch.uint8), d.view(torch.uint8))):
             raise OptionError(f"Imex.run: {self.scheme} holds the history of a previous run, and (x0, t0) "
                               "is not the state and time that run returned",
                               fix="call reset() to start a new trajectory (lower-order startup again), "
@@ -360,6 +361,23 @@ class Imex:
             done += 1
         self._remember(d, t)
         return t
+
+    def replay_step(self, d) -> None:
+        """One full-order step of ``d`` from the captured graphs, for an owner that keeps the
+        trajectory itself (no continuation check, no host synchronisation): the history must be
+        complete (startup done, or levels adopted from other integrators)."""
+        if self.order and self.history_length < self.order - 1:
+            raise OptionError("Imex.replay_step: the SBDF history is not complete")
+        if self.order:
+            a, _ = sbdf_coefficients(self.order)
+            self._factor(a[0] / self.dt, 1.0)
+        gk, fn, commit = self._graph_step(d)
+        g = self._graphs.get(gk)
+        if g is None:
+            g = capture_graph(fn)
+            self._graphs[gk] = g
+        g.replay()
+        commit()
 
     # -- graph steps (shared with ensemble sets that capture several integrators in one graph) --
     def _graph_startup(self, d, t: float, nsteps: int):

@@ -123,6 +123,13 @@ int cn_bb_cond(cn_bb_t h, double* out);
 /* Release the handle and its device memory. */
 void cn_bb_destroy(cn_bb_t h);
 
+/* Copy the factored data of systems src_first .. src_first+count-1 of `src` into systems
+ * dst_first .. of `dst` (asynchronous device copies on `stream`; no refactorization): both handles
+ * must have the same type, n, band, border rows, kernel mode and device.  For servers that move a
+ * run's factored mode systems into a free slot of a resident batch; no counterpart in the
+ * reference (its batches are built once, core/include/conatus/cuda.hpp:57-120). */
+int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, int count, void* stream);
+
 /* Kernel mode of the handles the calling thread creates from now on: 0 automatic (lane mode for
  * many narrow-band systems, wave mode otherwise), 1 lane mode (one lane per system and
  * right-hand side, streamed sweeps), 2 wave mode (one wavefront per system and right-hand side);

@@ -2,9 +2,9 @@
 
 Each result must be the same alone and in the mix: every request's final state is compared bit
 for bit with the same run stepped alone by IVPSolver (same scheme, dt and kernel mode), for
-requests of several sizes and step counts arriving at different ticks, so that cohorts start
-while others run, are merged with running cohorts of their size class once at full order, and are
-compacted when members finish."""
+requests of several sizes, Rayleigh numbers and step counts arriving at different ticks, so that
+runs start while others run, join the resident slot ensemble of their size class (state, history
+and factors copied in), leave it when they finish, and the slot ensembles grow when full."""
 import numpy as np
 import pytest
 
@@ -40,19 +40,19 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     st = srv.serve(reqs, clock="ticks")
     print(st, flush=True)
     assert st["requests"] == len(spec) and st["steps"] == sum(s[3] for s in spec)
-    assert st["merges"] >= 2                                  # later arrivals joined running cohorts
+    assert st["joins"] == len(spec)                           # every run joined its class after startup
     for q in reqs:
         assert len(q.step_latencies) == q.steps
         ref = _alone(q, 1e-3, srv._modes[discretization_key(q.problem)])
         assert np.array_equal(q.result, ref), (q.rid, np.max(abs(q.result - ref)))
 
 
-def test_without_merging_results_are_the_same():
+def test_slot_growth_keeps_results():
     from conatus_amd.serving import Server
-    reqs = _requests([(16, 12, 2e4, 6, 0), (16, 12, 5e4, 5, 2), (16, 12, 9e4, 4, 3)])
-    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave", merge=False)
+    reqs = _requests([(16, 12, 2e4, 9, 0), (16, 12, 5e4, 7, 1), (16, 12, 9e4, 6, 2), (16, 12, 7e4, 5, 2)])
+    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave", capacity=1)
     st = srv.serve(reqs, clock="ticks")
-    assert st["merges"] == 0
+    assert st["slots"]["16x12"] == 4                          # grew 1 -> 2 -> 4 with runs in flight
     for q in reqs:
         assert np.array_equal(q.result, _alone(q, 1e-3, "wave"))
 

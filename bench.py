@@ -262,9 +262,10 @@ def served_leg(nreq: int, dt: float) -> dict:
     for i in range(nreq):
         pb = RayleighBenard2D(Nx=2 * int(nys[i]), Nz=int(nys[i]), Lx=4.0, Ra=1e5, Pr=1.0)
         reqs.append(Request(pb, int(steps[i]), pb.initial_state(seed=2000 + i), arrival=float(arrivals[i])))
-    srv = Server("SBDF2", dt, hbm_budget=200e9, capacity=16)
-    # every size class tuned and its resident slot ensemble built before the clock starts
-    srv.prepare([RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0) for ny in sorted(set(int(v) for v in nys))])
+    srv = Server("SBDF2", dt, hbm_budget=200e9, page_size=8)
+    # every size class tuned, its pages and the single-run factors built before the clock starts
+    srv.prepare([RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0) for ny in sorted(set(int(v) for v in nys))],
+                pages=3)
     quiet_heap()
     st = srv.serve(reqs, clock="wall")
     bad = sum(not np.all(np.isfinite(q.result)) for q in reqs)

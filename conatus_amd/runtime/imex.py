@@ -102,6 +102,15 @@ class DeviceCsr:
                 pass
 
 
+def factor_operator(Ms, Ls, alpha: float, beta: float, border_rows, dtype, pivot: bool = True,
+                    mode: str | None = None) -> BatchHandle:
+    """The factored batch of alpha M_k + beta L_k (the implicit operator of a stage), built the
+    same way wherever it is needed, so that equal operators give bit-identical factors."""
+    dtype = np.dtype(dtype)
+    lhs = [csr_args((dtype.type(alpha) * M + dtype.type(beta) * L).tocsr(), dtype) for M, L in zip(Ms, Ls)]
+    return BatchHandle(lhs, border_rows, dtype, pivot, mode=mode)
+
+
 def capture_graph(fn):
     """Capture the device work that ``fn`` enqueues into a HIP graph (torch's CUDAGraph on a side
     stream ordered after the current one)."""
@@ -203,9 +212,7 @@ class Imex:
         key = (alpha, beta)
         f = self._factors.get(key)
         if f is None:
-            lhs = [csr_args((self.dtype.type(alpha) * M + self.dtype.type(beta) * L).tocsr(), self.dtype)
-                   for M, L in zip(self._Mh, self._Lh)]
-            f = BatchHandle(lhs, self.border_rows, self.dtype, self.pivot, mode=self.mode)
+            f = factor_operator(self._Mh, self._Lh, alpha, beta, self.border_rows, self.dtype, self.pivot, self.mode)
             self._factors[key] = f
         return f
 

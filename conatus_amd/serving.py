@@ -4,27 +4,29 @@ Requests (a problem, an initial state, a number of steps) arrive at any time.  T
 every admitted run's state, SBDF history and factored mode systems resident in HBM and advances
 all of them in ticks; one tick is one step of every active run.
 
-* Size classes: every discretization (grid) has one resident slot ensemble, an ``Ensemble`` of
-  ``capacity`` slots whose full-order steps are replayed from its captured graphs (one batched
-  explicit term, stage and solve for all slots).  The kernel mode of a class is picked once by
-  ``autotune_mode``.
+* Pages: every size class (grid) holds pages, resident ``Ensemble``s of ``page_size`` slots built
+  once (explicit-term tables, CSR, factors, captured graphs).  A stepping page runs at full order
+  and is replayed from its graphs (one batched explicit term, stage and solve for its slots); a
+  startup page makes the lower-order startup steps of the runs placed in it.  Only pages that
+  hold runs are stepped, each on its own stream, so that the pages' kernels overlap.  The kernel
+  mode of a class is picked once by ``autotune_mode``.
 * Admission, at the start of a tick: waiting requests are admitted in arrival order while the
-  estimated HBM footprint of the admitted runs stays within the budget.  The runs admitted in one
-  tick make their lower-order startup steps together in a startup cohort (one Ensemble per class)
-  on its own stream, beside the slot ensembles.
-* Joining: a run whose startup is complete moves into a free slot of its class by device copies
-  only: its state, its SBDF history levels (into the ring slots the class's next step reads) and
-  its factored full-order mode systems (``cn_bb_copy_systems``; skipped when the slot already
-  holds the factors of the same Ra and Pr).  From then on it shares the class's batched step.
-  A full class grows to twice the slots (states, history and factors copied over).
+  estimated HBM footprint of the admitted runs stays within the budget, into free slots of a
+  startup page of their class (state copied in).
+* Factors: a slot holds the factored operators of its run (one per SBDF order it steps at),
+  copied from the run's single-run handles (``cn_bb_copy_systems``), which are built once per
+  (grid, Ra, Pr, order) and cached; a slot that already holds the same Ra, Pr copies nothing.
+* Joining: a run whose startup is complete moves into a free slot of a stepping page of its class
+  (the fullest first) by device copies only: its state and its SBDF history levels (into the
+  ring slots the page's next step reads).  From then on it shares the page's batched step.
 * Retirement: a run that reached its step count leaves with its state copied to the host; its
   slot is free for the next run (a free slot keeps being stepped, its values are never read).
 
 Moving a run only copies values: every run's arithmetic stays the one of its single run (tested:
 each result is the same alone and in the mix).  The step latency of a run is the wall time of
-the tick it advanced in (admission, joins, every class's step and every startup step, until the
-device finished).  There is no counterpart in the reference (it has no server); the per-run step
-is the reference's IVPSolver step (python/conatus/timestep/integrator.py:648-682).
+the tick it advanced in (joins, admission, every page's step, until the device finished).  There
+is no counterpart in the reference (it has no server); the per-run step is the reference's
+IVPSolver step (python/conatus/timestep/integrator.py:648-682).
 """
 from __future__ import annotations
 
@@ -36,7 +38,7 @@ import numpy as np
 from .ensembles import Ensemble, autotune_mode, discretization_key
 from .errors import OptionError
 from .problems import RayleighBenard2D
-from .runtime.imex import sbdf_coefficients
+from .runtime.imex import factor_operator, sbdf_coefficients
 
 __all__ = ["Request", "Server", "footprint_bytes"]
 
@@ -75,103 +77,77 @@ def _params(pb: RayleighBenard2D) -> tuple:
     return (float(pb.Ra), float(pb.Pr))
 
 
-def _full_order_factor(imex):
-    """The factored full-order operator a0/dt M + L of an SBDF integrator."""
-    return imex._factor(sbdf_coefficients(imex.order)[0][0] / imex.dt, 1.0)
+def _alpha(order: int, dt: float) -> float:
+    """The implicit coefficient a0/dt of an SBDF step of the given order."""
+    return sbdf_coefficients(order)[0][0] / dt
 
 
-class _SlotClass:
-    """The resident slot ensemble of one size class."""
+class _Page:
+    """A resident ensemble of ``P`` slots of one size class (the template's problem in every slot).
+    A stepping page runs at full order from the start (runs join with their own history) and is
+    replayed from its captured graphs; a startup page makes the lower-order startup steps of the
+    runs placed in it, eagerly.  ``facs`` maps an SBDF order to the page's factored operator of
+    that order; ``slot_params[q][s]`` is the (Ra, Pr) whose factors slot s holds for order q."""
 
-    def __init__(self, template: RayleighBenard2D, capacity: int, scheme: str, dt: float, mode: str | None):
+    def __init__(self, template: RayleighBenard2D, P: int, scheme: str, dt: float, mode: str | None,
+                 startup: bool):
         torch = _torch()
-        self.template, self.scheme, self.dt, self.mode = template, scheme, dt, mode
+        self.P, self.startup, self.dt = P, startup, dt
         self.stream = torch.cuda.Stream()
         self.members: dict = {}
-        self.slot_params: list = []
-        self.capacity = 0
-        self.ens = None
-        self.fac = None
-        self._grow(capacity)
-
-    def _grow(self, capacity: int) -> None:
-        """A slot ensemble of ``capacity`` slots (the template's problem in every slot, full order
-        from the start: runs join with their own history); the current slots' states, history
-        levels and factors are carried over into its first slots."""
-        torch = _torch()
-        old_ens, old_fac, oc = self.ens, self.fac, self.capacity
-        ens = Ensemble.from_problems([self.template] * capacity, self.scheme, self.dt, mode=self.mode)
-        im = ens.imex
-        K, n = ens.K, ens.n
+        self.ens = Ensemble.from_problems([template] * P, scheme, dt, mode=mode)
+        im = self.ens.imex
         with torch.cuda.stream(self.stream):
-            fac = _full_order_factor(im)
-            im._levels = list(range(im.order))
-            if old_ens is not None:
-                oim = old_ens.imex
-                ens._flat()[: oc * K].copy_(old_ens._flat())
-                for i, s_old in enumerate(oim._levels):
-                    for j in range(2):
-                        im._ring[i][j].view(im.batch, n)[: oc * K].copy_(oim._ring[s_old][j].view(oim.batch, n))
-                fac.copy_systems_from(old_fac, 0, 0, oc * K, self.stream.cuda_stream)
-            ens.t = 0.0
-        self.slot_params = self.slot_params + [_params(self.template)] * (capacity - oc)
-        self.ens, self.fac, self.capacity = ens, fac, capacity
+            orders = range(1, im.order) if startup else [im.order]
+            self.facs = {q: im._factor(_alpha(q, dt), 1.0) for q in orders}
+            if not startup:
+                im._levels = list(range(im.order))
+                for _ in range(im.order):            # capture every ring phase's graph now
+                    im.replay_step(self.ens._flat())
+        self.slot_params = {q: [_params(template)] * P for q in self.facs}
 
-    def free_slot(self) -> int:
-        for s in range(self.capacity):
-            if s not in self.members:
-                return s
-        self._grow(2 * self.capacity)
-        return self.free_slot()
+    def free_slots(self) -> list:
+        return [s for s in range(self.P) if s not in self.members]
 
-    def join(self, cohort: Ensemble, b: int, req: Request, cohort_fac) -> None:
-        """Move member b of a startup cohort (history complete) into a free slot."""
-        torch = _torch()
-        s = self.free_slot()
-        ens, im = self.ens, self.ens.imex
-        K, n = ens.K, ens.n
-        cim = cohort.imex
-        with torch.cuda.stream(self.stream):
-            self.stream.wait_stream(torch.cuda.current_stream())
-            ens._flat()[s * K:(s + 1) * K].copy_(cohort._flat()[b * K:(b + 1) * K])
-            # history level i of the run goes to the ring slot that is level i of the class now:
-            # the class's next step reads it as level i + 1
-            for i, cs in enumerate(cim._levels):
-                for j in range(2):
-                    im._ring[im._levels[i]][j].view(im.batch, n)[s * K:(s + 1) * K].copy_(
-                        cim._ring[cs][j].view(cim.batch, n)[b * K:(b + 1) * K])
-            if self.slot_params[s] != _params(req.problem):
-                self.fac.copy_systems_from(cohort_fac, b * K, s * K, K, self.stream.cuda_stream)
-                self.slot_params[s] = _params(req.problem)
-        self.members[s] = req
+    def set_factors(self, s: int, req: Request, runfac) -> None:
+        """Slot s gets the factors of req's operators (copied from its single-run handles)."""
+        K = self.ens.K
+        for q, fac in self.facs.items():
+            if self.slot_params[q][s] != _params(req.problem):
+                fac.copy_systems_from(runfac(req.problem, q), 0, s * K, K, self.stream.cuda_stream)
+                self.slot_params[q][s] = _params(req.problem)
 
     def step(self) -> None:
         torch = _torch()
         with torch.cuda.stream(self.stream):
-            self.ens.imex.replay_step(self.ens._flat())
-        self.ens.t += self.ens.dt
+            if self.startup:
+                self.ens.run(1)
+            else:
+                self.ens.imex.replay_step(self.ens._flat())
+                self.ens.t += self.ens.dt
 
 
 class Server:
     """Continuous-batching server of 2D Rayleigh-Benard IVP requests (module doc)."""
 
     def __init__(self, scheme: str = "SBDF2", dt: float = 1e-4, hbm_budget: float = 200e9,
-                 autotune: bool = True, mode: str | None = None, capacity: int = 8):
+                 autotune: bool = True, mode: str | None = None, page_size: int = 8):
         if autotune and mode is not None:
             raise OptionError("Server: autotune and a fixed mode exclude each other")
         if scheme not in ("SBDF1", "SBDF2", "SBDF3", "SBDF4"):
             raise OptionError(f"Server: scheme {scheme!r} not served", fix="one of SBDF1..SBDF4")
-        if int(capacity) != capacity or capacity < 1:
-            raise OptionError("Server: capacity must be a whole number >= 1")
+        if int(page_size) != page_size or page_size < 1:
+            raise OptionError("Server: page_size must be a whole number >= 1")
         self.scheme, self.dt = scheme, float(dt)
         self.budget = float(hbm_budget)
         self.autotune, self.mode = autotune, mode
-        self.capacity = int(capacity)
+        self.page_size = int(page_size)
         self._order = int(scheme[-1])
         self._modes: dict = {}
-        self.classes: dict = {}
+        self.pages: dict = {}             # (grid, startup) -> [_Page]
+        self._runfacs: dict = {}
+        self.pages_built = 0
         self.waiting: list[Request] = []
-        self.startups: list = []          # (Ensemble, [requests], stream)
         self.finished: list[Request] = []
         self.resident = 0
         self.ticks: list[float] = []
@@ -200,23 +176,44 @@ class Server:
             self._modes[key] = autotune_mode([pb], self.scheme, self.dt)[0]
         return self._modes[key]
 
-    def _class(self, pb: RayleighBenard2D) -> _SlotClass:
-        key = discretization_key(pb)
-        c = self.classes.get(key)
-        if c is None:
-            c = _SlotClass(pb, self.capacity, self.scheme, self.dt, self._mode_for(pb))
-            self.classes[key] = c
-        return c
+    def _runfac(self, pb: RayleighBenard2D, q: int):
+        """Factored operator of one run's K mode systems for SBDF order q, cached per (grid, Ra,
+        Pr, q): the source of the factors copied into slots."""
+        key = (discretization_key(pb), _params(pb), q)
+        f = self._runfacs.get(key)
+        if f is None:
+            Ms = [M.astype(np.complex128) for M, L in pb.mats]
+            Ls = [L.astype(np.complex128) for M, L in pb.mats]
+            f = factor_operator(Ms, Ls, _alpha(q, self.dt), 1.0, pb.border, np.complex128, True,
+                                self._mode_for(pb))
+            self._runfacs[key] = f
+        return f
 
-    def prepare(self, problems) -> None:
-        """Tune and build the size classes of ``problems`` ahead of the requests."""
+    def _pages(self, pb: RayleighBenard2D, startup: bool) -> list:
+        return self.pages.setdefault((discretization_key(pb), startup), [])
+
+    def _new_page(self, pb: RayleighBenard2D, startup: bool) -> _Page:
+        pg = _Page(pb, self.page_size, self.scheme, self.dt, self._mode_for(pb), startup)
+        self._pages(pb, startup).append(pg)
+        self.pages_built += 1
+        return pg
+
+    def prepare(self, problems, pages: int = 2) -> None:
+        """Tune the size classes of ``problems`` and build their pages (``pages`` stepping pages
+        and one startup page each) and the single-run factors of their parameters, ahead of the
+        requests."""
         for pb in problems:
-            self._class(pb)
+            while len(self._pages(pb, False)) < pages:
+                self._new_page(pb, False)
+            if self._order > 1 and not self._pages(pb, True):
+                self._new_page(pb, True)
+            for q in range(1, self._order + 1):
+                self._runfac(pb, q)
 
     # ------------------------------------------------------------------ scheduling
     def _admit(self, now: float) -> None:
+        """Admit waiting requests (arrival order, HBM footprint) into free startup slots."""
         torch = _torch()
-        take: list[Request] = []
         rest: list[Request] = []
         blocked = False
         for q in self.waiting:
@@ -224,75 +221,91 @@ class Server:
                 rest.append(q)
                 continue
             fp = footprint_bytes(q.problem, self._order)
-            if not blocked and self.resident + fp <= self.budget:
-                self.resident += fp
-                q.admitted = now
-                take.append(q)
-            else:
+            if blocked or self.resident + fp > self.budget:
                 blocked = True          # arrival order: nothing overtakes a request that waits
                 rest.append(q)
+                continue
+            self.resident += fp
+            q.admitted = now
+            if self._order == 1:
+                self._join(q, None, 0)
+                continue
+            # a startup page that has not started stepping (no member has stepped yet)
+            pg = next((p for p in self._pages(q.problem, True)
+                       if p.free_slots() and not any(m.step_latencies for m in p.members.values())), None)
+            if pg is None:
+                pg = next((p for p in self._pages(q.problem, True) if not p.members), None) \
+                    or self._new_page(q.problem, True)
+            s = pg.free_slots()[0]
+            if not pg.members:
+                pg.ens.imex.reset()
+                pg.ens.t = 0.0
+            K = pg.ens.K
+            with torch.cuda.stream(pg.stream):
+                pg.ens._flat()[s * K:(s + 1) * K].copy_(torch.from_numpy(np.ascontiguousarray(q.X0)).to("cuda", non_blocking=False))
+                pg.set_factors(s, q, self._runfac)
+            pg.members[s] = q
         self.waiting = rest
-        by: dict = {}
-        for q in take:
-            by.setdefault(discretization_key(q.problem), []).append(q)
-        for reqs in by.values():
-            cls = self._class(reqs[0].problem)
-            ens = Ensemble.from_problems([q.problem for q in reqs], self.scheme, self.dt, mode=cls.mode)
-            ens.set_states(np.stack([q.X0 for q in reqs]))
-            self.startups.append((ens, reqs, torch.cuda.Stream()))
+
+    def _join(self, q: Request, src, sslot: int) -> None:
+        """Move run q (state and history in slot sslot of startup page src, or its initial state
+        when the scheme has no startup) into a free slot of a stepping page of its class."""
+        torch = _torch()
+        pages = self._pages(q.problem, False)
+        pg = max((p for p in pages if p.free_slots()), key=lambda p: len(p.members), default=None) \
+            or self._new_page(q.problem, False)
+        s = pg.free_slots()[0]
+        im, K, n = pg.ens.imex, pg.ens.K, pg.ens.n
+        with torch.cuda.stream(pg.stream):
+            if src is not None:
+                pg.stream.wait_stream(src.stream)
+                pg.ens._flat()[s * K:(s + 1) * K].copy_(src.ens._flat()[sslot * K:(sslot + 1) * K])
+                cim = src.ens.imex
+                # history level i of the run goes to the ring slot that is level i of the page
+                # now: the page's next step reads it as level i + 1
+                for i, cs in enumerate(cim._levels):
+                    for j in range(2):
+                        im._ring[im._levels[i]][j].view(im.batch, n)[s * K:(s + 1) * K].copy_(
+                            cim._ring[cs][j].view(cim.batch, n)[sslot * K:(sslot + 1) * K])
+            else:
+                pg.ens._flat()[s * K:(s + 1) * K].copy_(torch.from_numpy(np.ascontiguousarray(q.X0)).to("cuda"))
+            pg.set_factors(s, q, self._runfac)
+        pg.members[s] = q
+        self.joins += 1
 
     def _join_ready(self) -> None:
-        """Startup cohorts whose history is complete move their unfinished runs into slots."""
-        keep = []
-        for ens, reqs, st in self.startups:
-            im = ens.imex
-            if im.history_length < im.order - 1:
-                keep.append((ens, reqs, st))
+        """Startup pages whose history is complete move their unfinished runs into stepping pages."""
+        for (key, startup), pages in list(self.pages.items()):
+            if not startup:
                 continue
-            cls = self._class(reqs[0].problem)
-            fac = _full_order_factor(im)
-            for b, q in enumerate(reqs):
-                if q.result is None:
-                    cls.join(ens, b, q, fac)
-                    self.joins += 1
-            _torch().cuda.current_stream().wait_stream(cls.stream)
-        self.startups = keep
+            for pg in pages:
+                im = pg.ens.imex
+                if pg.members and im.history_length >= im.order - 1:
+                    for s, q in sorted(pg.members.items()):
+                        self._join(q, pg, s)
+                    pg.members = {}
 
     def tick(self, now: float) -> float:
-        """Admit, join, step every class and every startup cohort once, retire; returns the
-        tick's wall time (s)."""
+        """Admit, join, step every page that holds runs, retire; returns the tick's wall time (s)."""
         torch = _torch()
         t0 = time.perf_counter()
-        self._admit(now)
         self._join_ready()
-        active = [c for c in self.classes.values() if c.members]
-        for c in active:
-            c.step()
-        for ens, reqs, st in self.startups:
-            with torch.cuda.stream(st):
-                ens.run(1)
+        self._admit(now)
+        active = [pg for pages in self.pages.values() for pg in pages if pg.members]
+        for pg in active:
+            pg.step()
         torch.cuda.synchronize()
         lat = time.perf_counter() - t0
-        for c in active:
+        for pg in active:
             done = []
-            for s, q in c.members.items():
+            for s, q in pg.members.items():
                 q.step_latencies.append(lat)
                 if len(q.step_latencies) >= q.steps:
                     done.append(s)
             if done:
-                X = c.ens.states()
+                X = pg.ens.states()
                 for s in done:
-                    self._retire(c.members.pop(s), X[s], now + lat)
-        for ens, reqs, st in self.startups:
-            live = [(b, q) for b, q in enumerate(reqs) if q.result is None]
-            for b, q in live:
-                q.step_latencies.append(lat)
-            if any(len(q.step_latencies) >= q.steps for b, q in live):
-                X = ens.states()
-                for b, q in live:
-                    if len(q.step_latencies) >= q.steps:
-                        self._retire(q, X[b], now + lat)
-        self.startups = [s for s in self.startups if any(q.result is None for q in s[1])]
+                    self._retire(pg.members.pop(s), X[s], now + lat)
         self.ticks.append(lat)
         return lat
 
@@ -303,7 +316,7 @@ class Server:
         self.resident -= footprint_bytes(q.problem, self._order)
 
     def _busy(self) -> bool:
-        return bool(self.startups) or any(c.members for c in self.classes.values())
+        return any(pg.members for pages in self.pages.values() for pg in pages)
 
     def serve(self, requests: list[Request], clock: str = "wall") -> dict:
         """Submit ``requests`` (their ``arrival`` times) and serve until all finished.
@@ -334,6 +347,6 @@ class Server:
                 "steps_per_s": steps / wall if wall > 0 else float("nan"),
                 "p50_step_ms": float(np.percentile(lat, 50)) if steps else float("nan"),
                 "p99_step_ms": float(np.percentile(lat, 99)) if steps else float("nan"),
-                "ticks": len(self.ticks), "joins": self.joins,
-                "slots": {f"{k[0]}x{k[1]}": c.capacity for k, c in self.classes.items()},
+                "ticks": len(self.ticks), "joins": self.joins, "pages_built": self.pages_built,
+                "pages": {f"{k[0][0]}x{k[0][1]}{' startup' if k[1] else ''}": len(v) for k, v in self.pages.items()},
                 "modes": {f"{k[0]}x{k[1]}": v for k, v in self._modes.items()}}

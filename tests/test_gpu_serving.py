@@ -47,14 +47,27 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
         assert np.array_equal(q.result, ref), (q.rid, np.max(abs(q.result - ref)))
 
 
-def test_slot_growth_keeps_results():
+def test_pages_built_on_demand_keep_results():
     from conatus_amd.serving import Server
     reqs = _requests([(16, 12, 2e4, 9, 0), (16, 12, 5e4, 7, 1), (16, 12, 9e4, 6, 2), (16, 12, 7e4, 5, 2)])
-    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave", capacity=1)
+    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave", page_size=2)
     st = srv.serve(reqs, clock="ticks")
-    assert st["slots"]["16x12"] == 4                          # grew 1 -> 2 -> 4 with runs in flight
+    assert st["pages"]["16x12"] == 2                          # two stepping pages of two slots
     for q in reqs:
         assert np.array_equal(q.result, _alone(q, 1e-3, "wave"))
+
+
+def test_sbdf3_startup_pages():
+    from conatus_amd.serving import Server
+    from conatus_amd.timestep import IVPSolver
+    reqs = _requests([(16, 12, 2e4, 8, 0), (16, 12, 5e4, 2, 0), (16, 12, 9e4, 6, 3)])
+    srv = Server("SBDF3", 1e-3, autotune=False, mode="wave", page_size=4)
+    srv.serve(reqs, clock="ticks")
+    for q in reqs:
+        s = IVPSolver(q.problem, "SBDF3", 1e-3, mode="wave")
+        s.set_state(q.X0)
+        s.run(q.steps)
+        assert np.array_equal(q.result, s.state())
 
 
 def test_admission_by_footprint_serializes_when_the_budget_is_small():

@@ -76,13 +76,14 @@ class BandedBatch:
     """
 
     def __init__(self, mats: Sequence, border_rows: Sequence[int], pivot: bool = True, dtype=None,
-                 level_schedule: bool = True, level_merge: bool = True, col_border: bool = False):
+                 level_schedule: bool = True, level_merge: bool = True, col_border: bool = False,
+                 mode: str | None = None):
         _need()
         mats = [sp.csr_matrix(A) for A in mats]
         self.dtype = _factor_dtype(mats, dtype)
         self.border_rows = [int(i) for i in border_rows]
         self._h = BatchHandle([csr_args(A, self.dtype) for A in mats], self.border_rows, self.dtype,
-                              bool(pivot), bool(col_border))
+                              bool(pivot), bool(col_border), mode=mode)
         p = self._h.props()
         self.n, self.kl, self.ku, self.rank, self.batch = p.n, p.kl, p.ku, p.rank, p.batch
         self.pivot = bool(p.pivot)

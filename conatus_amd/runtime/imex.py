@@ -192,6 +192,10 @@ class Imex:
             self._MX0 = z()
         self._graphs = {}
         self.reset()
+        # the continuation check compares states byte for byte (as imex.cu:145-151 does); run it
+        # once here so that its device code is loaded before any timed step
+        zz = torch.zeros((self.batch, self.n), dtype=self._tdt, device="cuda")
+        torch.equal(zz.view(torch.uint8), zz.clone().view(torch.uint8))
 
     # -- state ---------------------------------------------------------------------------------
     def reset(self) -> None:
@@ -349,14 +353,29 @@ class Imex:
         self._remember(d, t)
         return t
 
-    def run_graph(self, d, t0: float, nsteps: int) -> float:
-        """As ``run``, with every full-order step replayed from a captured HIP graph (one graph per
-        ring phase of the SBDF history; one graph for RK).  Startup steps run eagerly."""
-        torch = _torch()
+    def run_graph(self, d, t0: float, nsteps: int, chunk: int = 8) -> float:
+        """As ``run``, with every full-order step replayed from captured HIP graphs.  Whole chunks
+        of steps (``chunk`` rounded up to a multiple of the SBDF ring period) are one graph each,
+        so that the host launches one graph per chunk; the remaining steps use one graph per ring
+        phase (one graph for RK).  Startup steps run eagerly."""
         self._check_state(d)
         self._continuation(d, t0)
         t, done = self._graph_startup(d, float(t0), nsteps)
+        period = max(self.order, 1)
+        chunk = period * max(1, -(-int(chunk) // period))
         while done < nsteps:
+            if chunk > 1 and nsteps - done >= chunk and (not self.order or self.history_length == self.order):
+                gk = (d.data_ptr(), tuple(self._levels) if self.order else "rk", "chunk", chunk)
+                g = self._graphs.get(gk)
+                if g is None:
+                    g = self._capture_chunk(d, chunk)
+                    self._graphs[gk] = g
+                g.replay()
+                for _ in range(chunk):
+                    self._graph_step(d)[2]()
+                    t += self.dt
+                done += chunk
+                continue
             gk, fn, commit = self._graph_step(d)
             g = self._graphs.get(gk)
             if g is None:
@@ -368,6 +387,18 @@ class Imex:
             done += 1
         self._remember(d, t)
         return t
+
+    def _capture_chunk(self, d, chunk: int):
+        """One graph of ``chunk`` consecutive full-order steps from the current ring phase (the
+        bookkeeping is restored: the caller commits the steps after each replay)."""
+        saved = (list(self._levels), self.iteration)
+        fns = []
+        for _ in range(chunk):
+            _, fn, commit = self._graph_step(d)
+            fns.append(fn)
+            commit()
+        self._levels, self.iteration = saved
+        return capture_graph(lambda: [f() for f in fns])
 
     def replay_step(self, d) -> None:
         """One full-order step of ``d`` from the captured graphs, for an owner that keeps the

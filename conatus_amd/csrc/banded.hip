@@ -317,26 +317,34 @@ size_t wave_lds_bytes(const cn_bb& h) {
     return 2 * (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h);
 }
 
-// Block length of the streamed sweeps: 16 steps when the two buffers of buf16 bytes and the two
-// stages fit in LDS, else 8 (0: none fits); CNHIP_SPX=8 forces the short blocks (tests of both
-// variants).
+// Block length of the streamed sweeps: 16 steps when two buffers of buf16 bytes (the largest sweep
+// buffer) and the two stages fit in LDS, else 8 (0: none fits); CNHIP_SPX=8 forces the short
+// blocks (tests of both variants).
 int pick_spx(int buf16, int buf8, int esz) {
     const char* e = std::getenv("CNHIP_SPX");
     const bool force8 = e && e[0] == '8' && e[1] == 0;
     if (!force8 && 2 * buf16 + 2 * stage_bytes(16, esz) <= LDS_CAP) return 16;
     return 2 * buf8 + 2 * stage_bytes(8, esz) <= LDS_CAP ? 8 : 0;
 }
-// LDS bytes of a streamed launch: two buffers and two stages
-size_t streamed_lds(int bufbytes, int spx, int esz) { return 2 * (size_t)bufbytes + 2 * (size_t)stage_bytes(spx, esz); }
+// Buffer region of a streamed launch: every sweep gets three buffers when they fit beside the
+// stages, else two (the kernels derive the same counts from the region, sweep_nbuf)
+int buffer_region(std::initializer_list<int> sweeps, int spx, int esz) {
+    const int room = LDS_CAP - 2 * stage_bytes(spx, esz);
+    int region = 0;
+    for (int x : sweeps) region = std::max(region, (3 * x <= room ? 3 : 2) * x);
+    return region;
+}
+// LDS bytes of a streamed launch: the buffer region and two stages
+size_t streamed_lds(int region, int spx, int esz) { return (size_t)region + 2 * (size_t)stage_bytes(spx, esz); }
 
-// LDS bytes of one buffer of the streamed border completion (block length in spx), or -1
+// LDS buffer region of the streamed border completion (block length in spx), or -1
 int border_bufbytes(const cn_bb& h, int& spx) {
     spx = 0;
     if (!h.streamed || h.WU > 16) return -1;
     const int e = (int)h.esz;
     const int rb = rb_for(h.r);
     spx = pick_spx(bwd_bytes(16, h.US, rb, e), bwd_bytes(8, h.US, rb, e), e);
-    return spx ? bwd_bytes(spx, h.US, rb, e) : -1;
+    return spx ? buffer_region({bwd_bytes(spx, h.US, rb, e)}, spx, e) : -1;
 }
 
 int nr_for(int r) { return r >= 4 ? 4 : (r >= 2 ? 2 : 1); }
@@ -616,7 +624,7 @@ void solve_t_impl(cn_bb& h, void* x, int nrhs, int64_t rs, hipStream_t s) {
     hip_check(hipGetLastError(), "k_solve_t launch");
 }
 
-// LDS bytes of one buffer of the streamed solve (k_solve_s), or -1 if it does not apply
+// LDS buffer region of the streamed solve (k_solve_s), or -1 if it does not apply
 int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int& spx) {
     spx = 0;
     if (!h.streamed) return -1;
@@ -626,7 +634,8 @@ int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int& spx) {
     const int rb = rb_for(h.r);
     auto buf = [&](int x) { return std::max({fwd_bytes(x, h.KLP, e), bwd_bytes(x, h.US, rb, e), cor_bytes(x, rb, e)}); };
     spx = pick_spx(buf(16), buf(8), e);
-    return spx ? buf(spx) : -1;
+    return spx ? buffer_region({fwd_bytes(spx, h.KLP, e), bwd_bytes(spx, h.US, rb, e), cor_bytes(spx, rb, e)}, spx, e)
+               : -1;
 }
 
 void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {

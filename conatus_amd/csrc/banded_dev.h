@@ -473,6 +473,33 @@ __device__ __forceinline__ void dma16(const void* src_, char* lds_dst, int nbyte
 
 __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// vector memory instructions dma16 issues for nbytes (one per started KiB)
+__device__ __forceinline__ int dma16_count(int nbytes) { return (nbytes + 1023) >> 10; }
+
+// workgroup barrier of the two-wave sweeps: the wave's LDS operations are complete, its vector
+// memory operations are not waited for (the memory wave waits for exactly the copies the next
+// block needs); the memory clobber keeps the compiler from moving memory accesses across it
+__device__ __forceinline__ void wg_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// s_waitcnt vmcnt(k) for a wave-uniform k in [0, 63]: wait until at most k vector memory
+// operations of this wave are outstanding.  Loads complete in issue order, so after the wait every
+// load older than the newest k memory operations has completed (stores may complete out of
+// order; callers that need them done wait for 0).
+__device__ __forceinline__ void vm_wait_le(int k) {
+    switch (k) {
+#define CN_VMW(K) case K: asm volatile("s_waitcnt vmcnt(" #K ")" ::: "memory"); break;
+        CN_VMW(0) CN_VMW(1) CN_VMW(2) CN_VMW(3) CN_VMW(4) CN_VMW(5) CN_VMW(6) CN_VMW(7) CN_VMW(8) CN_VMW(9)
+        CN_VMW(10) CN_VMW(11) CN_VMW(12) CN_VMW(13) CN_VMW(14) CN_VMW(15) CN_VMW(16) CN_VMW(17) CN_VMW(18)
+        CN_VMW(19) CN_VMW(20) CN_VMW(21) CN_VMW(22) CN_VMW(23) CN_VMW(24) CN_VMW(25) CN_VMW(26) CN_VMW(27)
+        CN_VMW(28) CN_VMW(29) CN_VMW(30) CN_VMW(31) CN_VMW(32) CN_VMW(33) CN_VMW(34) CN_VMW(35) CN_VMW(36)
+        CN_VMW(37) CN_VMW(38) CN_VMW(39) CN_VMW(40) CN_VMW(41) CN_VMW(42) CN_VMW(43) CN_VMW(44) CN_VMW(45)
+        CN_VMW(46) CN_VMW(47) CN_VMW(48) CN_VMW(49) CN_VMW(50) CN_VMW(51) CN_VMW(52) CN_VMW(53) CN_VMW(54)
+        CN_VMW(55) CN_VMW(56) CN_VMW(57) CN_VMW(58) CN_VMW(59) CN_VMW(60) CN_VMW(61) CN_VMW(62)
+#undef CN_VMW
+        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
 template <class T>
 __device__ __forceinline__ T lds_at(const char* p) { return *reinterpret_cast<const T*>(p); }
 
@@ -557,21 +584,25 @@ __host__ __device__ constexpr int rb_for(int r) { return r <= 2 ? 2 : MAXR; }
 
 // backward block blk (rows jhi = n-1-blk*SPX down to jlo) into buf:
 // [U: SPX rows x US][V^T: RB x SPX (rows >= r: pads)][in: SPX lane slots, slot s = row jhi - s]
+// The lane-dependent copies go first, the whole-wave copies last; returns how many instructions
+// the whole-wave copies issued (the newest ones: a wait for at most that many outstanding
+// operations leaves only this block's last copies in flight).
 template <class T, int SPX, int RB>
-__device__ __forceinline__ void bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
-                                        int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
+__device__ __forceinline__ int bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
+                                       int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
     constexpr int CE = IL * (int)sizeof(T);
     const int jhi = n - 1 - blk * SPX;
     const int jlo = max(0, jhi - SPX + 1);
     const int cnt = jhi - jlo + 1;
     const int bU = SPX * US * CE, bV = RB * SPX * CE;
-    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
-    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
     if (live) {
 #pragma unroll
         for (int s = 0; s < SPX; ++s)
             if (jhi - s >= jlo) LaneSlot<T>::load(in + (size_t)(jhi - s) * in_st, buf + bU + bV + s * LaneSlot<T>::BYTES);
     }
+    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
+    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
+    return r * dma16_count(cnt * CE) + dma16_count(cnt * US * CE);
 }
 
 // One backward step of a lane: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), t += V^T(:, j) out_j.
@@ -635,10 +666,15 @@ __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0
 // block b is computed.  A wave that issues memory instructions stalls while the CU's share of
 // the HBM bandwidth drains them; with the roles split, that stall no longer holds up the
 // arithmetic, and one workgroup barrier per block hands the buffers over.
-// LDS: [buffer 0][buffer 1][stage 0][stage 1], stage = STAGE_BYTES (the larger of SPX lane slots
-// and the segment layout).
+// LDS: [buffer 0][buffer 1][buffer 2][stage 0][stage 1], stage = stage_bytes (the larger of SPX
+// lane slots and the segment layout).  The memory wave keeps two blocks in flight and, before each
+// barrier, waits only until the block needed next has landed (counted waits), so that HBM requests
+// keep flowing across block boundaries.
 // ------------------------------------------------------------------------------------------
 __host__ __device__ constexpr int stage_bytes(int SPX, int esz) { return seg_bytes(SPX, esz); }
+// LDS buffers of a sweep: nb = 3 when three of the sweep's buffers fit the launch's buffer region
+// (block b is computed from buffer b % 3 while blocks b+1 and b+2 are in flight), else 2
+__host__ __device__ constexpr int sweep_nbuf(int sweep_bytes, int region) { return 3 * sweep_bytes <= region ? 3 : 2; }
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -647,37 +683,41 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 // `in` and `out` may be the same array (in place).  Both waves call it (t is the compute wave's).
 // Full blocks run without per-step tests; a partial last block step by step.
 template <class T, int WU, int SPX, int RB>
-__device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int bufbytes,
+__device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int region,
                                           size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
                                           size_t out_st, bool live, int wave, int lane, int mlx, T (&t)[RB]) {
     constexpr int GB = LaneSlot<T>::BYTES;
     const int nblk = (n + SPX - 1) / SPX;
-    char* stage = lds + 2 * bufbytes;
+    char* stage = lds + region;
     const int SB = stage_bytes(SPX, (int)sizeof(T));
+    const int bufbytes = bwd_bytes(SPX, US, RB, (int)sizeof(T));
+    const int nb = sweep_nbuf(bufbytes, region), ahead = nb - 1;
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    if (wave == 1) {
+    if (wave == 1) {   // blocks 0 .. ahead-1 in flight, block 0 complete
         bwd_dma<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
-        dma_wait();
+        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, in, in_st, live, lane) : 0);
     }
-    __syncthreads();
+    wg_sync();
     for (int b = 0; b < nblk; ++b) {
-        const int buf = b & 1;
         const int jhi = n - 1 - b * SPX;
         const int jlo = max(0, jhi - SPX + 1);
         const int cnt = jhi - jlo + 1;
         if (wave == 1) {
             if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
-                const char* sb = stage + (buf ^ 1) * SB;
+                const char* sb = stage + ((b - 1) & 1) * SB;
 #pragma unroll
                 for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
-            if (b + 1 < nblk) bwd_dma<T, SPX, RB>(Uf, Vt, lds + (buf ^ 1) * bufbytes, ch, n, r, US, b + 1, in, in_st, live, lane);
-            dma_wait();
+            // block b+ahead into the buffer block b-1 used; then block b+1 must be complete
+            const int c = b + ahead < nblk ? bwd_dma<T, SPX, RB>(Uf, Vt, lds + ((b + ahead) % nb) * bufbytes, ch, n, r,
+                                                                  US, b + ahead, in, in_st, live, lane)
+                                           : 0;
+            vm_wait_le(ahead > 1 ? c : 0);
         } else {
-            const char* bb = lds + buf * bufbytes;
-            char* sb = stage + buf * SB;
+            const char* bb = lds + (b % nb) * bufbytes;
+            char* sb = stage + (b & 1) * SB;
             if (cnt == SPX) {   // operands of step s+1 in flight during step s
                 BwdOps<T, WU, RB> op[2];
                 bwd_ops_load<T, WU, SPX, RB>(op[0], bb, SPX - 1, 0, US, mlx, lane);
@@ -694,17 +734,19 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
                     if (s < cnt) LaneSlot<T>::write(sb + s * GB, lane, bwd_lane_step<T, WU, SPX, RB>(bb, cnt - 1 - s, s, US, mlx, lane, xw, t));
             }
         }
-        __syncthreads();
+        wg_sync();
     }
-    if (wave == 1 && live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
-        const int jhi = n - 1 - (nblk - 1) * SPX;
-        const char* sb = stage + ((nblk - 1) & 1) * SB;
+    if (wave == 1) {
+        if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
+            const int jhi = n - 1 - (nblk - 1) * SPX;
+            const char* sb = stage + ((nblk - 1) & 1) * SB;
 #pragma unroll
-        for (int s = 0; s < SPX; ++s)
-            if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
+            for (int s = 0; s < SPX; ++s)
+                if (jhi - s >= 0) out[(size_t)(jhi - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
+        }
+        dma_wait();
     }
-    if (wave == 1) dma_wait();
-    __syncthreads();
+    wg_sync();
 }
 
 // One forward step of a lane (window y[0..KL] = rows j .. j+KL): optional row interchange,
@@ -798,7 +840,8 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     T* wl = ws + (size_t)m * NR + k;
     const int fL = SPX * KL * CE, fP = SPX * IL * 4;
     const int nblk = (n + SPX - 1) / SPX;
-    char* stage = lds + 2 * bufbytes;
+    const int region = bufbytes;                // the buffers of every sweep, then the two stages
+    char* stage = lds + region;
     constexpr int SB = stage_bytes(SPX, (int)sizeof(T));
     // the memory wave's part of the segment DMA: entry se of segment SPI*i + sg in instruction i,
     // i.e. system ch*IL + (SPI/NR)*i + sg/NR, right-hand side sg % NR
@@ -809,15 +852,16 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int my_slot = SG::slot(lane);                // where this lane's own segment is read/written
 
     // ---- forward sweep: y_c = L^-1 P_B P b into the work space
+    const int fbuf = fwd_bytes(SPX, KL, (int)sizeof(T));
+    const int fnb = sweep_nbuf(fbuf, region), fahead = fnb - 1;
     T y[KL + 1];
     // memory wave: copy of forward block bn into buffer nb; pc = row of the right-hand side behind
     // entry se of block bn, on return the one of block bn+1
     int pc = (KL + 1 + se < n) ? perm[KL + 1 + se] : -1;
-    auto fwd_copy = [&](int bn, char* nb) {
+    // (lane-dependent copies first; returns the number of whole-wave copies issued last)
+    auto fwd_copy = [&](int bn, char* nb) -> int {
         const int j0 = bn * SPX;
         const int cnt = min(SPX, n - j0);
-        dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
-        dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
         char* cs = nb + fL + fP;
         if (pc >= 0) {
             const T* src = bin + seg_off + pc;
@@ -830,6 +874,9 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         const int jn = j0 + SPX + KL + 1 + se;    // entry se of block bn+1
         pc = jn < n ? perm[jn] : -1;
+        dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
+        dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
+        return dma16_count(cnt * KL * CE) + dma16_count(cnt * IL * 4);
     };
     auto fwd_store = [&](int bs) {                // forward results of block bs from its stage
         const char* sb = stage + (bs & 1) * SB;
@@ -842,21 +889,20 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         const T* bl = bin + k * rhs_stride + (size_t)m * n;
 #pragma unroll
         for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
-    } else {
+    } else {   // blocks 0 .. ahead-1 in flight, block 0 complete
         fwd_copy(0, lds);
-        dma_wait();
+        vm_wait_le(fahead > 1 && nblk > 1 ? fwd_copy(1, lds + fbuf) : 0);
     }
-    __syncthreads();
+    wg_sync();
     for (int b = 0; b < nblk; ++b) {
-        const int buf = b & 1;
         if (wave == 1) {
             if (b >= 1 && live) fwd_store(b - 1);
-            if (b + 1 < nblk) fwd_copy(b + 1, lds + (buf ^ 1) * bufbytes);
-            dma_wait();
+            const int c = b + fahead < nblk ? fwd_copy(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
+            vm_wait_le(fahead > 1 ? c : 0);
         } else {
-            const char* fb = lds + buf * bufbytes;
+            const char* fb = lds + (b % fnb) * fbuf;
             const char* cb = fb + fL + fP + my_slot;
-            char* sb = stage + buf * SB;
+            char* sb = stage + (b & 1) * SB;
             const int j0 = b * SPX, cnt = min(SPX, n - j0);
             if (cnt == SPX) {
                 // any row interchange in this block for the chunk's systems?
@@ -876,23 +922,25 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
                                                                       LaneSlot<T>::read(cb, SG::pos(lane, s)), mlx));
             }
         }
-        __syncthreads();
+        wg_sync();
     }
     if (wave == 1) {
         if (live) fwd_store(nblk - 1);
         dma_wait();
     }
-    __syncthreads();
+    wg_sync();
 
     // ---- backward sweep: y = U^-1 y_c in place, t = V^T y
     T t[RB];
 #pragma unroll
     for (int a = 0; a < RB; ++a) t[a] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t);
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, region, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t);
 
     // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
     // slots]; x leaves by segments through the stages
     const int cW = RB * SPX * CE;
+    const int cbuf = cor_bytes(SPX, RB, (int)sizeof(T));
+    const int cnb = sweep_nbuf(cbuf, region), cahead = cnb - 1;
     if (wave == 0) {
         T z[MAXR];   // the pad entries t[a >= r] are dropped: W z must not see them
 #pragma unroll
@@ -902,21 +950,21 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         if (live && r > 0) cap_solve<T>(z, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
 #pragma unroll
         for (int a = 0; a < RB; ++a) t[a] = z[a];
-        for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in both buffers
-            *reinterpret_cast<T*>(lds + q * (int)sizeof(T)) = cn::zero<T>();
-            *reinterpret_cast<T*>(lds + bufbytes + q * (int)sizeof(T)) = cn::zero<T>();
+        for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in every buffer
+            for (int u = 0; u < cnb; ++u) *reinterpret_cast<T*>(lds + u * cbuf + q * (int)sizeof(T)) = cn::zero<T>();
         }
     }
     T* xseg = xout + seg_off + se;
-    auto cor_copy = [&](int bn, char* nb) {
+    auto cor_copy = [&](int bn, char* nb) -> int {   // lane-dependent copies first (as fwd_copy)
         const int j0 = bn * SPX;
         const int cnt = min(SPX, n - j0);
-        for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
         if (live) {
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
                 if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
         }
+        for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
+        return r * dma16_count(cnt * CE);
     };
     auto cor_store = [&](int bs) {                // x entries of block bs from its stage, whole segment runs
         const char* sb = stage + (bs & 1) * SB;
@@ -929,18 +977,17 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     };
     if (wave == 1) {
         cor_copy(0, lds);
-        dma_wait();
+        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy(1, lds + cbuf) : 0);
     }
-    __syncthreads();
+    wg_sync();
     for (int b = 0; b < nblk; ++b) {
-        const int buf = b & 1;
         if (wave == 1) {
             if (b >= 1) cor_store(b - 1);
-            if (b + 1 < nblk) cor_copy(b + 1, lds + (buf ^ 1) * bufbytes);
-            dma_wait();
+            const int c = b + cahead < nblk ? cor_copy(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
+            vm_wait_le(cahead > 1 ? c : 0);
         } else {
-            const char* cb = lds + buf * bufbytes;
-            char* sb = stage + buf * SB + my_slot;
+            const char* cb = lds + (b % cnb) * cbuf;
+            char* sb = stage + (b & 1) * SB + my_slot;
 #pragma unroll
             for (int s = 0; s < SPX; ++s) {
                 T v = LaneSlot<T>::read(cb + cW + s * GB, lane);
@@ -949,7 +996,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
                 LaneSlot<T>::write(sb, SG::pos(lane, s), v);
             }
         }
-        __syncthreads();
+        wg_sync();
     }
     if (wave == 1) {
         cor_store(nblk - 1);
@@ -1070,22 +1117,6 @@ __device__ __forceinline__ int dmaT_count(int count) {
 }
 __device__ __forceinline__ int dma4_count(int nbytes) { return (nbytes + 255) / 256; }
 
-// s_waitcnt vmcnt(k) for a wave-uniform k in [0, 63]: wait until at most k vector memory
-// operations of this wave are outstanding (they complete in order: only loads are in flight).
-__device__ __forceinline__ void vm_wait_le(int k) {
-    switch (k) {
-#define CN_VMW(K) case K: asm volatile("s_waitcnt vmcnt(" #K ")" ::: "memory"); break;
-        CN_VMW(0) CN_VMW(1) CN_VMW(2) CN_VMW(3) CN_VMW(4) CN_VMW(5) CN_VMW(6) CN_VMW(7) CN_VMW(8) CN_VMW(9)
-        CN_VMW(10) CN_VMW(11) CN_VMW(12) CN_VMW(13) CN_VMW(14) CN_VMW(15) CN_VMW(16) CN_VMW(17) CN_VMW(18)
-        CN_VMW(19) CN_VMW(20) CN_VMW(21) CN_VMW(22) CN_VMW(23) CN_VMW(24) CN_VMW(25) CN_VMW(26) CN_VMW(27)
-        CN_VMW(28) CN_VMW(29) CN_VMW(30) CN_VMW(31) CN_VMW(32) CN_VMW(33) CN_VMW(34) CN_VMW(35) CN_VMW(36)
-        CN_VMW(37) CN_VMW(38) CN_VMW(39) CN_VMW(40) CN_VMW(41) CN_VMW(42) CN_VMW(43) CN_VMW(44) CN_VMW(45)
-        CN_VMW(46) CN_VMW(47) CN_VMW(48) CN_VMW(49) CN_VMW(50) CN_VMW(51) CN_VMW(52) CN_VMW(53) CN_VMW(54)
-        CN_VMW(55) CN_VMW(56) CN_VMW(57) CN_VMW(58) CN_VMW(59) CN_VMW(60) CN_VMW(61) CN_VMW(62)
-#undef CN_VMW
-        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
-    }
-}
 
 // Staging copies of one block (returns the number of copy instructions issued)
 template <class T>

@@ -168,14 +168,13 @@ def rb2d_step_bytes(pb) -> float:
     """Algorithmic HBM bytes of one SBDF2 step of a RB 2D run: the batched solve of its K mode
     systems (band rows of kl + ku + 1 = 31 entries for U and 12 for L, pivots, V^T and W of r = 7
     rows, S, the right-hand side read and x written: 16 (43 n + 2 r n + 2 n) + 4 n + 16 r^2 per
-    system), the explicit term (X read, the 6 K Mz line values and the 3 K Mz line products each
-    written and read, F rows written: 16 K (2 n + 18 Mz)) and the stage (history M X and F of two
-    levels read, M X and the
+    system), the explicit term (X read, the 3 K Mz line products written and read, F rows
+    written: 16 K (2 n + 6 Mz)) and the stage (history M X and F of two levels read, M X and the
     right-hand side written, CSR of M with ~3 entries per row: 16 K n (4 + 2) + 12 * 3 K n)."""
     K, n, Mz = pb.K, pb.n, pb.Mz
     kl, ku, r = 12, 18, 7
     solve = K * (16 * ((kl + ku + 1 + kl) * n + 2 * r * n + 2 * n) + 4 * n + 16 * r * r)
-    explicit = 16 * K * (2 * n + 18 * Mz)
+    explicit = 16 * K * (2 * n + 6 * Mz)
     stage = 16 * K * n * 6 + 12 * 3 * K * n
     return float(solve + explicit + stage)
 

@@ -4,12 +4,10 @@
 // 3 outputs, derivative chains, backward transforms, products, forward transforms, C^(lam)
 // conversion into the stacked rows).
 //
-// Three launches (grid y: ensemble member, the members' states (K, n) one after another):
-//   Z  one workgroup per mode k: z-backward of the 6 factor columns (u, u_z, w, w_z, T, T_z; the
-//      z-derivative is folded into the matrix Bz_1 = T_n'(z_j)) at every grid line, the mode's
-//      coefficients staged in LDS; written line-major to the scratch behind the products.
-//   A  one workgroup per grid line z_j (Mz workgroups per member), everything in LDS: the line's
-//      6 K values read as one contiguous row, x-backward of the 8 factors u, u_x, u_z, w,
+// Two launches (grid y: ensemble member, the members' states (K, n) one after another):
+//   A  one workgroup per grid line z_j (Mz workgroups per member), everything in LDS:
+//      z-backward of the 6 factor columns of the modes (u, u_z, w, w_z, T, T_z; the z-derivative
+//      is folded into the matrix Bz_1 = T_n'(z_j)), x-backward of the 8 factors u, u_x, u_z, w,
 //      w_x, w_z, T_x, T_z at the Mx points of the line (x-derivative i k kappa on the modes), the 3
 //      products, and the x-forward transform of the products (modes k < K: dealiasing).  The
 //      Fourier weights come from one table of Mx twiddles W_m = exp(2 pi i m / Mx)
@@ -29,52 +27,40 @@ using cn::zc;
 
 namespace {
 
-// z-backward of one mode: Zall[j][2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)] for every grid line
-// z_j (v = 0: value, v = 1: z-derivative), the mode's row of X staged in LDS; one workgroup per
-// (mode, member).  The sums run over n in order, as the line kernel's did.
 __global__ void __launch_bounds__(256)
-k_fc_zback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
-           zc* __restrict__ Zall, int K, int n, int N, int Mz) {
-    extern __shared__ __align__(16) char lds[];
-    zc* xs = reinterpret_cast<zc*>(lds);            // [3][N]: the three fields' coefficients of mode k
-    const int k = blockIdx.x, tid = threadIdx.x;
-    X += (size_t)blockIdx.y * K * n + (size_t)k * n;
-    Zall += (size_t)blockIdx.y * Mz * 6 * K;
-    for (int q = tid; q < 3 * N; q += blockDim.x) xs[q] = X[fidx[q]];
-    __syncthreads();
-    for (int p = tid; p < 3 * Mz; p += blockDim.x) {
-        const int f = p / Mz, j = p - f * Mz;
-        const zc* xf = xs + f * N;
-        const double* b0 = Bz + j;
-        const double* b1 = Bz + (size_t)N * Mz + j;
-        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
-#pragma unroll 16
-        for (int nn = 0; nn < N; ++nn) {
-            const zc x = xf[nn];
-            const double c0 = b0[(size_t)nn * Mz], c1 = b1[(size_t)nn * Mz];
-            r0 = fma(c0, x.re, r0);
-            i0 = fma(c0, x.im, i0);
-            r1 = fma(c1, x.re, r1);
-            i1 = fma(c1, x.im, i1);
-        }
-        zc* zj = Zall + (size_t)j * 6 * K;
-        zj[(2 * f) * K + k] = zc{r0, i0};
-        zj[(2 * f + 1) * K + k] = zc{r1, i1};
-    }
-}
-
-__global__ void __launch_bounds__(256)
-k_fc_line(const zc* __restrict__ Zall, const zc* __restrict__ Wt, zc* __restrict__ Gh, int K, int Mz, int Mx,
-          double kappa) {
+k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
+          const zc* __restrict__ Wt, zc* __restrict__ Gh, int K, int n, int N, int Mz, int Mx, double kappa) {
     extern __shared__ __align__(16) char lds[];
     zc* Zj = reinterpret_cast<zc*>(lds);            // [6][K]: (u, uz, w, wz, T, Tz) at z_j per mode
     zc* W = Zj + 6 * K;                              // [Mx] twiddles
     double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx] products on the line
+    double* b = G + 3 * Mx;                          // [2][N]: T_n(z_j), T_n'(z_j)
     const int j = blockIdx.x, tid = threadIdx.x;
-    Zall += (size_t)blockIdx.y * Mz * 6 * K + (size_t)j * 6 * K;
+    X += (size_t)blockIdx.y * K * n;
     Gh += (size_t)blockIdx.y * 3 * K * Mz;
     for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wt[q];
-    for (int q = tid; q < 6 * K; q += blockDim.x) Zj[q] = Zall[q];
+    for (int q = tid; q < 2 * N; q += blockDim.x) {
+        const int v = q / N, nn = q - v * N;
+        b[q] = Bz[((size_t)v * N + nn) * Mz + j];
+    }
+    __syncthreads();
+    // z-backward: Zj[2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)]
+    for (int p = tid; p < 3 * K; p += blockDim.x) {
+        const int f = p / K, k = p - f * K;
+        const zc* Xk = X + (size_t)k * n;
+        const int* pos = fidx + (size_t)f * N;
+        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+        for (int nn = 0; nn < N; ++nn) {
+            const zc x = Xk[pos[nn]];
+            const double b0 = b[nn], b1 = b[N + nn];
+            r0 = fma(b0, x.re, r0);
+            i0 = fma(b0, x.im, i0);
+            r1 = fma(b1, x.re, r1);
+            i1 = fma(b1, x.im, i1);
+        }
+        Zj[(2 * f) * K + k] = zc{r0, i0};
+        Zj[(2 * f + 1) * K + k] = zc{r1, i1};
+    }
     __syncthreads();
     // x-backward of the 8 factors at x_i and the products
     for (int i = tid; i < Mx; i += blockDim.x) {
@@ -168,19 +154,15 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
         cn::set_error("cn_rb2d_explicit: inconsistent sizes");
         return CN_EINVAL;
     }
-    const size_t lds_z = (size_t)3 * Nz * sizeof(zc);
-    const size_t lds_a = (size_t)(6 * K + Mx) * sizeof(zc) + (size_t)3 * Mx * sizeof(double);
+    const size_t lds_a = (size_t)(6 * K + Mx) * sizeof(zc) + (size_t)(3 * Mx + 2 * Nz) * sizeof(double);
     const size_t lds_b = (size_t)3 * Mz * sizeof(zc);
-    if (lds_a > 64 * 1024 || lds_b > 64 * 1024 || lds_z > 64 * 1024) {
+    if (lds_a > 64 * 1024 || lds_b > 64 * 1024) {
         cn::set_error("cn_rb2d_explicit: grid too large for the line kernels (LDS)");
         return CN_EUNSUPPORTED;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    zc* zall = static_cast<zc*>(Gh) + (size_t)members * 3 * K * Mz;   // members x Mz x 6K behind the products
-    hipLaunchKernelGGL(k_fc_zback, dim3(K, members), dim3(256), lds_z, s, static_cast<const zc*>(X), fidx, Bz,
-                       zall, K, n, Nz, Mz);
-    hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, zall, static_cast<const zc*>(Wx),
-                       static_cast<zc*>(Gh), K, Mz, Mx, kappa);
+    hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
+                       static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
     hipLaunchKernelGGL(k_fc_mode_rows, dim3(K, members), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
                        static_cast<zc*>(F), K, Mz, Nz, n, R);
     hipError_t e = hipGetLastError();

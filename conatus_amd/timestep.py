@@ -28,7 +28,7 @@ def _torch():
 
 class RB2DExplicit:
     """F(X) of 2D Rayleigh-Benard on the device (cn_rb2d_explicit_many): `members` states of one
-    discretization stored one after another (members x K x n), one launch sequence for all of them.
+    discretization stored one after another (members x K x n), one launch pair for all of them.
     F does not depend on Ra or Pr, so members with different parameters share it."""
 
     def __init__(self, pb: RayleighBenard2D, members: int = 1):
@@ -42,7 +42,7 @@ class RB2DExplicit:
         self.Wx = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mx) / Mx)).to(dev)        # twiddles
         Az = (pb.S02 @ S.cheb_forward_matrix(N, Mz)).T
         self.Az = torch.from_numpy(np.ascontiguousarray(Az)).to(dev)                       # (Mz, N)
-        self.Gh = torch.empty(self.members * 9 * K * Mz, dtype=torch.complex128, device=dev)   # scratch
+        self.Gh = torch.empty(self.members * 3 * K * Mz, dtype=torch.complex128, device=dev)   # scratch
         self._base = torch.from_numpy(np.tile(pb.b0().ravel(), self.members)).to(dev)
         fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
         self.fidx = torch.from_numpy(np.ascontiguousarray(fidx)).to(dev)

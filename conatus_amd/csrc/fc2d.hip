@@ -50,6 +50,7 @@ k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* 
         const zc* Xk = X + (size_t)k * n;
         const int* pos = fidx + (size_t)f * N;
         double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 8
         for (int nn = 0; nn < N; ++nn) {
             const zc x = Xk[pos[nn]];
             const double b0 = b[nn], b1 = b[N + nn];
@@ -93,22 +94,44 @@ k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* 
         G[2 * Mx + i] = -fma(u, Tx, w * Tz);
     }
     __syncthreads();
-    // x-forward: Gh[o][k][j] = (1/Mx) sum_i conj(W_{k i mod Mx}) G[o][i]
+    // x-forward: Gh[o][k][j] = (1/Mx) sum_i conj(W_{k i mod Mx}) G[o][i].  When the 3K outputs
+    // fit the workgroup, one thread per output; else the three products of mode k by one thread
+    // (one twiddle read for three outputs).  Same sums in the same order either way.
     const double inv = 1.0 / Mx;
-    for (int p = tid; p < 3 * K; p += blockDim.x) {
-        const int o = p / K, k = p - o * K;
-        const double* g = G + (size_t)o * Mx;
-        double re = 0.0, im = 0.0;
+    if (3 * K <= (int)blockDim.x) {
+        for (int p = tid; p < 3 * K; p += blockDim.x) {
+            const int o = p / K, k = p - o * K;
+            const double* g = G + (size_t)o * Mx;
+            double re = 0.0, im = 0.0;
+            int m = 0;
+            for (int i = 0; i < Mx; ++i) {
+                const zc t = W[m];
+                const double gi = g[i] * inv;
+                re = fma(t.re, gi, re);
+                im = fma(-t.im, gi, im);
+                m += k;
+                if (m >= Mx) m -= Mx;
+            }
+            Gh[((size_t)o * K + k) * Mz + j] = zc{re, im};
+        }
+        return;
+    }
+    for (int k = tid; k < K; k += blockDim.x) {
+        double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
         int m = 0;
         for (int i = 0; i < Mx; ++i) {
             const zc t = W[m];
-            const double gi = g[i] * inv;
-            re = fma(t.re, gi, re);
-            im = fma(-t.im, gi, im);
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const double gi = G[o * Mx + i] * inv;
+                re[o] = fma(t.re, gi, re[o]);
+                im[o] = fma(-t.im, gi, im[o]);
+            }
             m += k;
             if (m >= Mx) m -= Mx;
         }
-        Gh[((size_t)o * K + k) * Mz + j] = zc{re, im};
+#pragma unroll
+        for (int o = 0; o < 3; ++o) Gh[((size_t)o * K + k) * Mz + j] = zc{re[o], im[o]};
     }
 }
 

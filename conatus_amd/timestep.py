@@ -100,6 +100,18 @@ class IVPSolver:
         self.t = self.imex.run_graph(self.X, self.t, steps)
         return self.t
 
+    def save_checkpoint(self, path):
+        """State + SBDF history + clock (``conatus.checkpoint/1``, ``.npz``): a restart with
+        ``load_checkpoint`` continues bit for bit (integrator.py:997-1021)."""
+        from .checkpoint import save_checkpoint
+        return save_checkpoint(self, path)
+
+    def load_checkpoint(self, path) -> None:
+        """Continue from ``save_checkpoint``; the numerics and the problem must match
+        (integrator.py:1023-1076)."""
+        from .checkpoint import load_checkpoint
+        load_checkpoint(self, path)
+
     def explain(self) -> str:
         pb = self.problem
         return (f"IVPSolver(RB 2D {pb.Nx}x{pb.Nz}, Ra={pb.Ra:g}, Pr={pb.Pr:g}, {self.scheme}, dt={self.dt:g}): "

@@ -144,6 +144,26 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
            "steps": steps, "ms_per_step": ms, "steps_per_s": 1e3 / ms,
            "wall_ms_per_step": wall * 1e3 / steps, "mode_solves_per_s": pb.K * 1e3 / ms,
            "gate_relative_difference_20_steps": gate, "graph": True}
+    # the same problem stepped with RK443 (four stages, four solves and three explicit terms per
+    # step: the scheme of the reference's published GPU timings at this size)
+    rk = IVPSolver(pb, "RK443", 1e-4)
+    tw = time.perf_counter()
+    while time.perf_counter() - tw < 0.3:
+        rk.set_state(X0)
+        rk.run_graph(32)
+        rk.run_graph(8)
+        torch.cuda.synchronize()
+    rk.set_state(X0)
+    rk.run_graph(8)
+    torch.cuda.synchronize()
+    quiet_heap()
+    e0.record(stream)
+    rk.run_graph(steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    if not np.all(np.isfinite(rk.state())):
+        raise RuntimeError("rb2d RK443 state not finite after the timed steps")
+    out["rk443_ms_per_step"] = e0.elapsed_time(e1) / steps
     if with_cpu:
         # blocks of 100 steps from the initial state (the flow stays resolved), factorizations of
         # both startup orders built before the timing
@@ -428,7 +448,8 @@ def main() -> int:
     if not args.no_rb2d:
         log(f"RB 2D IMEX leg ({args.rb2d_steps} graph-replayed SBDF2 steps)")
         out["imex_rb2d"] = rb2d_leg(args.rb2d_steps, args.warmup, min(args.cpu_seconds, 5.0), not args.no_cpu)
-        log(f"rb2d: {out['imex_rb2d']['ms_per_step']:.4f} ms per step")
+        log(f"rb2d: {out['imex_rb2d']['ms_per_step']:.4f} ms per SBDF2 step, "
+            f"{out['imex_rb2d']['rk443_ms_per_step']:.4f} ms per RK443 step")
     if not args.no_ensemble:
         log(f"uneven ensemble leg ({args.ensemble_runs} runs)")
         out["ensemble"] = ensemble_leg(args.ensemble_runs, args.ensemble_steps, 1e-4)

@@ -12,7 +12,9 @@ restated from the reference, independently of the device kernels:
   out_msin = k cos, z-derivative by the Chebyshev backward recurrence, FFT-based backward
   transforms to the 3/2 grid, the products -u.grad(u), -u.grad(T) on the grid, forward
   transforms (truncation = dealiasing projection), conversion S_{0->2} into the equation rows;
-* SBDF step of core/include/conatus/imex.hpp:158-176 per mode, solves by oracle.banded (LAPACK).
+* SBDF step of core/include/conatus/imex.hpp:158-176 per mode, and the IMEX Runge-Kutta stages
+  of python/conatus/timestep/integrator.py:608-646 (stage j's F and L X only if a later stage
+  uses them; rhs = M X0 + sum dt A F - dt H L X in that order), solves by oracle.banded (LAPACK).
 """
 from __future__ import annotations
 
@@ -20,7 +22,7 @@ import numpy as np
 
 from . import spectral as S
 from .banded import BorderedBanded
-from .imex import sbdf_coefficients
+from .imex import rk_tableau, sbdf_coefficients
 
 
 def to_pairs(c: np.ndarray) -> np.ndarray:
@@ -53,7 +55,8 @@ def dx_pairs(p: np.ndarray, kappa: float) -> np.ndarray:
 class RB2DOracle:
     def __init__(self, pb, scheme: str = "SBDF2", dt: float = 1e-3):
         self.pb = pb
-        self.order = int(scheme[4])
+        self.order = int(scheme[4]) if scheme.startswith("SBDF") else 0
+        self.tab = None if self.order else rk_tableau(scheme)
         self.dt = dt
         self.M = [m[0] for m in pb.mats]
         self.L = [m[1] for m in pb.mats]
@@ -81,10 +84,10 @@ class RB2DOracle:
             F[:, pb.equation_rows(e)] = rows
         return F
 
-    def factor(self, k, alpha):
-        key = (k, alpha)
+    def factor(self, k, alpha, beta=1.0):
+        key = (k, alpha, beta)
         if key not in self.factors:
-            A = (alpha * self.M[k] + self.L[k]).tocsr()
+            A = (np.complex128(alpha) * self.M[k] + np.complex128(beta) * self.L[k]).tocsr()
             self.factors[key] = BorderedBanded(A, self.pb.border, np.complex128)
         return self.factors[key]
 
@@ -104,7 +107,28 @@ class RB2DOracle:
             rhs = rhs + (c[j] * self.Fh[j - 1] - (a[j] / self.dt) * self.MXh[j - 1])
         return np.stack([self.factor(k, a[0] / self.dt).solve(rhs[k]) for k in range(K)])
 
+    def step_rk(self, X: np.ndarray) -> np.ndarray:
+        K, dt = self.pb.K, self.dt
+        A, H, cc = self.tab
+        s = len(cc) - 1
+        Xs, F, LX = [X] + [None] * s, [None] * (s + 1), [None] * (s + 1)
+        MX0 = np.stack([self.M[k] @ X[k] for k in range(K)])
+        for i in range(1, s + 1):
+            j = i - 1
+            if any(A[m][j] != 0 for m in range(j + 1, s + 1)):
+                F[j] = self.explicit(Xs[j])
+            if any(H[m][j] != 0 for m in range(j + 1, s + 1)):
+                LX[j] = np.stack([self.L[k] @ Xs[j][k] for k in range(K)])
+            rhs = MX0.copy()
+            for k in range(i):
+                if A[i][k] != 0:
+                    rhs = rhs + (dt * A[i][k]) * F[k]
+                if H[i][k] != 0:
+                    rhs = rhs - (dt * H[i][k]) * LX[k]
+            Xs[i] = np.stack([self.factor(kk, 1.0, dt * H[i][i]).solve(rhs[kk]) for kk in range(K)])
+        return Xs[s]
+
     def run(self, X: np.ndarray, steps: int) -> np.ndarray:
         for _ in range(steps):
-            X = self.step(X)
+            X = self.step(X) if self.order else self.step_rk(X)
         return X

@@ -66,6 +66,27 @@ def test_trajectory_matches_oracle(Nx, Nz, dt, steps):
     assert abs(s.t - steps * dt) < 1e-12
 
 
+@pytest.mark.parametrize("scheme", ["RK443", "RK222"])
+def test_rk_trajectory_matches_oracle(scheme):
+    """The reference's published RB 2D timings use RK443 (four stages: F of the first three
+    stages, L X of the stages the later ones use, one solve of M + dt H_ii L per stage)."""
+    from conatus_amd.timestep import IVPSolver
+    pb = RayleighBenard2D(Nx=32, Nz=16, Lx=4.0, Ra=1e5, Pr=1.0)
+    X0 = pb.initial_state(seed=2)
+    s = IVPSolver(pb, scheme, 1e-3)
+    s.set_state(X0)
+    s.run(15)
+    Xo = RB2DOracle(pb, scheme, 1e-3).run(X0, 15)
+    assert np.all(np.isfinite(Xo))
+    r = rel(s.state(), Xo)
+    print(f"{scheme}: relative difference after 15 steps {r:.3e}", flush=True)
+    assert r < 1e-11
+    g = IVPSolver(pb, scheme, 1e-3)
+    g.set_state(X0)
+    g.run_graph(15)
+    assert np.array_equal(g.state(), s.state())
+
+
 def test_graph_replay_equals_eager_steps():
     from conatus_amd.timestep import IVPSolver
     pb = RayleighBenard2D(Nx=32, Nz=16, Lx=4.0, Ra=1e5, Pr=1.0)

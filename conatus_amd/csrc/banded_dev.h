@@ -584,25 +584,31 @@ __host__ __device__ constexpr int rb_for(int r) { return r <= 2 ? 2 : MAXR; }
 
 // backward block blk (rows jhi = n-1-blk*SPX down to jlo) into buf:
 // [U: SPX rows x US][V^T: RB x SPX (rows >= r: pads)][in: SPX lane slots, slot s = row jhi - s]
-// The lane-dependent copies go first, the whole-wave copies last; returns how many instructions
-// the whole-wave copies issued (the newest ones: a wait for at most that many outstanding
-// operations leaves only this block's last copies in flight).
+// Backward block blk, in two parts: the factor rows (V^T rows, then U; returns the number of
+// instructions issued, all whole-wave) and the lane operands (in slots; lane-dependent).
 template <class T, int SPX, int RB>
-__device__ __forceinline__ int bwd_dma(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
-                                       int r, int US, int blk, const T* in, size_t in_st, bool live, int lane) {
+__device__ __forceinline__ int bwd_dma_f(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
+                                         int r, int US, int blk, int lane) {
     constexpr int CE = IL * (int)sizeof(T);
     const int jhi = n - 1 - blk * SPX;
     const int jlo = max(0, jhi - SPX + 1);
     const int cnt = jhi - jlo + 1;
+    const int bU = SPX * US * CE;
+    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
+    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
+    return r * dma16_count(cnt * CE) + dma16_count(cnt * US * CE);
+}
+template <class T, int SPX, int RB>
+__device__ __forceinline__ void bwd_dma_l(char* buf, int n, int US, int blk, const T* in, size_t in_st, bool live) {
+    constexpr int CE = IL * (int)sizeof(T);
+    const int jhi = n - 1 - blk * SPX;
+    const int jlo = max(0, jhi - SPX + 1);
     const int bU = SPX * US * CE, bV = RB * SPX * CE;
     if (live) {
 #pragma unroll
         for (int s = 0; s < SPX; ++s)
             if (jhi - s >= jlo) LaneSlot<T>::load(in + (size_t)(jhi - s) * in_st, buf + bU + bV + s * LaneSlot<T>::BYTES);
     }
-    for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
-    dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
-    return r * dma16_count(cnt * CE) + dma16_count(cnt * US * CE);
 }
 
 // One backward step of a lane: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), t += V^T(:, j) out_j.
@@ -660,16 +666,16 @@ __device__ __forceinline__ T bwd_ops_step(const BwdOps<T, WU, RB>& op, T (&xw)[W
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 // ------------------------------------------------------------------------------------------
-// Two-wave workgroups of the streamed sweeps.  Wave 0 (compute) runs the recurrences from LDS
-// and leaves its results in an LDS stage; wave 1 (memory) issues every HBM access: the copies
-// of block b+1 into the other buffer and the stores of block b-1 from the other stage while
-// block b is computed.  A wave that issues memory instructions stalls while the CU's share of
-// the HBM bandwidth drains them; with the roles split, that stall no longer holds up the
-// arithmetic, and one workgroup barrier per block hands the buffers over.
-// LDS: [buffer 0][buffer 1][buffer 2][stage 0][stage 1], stage = stage_bytes (the larger of SPX
-// lane slots and the segment layout).  The memory wave keeps two blocks in flight and, before each
-// barrier, waits only until the block needed next has landed (counted waits), so that HBM requests
-// keep flowing across block boundaries.
+// Three-wave workgroups of the streamed sweeps.  Wave 0 (compute) runs the recurrences from LDS
+// and leaves its results in an LDS stage; waves 1 and 2 issue every HBM access while block b is
+// computed: wave 1 the factor rows of the next blocks (whole-wave global_load_lds copies), wave 2
+// the lane operands of block b+1 and the stores of block b-1 from the other stage.  A wave that
+// issues memory instructions stalls while its own copies drain (at most 63 in flight per wave);
+// with the roles split, that stall holds up neither the arithmetic nor the other memory wave, and
+// one workgroup barrier per block hands buffers and stages over.
+// LDS: [buffers][stage 0][stage 1], stage = stage_bytes (the larger of SPX lane slots and the
+// segment layout).  Where three buffers of a sweep fit, wave 1 keeps two blocks in flight and,
+// before each barrier, waits only until the block needed next has landed (counted waits).
 // ------------------------------------------------------------------------------------------
 __host__ __device__ constexpr int stage_bytes(int SPX, int esz) { return seg_bytes(SPX, esz); }
 // LDS buffers of a sweep: nb = 3 when three of the sweep's buffers fit the launch's buffer region
@@ -680,7 +686,7 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 // Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
 // t[a] += V^T(a, j) out_j (a < RB; the entries a >= r accumulate the LDS pads and are not used).
-// `in` and `out` may be the same array (in place).  Both waves call it (t is the compute wave's).
+// `in` and `out` may be the same array (in place).  Every wave calls it (t is the compute wave's).
 // Full blocks run without per-step tests; a partial last block step by step.
 template <class T, int WU, int SPX, int RB>
 __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int region,
@@ -695,9 +701,12 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
-    if (wave == 1) {   // blocks 0 .. ahead-1 in flight, block 0 complete
-        bwd_dma<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, in, in_st, live, lane);
-        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, in, in_st, live, lane) : 0);
+    if (wave == 1) {   // factor rows of blocks 0 .. ahead-1 in flight, block 0 complete
+        bwd_dma_f<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, lane);
+        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, lane) : 0);
+    } else if (wave == 2) {
+        bwd_dma_l<T, SPX, RB>(lds, n, US, 0, in, in_st, live);
+        dma_wait();
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -705,16 +714,19 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         const int jlo = max(0, jhi - SPX + 1);
         const int cnt = jhi - jlo + 1;
         if (wave == 1) {
+            // factor rows of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
+            const int c = b + ahead < nblk ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + ((b + ahead) % nb) * bufbytes, ch, n, r,
+                                                                    US, b + ahead, lane)
+                                           : 0;
+            vm_wait_le(ahead > 1 ? c : 0);
+        } else if (wave == 2) {
             if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
                 const char* sb = stage + ((b - 1) & 1) * SB;
 #pragma unroll
                 for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
-            // block b+ahead into the buffer block b-1 used; then block b+1 must be complete
-            const int c = b + ahead < nblk ? bwd_dma<T, SPX, RB>(Uf, Vt, lds + ((b + ahead) % nb) * bufbytes, ch, n, r,
-                                                                  US, b + ahead, in, in_st, live, lane)
-                                           : 0;
-            vm_wait_le(ahead > 1 ? c : 0);
+            if (b + 1 < nblk) bwd_dma_l<T, SPX, RB>(lds + ((b + 1) % nb) * bufbytes, n, US, b + 1, in, in_st, live);
+            dma_wait();
         } else {
             const char* bb = lds + (b % nb) * bufbytes;
             char* sb = stage + (b & 1) * SB;
@@ -736,7 +748,7 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         }
         wg_sync();
     }
-    if (wave == 1) {
+    if (wave == 2) {
         if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
             const int jhi = n - 1 - (nblk - 1) * SPX;
             const char* sb = stage + ((nblk - 1) & 1) * SB;
@@ -814,10 +826,10 @@ __device__ __forceinline__ void fwd_block(T (&y)[KL + 1], char* sb, const char* 
 // contiguous out of place.  The work space holds entry j of every (system, rhs) lane of the
 // chunk in one contiguous row (coalesced); the right-hand side enters and x leaves by lane
 // segments (Seg): per block, each lane's SPX entries travel as one run.  Forward blocks without
-// any row interchange in the chunk skip the (select-based) interchanges.  Two waves: lanes
-// 0..63 compute, lanes 64..127 move the data (same lane mapping).
+// any row interchange in the chunk skip the (select-based) interchanges.  Three waves: lanes
+// 0..63 compute, 64..127 copy factor rows, 128..191 move the lane data (same lane mapping).
 template <class T, int KL, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
           const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
@@ -855,13 +867,11 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int fbuf = fwd_bytes(SPX, KL, (int)sizeof(T));
     const int fnb = sweep_nbuf(fbuf, region), fahead = fnb - 1;
     T y[KL + 1];
-    // memory wave: copy of forward block bn into buffer nb; pc = row of the right-hand side behind
-    // entry se of block bn, on return the one of block bn+1
+    // wave 2: right-hand-side segments of forward block bn into buffer nb; pc = row of the
+    // right-hand side behind entry se of block bn, on return the one of block bn+1
     int pc = (KL + 1 + se < n) ? perm[KL + 1 + se] : -1;
-    // (lane-dependent copies first; returns the number of whole-wave copies issued last)
-    auto fwd_copy = [&](int bn, char* nb) -> int {
+    auto fwd_copy_l = [&](int bn, char* nb) {
         const int j0 = bn * SPX;
-        const int cnt = min(SPX, n - j0);
         char* cs = nb + fL + fP;
         if (pc >= 0) {
             const T* src = bin + seg_off + pc;
@@ -874,6 +884,11 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         const int jn = j0 + SPX + KL + 1 + se;    // entry se of block bn+1
         pc = jn < n ? perm[jn] : -1;
+    };
+    // wave 1: L rows and pivots of block bn (returns the number of instructions issued)
+    auto fwd_copy_f = [&](int bn, char* nb) -> int {
+        const int j0 = bn * SPX;
+        const int cnt = min(SPX, n - j0);
         dma16(Lf + (ch * n * KL + (size_t)j0 * KL) * IL, nb, cnt * KL * CE, lane);
         dma16(piv + (ch * n + j0) * IL, nb + fL, cnt * IL * 4, lane);
         return dma16_count(cnt * KL * CE) + dma16_count(cnt * IL * 4);
@@ -889,16 +904,22 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         const T* bl = bin + k * rhs_stride + (size_t)m * n;
 #pragma unroll
         for (int i = 0; i <= KL; ++i) y[i] = (live && i < n) ? bl[perm[i]] : cn::zero<T>();
-    } else {   // blocks 0 .. ahead-1 in flight, block 0 complete
-        fwd_copy(0, lds);
-        vm_wait_le(fahead > 1 && nblk > 1 ? fwd_copy(1, lds + fbuf) : 0);
+    } else if (wave == 1) {   // blocks 0 .. ahead-1 in flight, block 0 complete
+        fwd_copy_f(0, lds);
+        vm_wait_le(fahead > 1 && nblk > 1 ? fwd_copy_f(1, lds + fbuf) : 0);
+    } else {
+        fwd_copy_l(0, lds);
+        dma_wait();
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
         if (wave == 1) {
-            if (b >= 1 && live) fwd_store(b - 1);
-            const int c = b + fahead < nblk ? fwd_copy(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
+            const int c = b + fahead < nblk ? fwd_copy_f(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
             vm_wait_le(fahead > 1 ? c : 0);
+        } else if (wave == 2) {
+            if (b >= 1 && live) fwd_store(b - 1);
+            if (b + 1 < nblk) fwd_copy_l(b + 1, lds + ((b + 1) % fnb) * fbuf);
+            dma_wait();
         } else {
             const char* fb = lds + (b % fnb) * fbuf;
             const char* cb = fb + fL + fP + my_slot;
@@ -924,7 +945,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         wg_sync();
     }
-    if (wave == 1) {
+    if (wave == 2) {
         if (live) fwd_store(nblk - 1);
         dma_wait();
     }
@@ -955,14 +976,17 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
     }
     T* xseg = xout + seg_off + se;
-    auto cor_copy = [&](int bn, char* nb) -> int {   // lane-dependent copies first (as fwd_copy)
+    auto cor_copy_l = [&](int bn, char* nb) {        // wave 2: the work-space rows y of block bn
         const int j0 = bn * SPX;
-        const int cnt = min(SPX, n - j0);
         if (live) {
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
                 if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
         }
+    };
+    auto cor_copy_f = [&](int bn, char* nb) -> int { // wave 1: the W rows of block bn
+        const int j0 = bn * SPX;
+        const int cnt = min(SPX, n - j0);
         for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
         return r * dma16_count(cnt * CE);
     };
@@ -976,15 +1000,21 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
     };
     if (wave == 1) {
-        cor_copy(0, lds);
-        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy(1, lds + cbuf) : 0);
+        cor_copy_f(0, lds);
+        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_f(1, lds + cbuf) : 0);
+    } else if (wave == 2) {
+        cor_copy_l(0, lds);
+        dma_wait();
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
         if (wave == 1) {
-            if (b >= 1) cor_store(b - 1);
-            const int c = b + cahead < nblk ? cor_copy(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
+            const int c = b + cahead < nblk ? cor_copy_f(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
             vm_wait_le(cahead > 1 ? c : 0);
+        } else if (wave == 2) {
+            if (b >= 1) cor_store(b - 1);
+            if (b + 1 < nblk) cor_copy_l(b + 1, lds + ((b + 1) % cnb) * cbuf);
+            dma_wait();
         } else {
             const char* cb = lds + (b % cnb) * cbuf;
             char* sb = stage + (b & 1) * SB + my_slot;
@@ -998,16 +1028,16 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         wg_sync();
     }
-    if (wave == 1) {
+    if (wave == 2) {
         cor_store(nblk - 1);
         dma_wait();
     }
 }
 
 // Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
-// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s (two waves).
+// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s (three waves).
 template <class T, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
            int bufbytes) {
     extern __shared__ __align__(16) char lds[];

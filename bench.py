@@ -184,6 +184,67 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     return out
 
 
+def channel_leg(steps: int, gate_steps: int = 10) -> dict:
+    """IMEX steps/s of the 3D plane channel at 128 x 97 x 128 (Lx = 4 pi, Lz = 2 pi, Re = 2000,
+    laminar Poiseuille flow + a seeded divergence-free perturbation of kinetic energy 1e-4,
+    SBDF3, dt = 5e-4): 8128 complex mode systems (n = 395, r = 7) per step and the explicit term
+    on the 192 x 145 x 192 grid.  Accuracy gate first, at 16 x 33 x 16 against the CPU oracle
+    (velocity <= 2e-10 relative after `gate_steps` steps, the whole state reported: two
+    backward-stable CPU solves of this case already differ by 3e-11 in the velocity and 3e-8 in
+    the tau multipliers, the mode systems' condition numbers being near 1e6; the oracle needs
+    minutes per step at full size);
+    at full size the state must stay finite and the perturbation energy bounded.  The timed steps
+    are graph replays of full-order steps."""
+    import torch
+    from conatus_amd.problems import Channel3D
+    from conatus_amd.timestep import IVPSolver
+    from oracle.channel3d import Channel3DOracle
+    small = Channel3D(Nx=16, Ny=33, Nz=16, Re=2000.0)
+    Xs = small.initial_state(seed=1, energy=1e-3)
+    g = IVPSolver(small, "SBDF3", 5e-4)
+    g.set_state(Xs)
+    g.run(gate_steps)
+    ref = Channel3DOracle(small, "SBDF3", 5e-4).run(Xs, gate_steps)
+    vel = np.concatenate([small.field_index(f) for f in (0, 1, 2)])
+    got = g.state()
+    gate = float(np.max(abs(got[:, vel] - ref[:, vel])) / np.max(abs(ref[:, vel])))
+    full = float(np.max(abs(got - ref)) / np.max(abs(ref)))
+    log(f"channel accuracy gate (16x33x16): velocity relative difference after {gate_steps} steps {gate:.2e} "
+        f"(whole state incl. pressure and taus {full:.2e})")
+    if not gate <= 2e-10:                # the rounding floor of this case is 3e-11 (two backward-stable solves)
+        raise RuntimeError(f"channel accuracy gate failed ({gate:.3e})")
+    del g
+    pb = Channel3D(Nx=128, Ny=97, Nz=128, Re=2000.0)
+    X0 = pb.initial_state(seed=0, energy=1e-4)
+    s = IVPSolver(pb, "SBDF3", 5e-4)
+    log("channel: " + s.explain())
+    s.set_state(X0)
+    s.run_graph(16)
+    torch.cuda.synchronize()
+    quiet_heap()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    s.run_graph(steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    X = s.state()
+    if not np.all(np.isfinite(X)):
+        raise RuntimeError("channel state not finite after the timed steps")
+    energy = pb.kinetic_energy(pb.velocity(X - pb.poiseuille()))
+    if not 0 < energy < 1e-3:
+        raise RuntimeError(f"channel perturbation energy {energy:.3e} out of range")
+    return {"workload": "channel 3D 128x97x128 SBDF3 (8128 complex mode systems n=395, r=7; grid 192x145x192)",
+            "steps": steps, "ms_per_step": ms, "steps_per_s": 1e3 / ms, "wall_ms_per_step": wall * 1e3 / steps,
+            "mode_solves_per_s": pb.K * 1e3 / ms, "gate_velocity_relative_difference": gate,
+            "gate_state_relative_difference": full,
+            "gate_workload": f"16x33x16, {gate_steps} SBDF3 steps at dt = 5e-4",
+            "perturbation_energy_after": energy, "graph": True}
+
+
 def rb2d_step_bytes(pb) -> float:
     """Algorithmic HBM bytes of one SBDF2 step of a RB 2D run: the batched solve of its K mode
     systems (band rows of kl + ku + 1 = 31 entries for U and 12 for L, pivots, V^T and W of r = 7
@@ -313,6 +374,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rb2d", action="store_true", help="skip the RB 2D IMEX steps/s leg")
     ap.add_argument("--rb2d-steps", type=int, default=200)
+    ap.add_argument("--no-channel", action="store_true", help="skip the 3D channel IMEX leg")
+    ap.add_argument("--channel-steps", type=int, default=50)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-ensemble leg")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -450,6 +513,10 @@ def main() -> int:
         out["imex_rb2d"] = rb2d_leg(args.rb2d_steps, args.warmup, min(args.cpu_seconds, 5.0), not args.no_cpu)
         log(f"rb2d: {out['imex_rb2d']['ms_per_step']:.4f} ms per SBDF2 step, "
             f"{out['imex_rb2d']['rk443_ms_per_step']:.4f} ms per RK443 step")
+    if not args.no_channel:
+        log(f"3D channel IMEX leg ({args.channel_steps} graph-replayed SBDF3 steps)")
+        out["imex_channel"] = channel_leg(args.channel_steps)
+        log(f"channel: {out['imex_channel']['ms_per_step']:.3f} ms per SBDF3 step")
     if not args.no_ensemble:
         log(f"uneven ensemble leg ({args.ensemble_runs} runs)")
         out["ensemble"] = ensemble_leg(args.ensemble_runs, args.ensemble_steps, 1e-4)

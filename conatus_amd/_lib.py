@@ -22,7 +22,7 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
            "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
            "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode",
-           "cn_bb_copy_systems")
+           "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit")
 
 
 class Props(ctypes.Structure):
@@ -78,6 +78,9 @@ def _declare(lib):
         "cn_rb2d_explicit_many": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_double, ctypes.c_int, _vp]),
+        "cn_ch3d_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 6),
+        "cn_ch3d_explicit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]
+                             + [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_double, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -29,9 +29,7 @@ FORMAT = "conatus.checkpoint/1"
 
 
 def _identity(pb) -> dict:
-    return {"problem": "rayleigh_benard_2d", "Nx": int(pb.Nx), "Nz": int(pb.Nz), "Lx": float(pb.Lx),
-            "z0": float(pb.z0), "z1": float(pb.z1), "dealias": float(pb.dealias), "Ra": float(pb.Ra),
-            "Pr": float(pb.Pr)}
+    return pb.identity()
 
 
 def payload(solver) -> tuple[dict, dict]:

@@ -1,0 +1,303 @@
+// libcnhip explicit term of the 3D plane channel (primitive variables):
+//   F = base + S_{0->2} coefficients of -u.grad(u) on the 3/2-dealiased grid, in the x, y and z
+// momentum rows of every Fourier mode (kx, kz) (the reference's Fourier x Chebyshev x Fourier
+// explicit pipeline: python/conatus/timestep/fused.py:994-1085 with the 3D transforms of
+// python/conatus/timestep/xp_transforms.py:197-548 — factors, derivative chains, backward
+// transforms, products, forward transforms, C^(lam) conversion into the stacked rows).
+//
+// State X: (Kx * Kz, n) complex, mode m = kx * Kz + iz, kz = iz (iz < Nz/2) or iz - Kz, Kz = Nz - 1;
+// fidx[f][nn] = column of Chebyshev coefficient nn of velocity component f (u, v, w).
+// Five launches, each a direct transform along one axis with its operands staged in LDS:
+//   A  y-backward (per 8 modes):    H1[kx][j][6][iz]  = (u, u_y, v, v_y, w, w_y) at y_j
+//   B  z-backward (per 4 kx, y_j):  H2[j][zm][9][kx]  = per component: value, d/dz, d/dy at z_m
+//   C  x-line (per y_j, z_m):       x-backward of the 12 factors (d/dx = i kx kx0 on the modes),
+//                                   the products -u.grad(u), x-forward (kx < Kx): H3[j][zm][3][kx]
+//   D  z-forward (per 4 kx, y_j):   H4[j][kx][3][iz]  (modes kz of the represented range)
+//   E  y-forward + conversion (per 8 modes): F[m][R + 4 nn + e] = base + sum_j Ay[j][nn] H4[..]
+// Fourier sums use one twiddle table per axis, W_q = exp(2 pi i q / M); the index (k q mod M) is
+// carried incrementally.  H3 reuses H1's space and H4 reuses H2's (each is dead by then).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/cnhip.h"
+#include "cn_common.h"
+
+using cn::zc;
+
+namespace {
+
+constexpr int MA = 8;   // modes per workgroup of the y transforms (A, E): 128-byte runs in iz
+constexpr int KB = 4;   // kx per workgroup of the z transforms (B, D)
+
+struct ChDims {
+    int Kx, Kz, N, Mx, My, Mz;
+    size_t h1() const { return (size_t)Kx * My * 6 * Kz; }
+    size_t h2() const { return (size_t)My * Mz * 9 * Kx; }
+    size_t h3() const { return (size_t)My * Mz * 3 * Kx; }
+    size_t h4() const { return (size_t)My * Kx * 3 * Kz; }
+    size_t region1() const { return h1() > h3() ? h1() : h3(); }
+    size_t region2() const { return h2() > h4() ? h2() : h4(); }
+    size_t bytes() const { return (region1() + region2()) * sizeof(zc); }
+};
+
+__global__ void __launch_bounds__(256)
+k_ch_yback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ By,
+           zc* __restrict__ H1, int Kx, int Kz, int N, int My, int n) {
+    extern __shared__ __align__(16) char lds[];
+    zc* c = reinterpret_cast<zc*>(lds);              // [3][N][MA]
+    const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x;
+    for (int q = tid; q < 3 * N * MA; q += blockDim.x) {
+        const int l = q % MA, fn = q / MA;           // fn = f * N + nn
+        const int iz = iz0 + l;
+        c[q] = iz < Kz ? X[(size_t)(kx * Kz + iz) * n + fidx[fn]] : zc{0.0, 0.0};
+    }
+    __syncthreads();
+    const int l = tid % MA, iz = iz0 + l;
+    for (int p = tid / MA; p < 3 * My; p += blockDim.x / MA) {
+        const int f = p / My, j = p - f * My;
+        const double* b = By + (size_t)j * N * 2;   // (T_n(y_j), T_n'(y_j)) pairs
+        const zc* cf = c + (size_t)f * N * MA + l;
+        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 4
+        for (int nn = 0; nn < N; ++nn) {
+            const double b0 = b[2 * nn], b1 = b[2 * nn + 1];
+            const zc x = cf[nn * MA];
+            r0 = fma(b0, x.re, r0);
+            i0 = fma(b0, x.im, i0);
+            r1 = fma(b1, x.re, r1);
+            i1 = fma(b1, x.im, i1);
+        }
+        if (iz < Kz) {
+            zc* h = H1 + ((size_t)(kx * My + j) * 6 + 2 * f) * Kz + iz;
+            h[0] = zc{r0, i0};
+            h[Kz] = zc{r1, i1};
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_zback(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, int Kx, int Kz, int Nz,
+           int My, int Mz, double kz0) {
+    extern __shared__ __align__(16) char lds[];
+    zc* a = reinterpret_cast<zc*>(lds);              // [KB][6][Kz]
+    zc* W = a + (size_t)KB * 6 * Kz;                 // [Mz]
+    const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
+    for (int q = tid; q < KB * 6 * Kz; q += blockDim.x) {
+        const int l = q / (6 * Kz), r = q - l * 6 * Kz, kx = kxb + l;
+        a[q] = kx < Kx ? H1[(size_t)(kx * My + j) * 6 * Kz + r] : zc{0.0, 0.0};
+    }
+    for (int q = tid; q < Mz; q += blockDim.x) W[q] = Wz[q];
+    __syncthreads();
+    const int half = Nz / 2;
+    const int neg0 = Mz - (Kz - half);               // (kz mod Mz) of the first negative kz
+    for (int p = tid; p < Mz * KB; p += blockDim.x) {
+        const int l = p % KB, zm = p / KB, kx = kxb + l;
+        const zc* al = a + (size_t)l * 6 * Kz;
+        double acc[18];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) acc[q] = 0.0;
+        int idx = 0;                                 // (kz mod Mz) * zm mod Mz
+        for (int iz = 0; iz < Kz; ++iz) {
+            if (iz == half) idx = (int)(((long long)neg0 * zm) % Mz);
+            const zc e = W[idx];
+            const double kz = (double)(iz < half ? iz : iz - Kz) * kz0;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const zc v = al[(2 * cc) * Kz + iz], d = al[(2 * cc + 1) * Kz + iz];
+                const double pr = fma(e.re, v.re, -(e.im * v.im)), pi = fma(e.re, v.im, e.im * v.re);
+                acc[6 * cc + 0] += pr;
+                acc[6 * cc + 1] += pi;
+                acc[6 * cc + 2] = fma(-kz, pi, acc[6 * cc + 2]);   // d/dz: i kz (e v)
+                acc[6 * cc + 3] = fma(kz, pr, acc[6 * cc + 3]);
+                acc[6 * cc + 4] += fma(e.re, d.re, -(e.im * d.im));
+                acc[6 * cc + 5] += fma(e.re, d.im, e.im * d.re);
+            }
+            idx += zm;
+            if (idx >= Mz) idx -= Mz;
+        }
+        if (kx < Kx) {
+            zc* h = H2 + (size_t)(j * Mz + zm) * 9 * Kx + kx;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) h[(size_t)q * Kx] = zc{acc[2 * q], acc[2 * q + 1]};
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_xline(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, int Kx, int Mz, int Mx,
+           double kx0) {
+    extern __shared__ __align__(16) char lds[];
+    zc* z = reinterpret_cast<zc*>(lds);              // [9][Kx]: (val, dz, dy) of u, v, w per kx
+    zc* W = z + 9 * Kx;                              // [Mx]
+    double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx] products on the line
+    const int zm = blockIdx.x, j = blockIdx.y, tid = threadIdx.x;
+    const zc* src = H2 + (size_t)(j * Mz + zm) * 9 * Kx;
+    for (int q = tid; q < 9 * Kx; q += blockDim.x) z[q] = src[q];
+    for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wx[q];
+    __syncthreads();
+    for (int i = tid; i < Mx; i += blockDim.x) {
+        double f[12];                                // per component: value, d/dx, d/dy, d/dz
+#pragma unroll
+        for (int q = 0; q < 12; ++q) f[q] = 0.0;
+        int m = 0;                                   // kx i mod Mx
+        for (int kx = 0; kx < Kx; ++kx) {
+            const zc t = W[m];
+            const double wk = kx == 0 ? 1.0 : 2.0;
+            const zc e{wk * t.re, wk * t.im};
+            const double kk = kx * kx0;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const zc v = z[(3 * cc) * Kx + kx], dz = z[(3 * cc + 1) * Kx + kx], dy = z[(3 * cc + 2) * Kx + kx];
+                const double pr = fma(e.re, v.re, -(e.im * v.im)), pi = fma(e.re, v.im, e.im * v.re);
+                f[4 * cc + 0] += pr;
+                f[4 * cc + 1] = fma(-kk, pi, f[4 * cc + 1]);
+                f[4 * cc + 2] += fma(e.re, dy.re, -(e.im * dy.im));
+                f[4 * cc + 3] += fma(e.re, dz.re, -(e.im * dz.im));
+            }
+            m += i;
+            if (m >= Mx) m -= Mx;
+        }
+        const double u = f[0], v = f[4], w = f[8];
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            G[o * Mx + i] = -fma(u, f[4 * o + 1], fma(v, f[4 * o + 2], w * f[4 * o + 3]));
+    }
+    __syncthreads();
+    const double inv = 1.0 / Mx;
+    zc* dst = H3 + (size_t)(j * Mz + zm) * 3 * Kx;
+    for (int p = tid; p < 3 * Kx; p += blockDim.x) {
+        const int o = p / Kx, kx = p - o * Kx;
+        const double* g = G + (size_t)o * Mx;
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int i = 0; i < Mx; ++i) {
+            const zc t = W[m];
+            const double gi = g[i] * inv;
+            re = fma(t.re, gi, re);
+            im = fma(-t.im, gi, im);
+            m += kx;
+            if (m >= Mx) m -= Mx;
+        }
+        dst[p] = zc{re, im};
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz, int Mz) {
+    extern __shared__ __align__(16) char lds[];
+    zc* g = reinterpret_cast<zc*>(lds);              // [KB][3][Mz]
+    zc* W = g + (size_t)KB * 3 * Mz;                 // [Mz]
+    const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
+    for (int q = tid; q < KB * 3 * Mz; q += blockDim.x) {
+        const int l = q / (3 * Mz), r = q - l * 3 * Mz, o = r / Mz, zm = r - o * Mz, kx = kxb + l;
+        g[q] = kx < Kx ? H3[((size_t)(j * Mz + zm) * 3 + o) * Kx + kx] : zc{0.0, 0.0};
+    }
+    for (int q = tid; q < Mz; q += blockDim.x) W[q] = Wz[q];
+    __syncthreads();
+    const int half = Nz / 2;
+    const double inv = 1.0 / Mz;
+    for (int p = tid; p < Kz * KB * 3; p += blockDim.x) {
+        const int iz = p % Kz, lo = p / Kz, l = lo / 3, o = lo - 3 * l, kx = kxb + l;
+        const int kzm = iz < half ? iz : iz - Kz + Mz;   // kz mod Mz
+        const zc* gl = g + (size_t)(l * 3 + o) * Mz;
+        double re = 0.0, im = 0.0;
+        int idx = 0;
+        for (int zm = 0; zm < Mz; ++zm) {
+            const zc t = W[idx];
+            const zc v = gl[zm];
+            re += fma(t.re, v.re, t.im * v.im);      // conj(t) v
+            im += fma(t.re, v.im, -(t.im * v.re));
+            idx += kzm;
+            if (idx >= Mz) idx -= Mz;
+        }
+        if (kx < Kx) H4[((size_t)(j * Kx + kx) * 3 + o) * Kz + iz] = zc{re * inv, im * inv};
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_yfwd(const zc* __restrict__ H4, const double* __restrict__ Ay, const zc* __restrict__ base, zc* __restrict__ F,
+          int Kx, int Kz, int N, int My, int n, int R) {
+    extern __shared__ __align__(16) char lds[];
+    zc* g = reinterpret_cast<zc*>(lds);              // [My][3][MA]
+    const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x;
+    for (int q = tid; q < My * 3 * MA; q += blockDim.x) {
+        const int l = q % MA, r = q / MA, jj = r / 3, o = r - 3 * jj, iz = iz0 + l;
+        g[q] = iz < Kz ? H4[((size_t)(jj * Kx + kx) * 3 + o) * Kz + iz] : zc{0.0, 0.0};
+    }
+    __syncthreads();
+    const int l = tid % MA, iz = iz0 + l;
+    for (int p = tid / MA; p < 3 * N; p += blockDim.x / MA) {
+        const int e = p / N, nn = p - e * N;
+        double re = 0.0, im = 0.0;
+        for (int jj = 0; jj < My; ++jj) {
+            const double a = Ay[(size_t)jj * N + nn];
+            const zc v = g[(jj * 3 + e) * MA + l];
+            re = fma(a, v.re, re);
+            im = fma(a, v.im, im);
+        }
+        if (iz < Kz) {
+            const size_t at = (size_t)(kx * Kz + iz) * n + R + 4 * nn + e;
+            F[at] = zc{base[at].re + re, base[at].im + im};
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t cn_ch3d_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz) {
+    if (Nx < 2 || Nz < 2 || Ny < 2 || Mx < 1 || My < 1 || Mz < 1) return 0;
+    return ChDims{Nx / 2, Nz - 1, Ny, Mx, My, Mz}.bytes();
+}
+
+int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
+                     const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
+                     int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream) {
+    if (!X || !fidx || !base || !F || !ws || !By || !Wx || !Wz || !Ay) {
+        cn::set_error("cn_ch3d_explicit: null pointer");
+        return CN_EINVAL;
+    }
+    if (Nx < 2 || Nx % 2 || Nz < 2 || Nz % 2 || Ny < 2 || Mx < Nx || My < Ny || Mz < Nz || R < 0 ||
+        n < R + 4 * Ny) {
+        cn::set_error("cn_ch3d_explicit: inconsistent sizes");
+        return CN_EINVAL;
+    }
+    const ChDims d{Nx / 2, Nz - 1, Ny, Mx, My, Mz};
+    if (ws_bytes < d.bytes()) {
+        cn::set_error("cn_ch3d_explicit: workspace smaller than cn_ch3d_workspace_bytes");
+        return CN_EINVAL;
+    }
+    const size_t lds_a = (size_t)3 * d.N * MA * sizeof(zc);
+    const size_t lds_b = ((size_t)KB * 6 * d.Kz + Mz) * sizeof(zc);
+    const size_t lds_c = ((size_t)9 * d.Kx + Mx) * sizeof(zc) + (size_t)3 * Mx * sizeof(double);
+    const size_t lds_d = ((size_t)KB * 3 * Mz + Mz) * sizeof(zc);
+    const size_t lds_e = (size_t)My * 3 * MA * sizeof(zc);
+    const size_t cap = 160 * 1024;
+    if (lds_a > cap || lds_b > cap || lds_c > cap || lds_d > cap || lds_e > cap) {
+        cn::set_error("cn_ch3d_explicit: grid too large for the transform kernels (LDS)");
+        return CN_EUNSUPPORTED;
+    }
+    zc* r1 = static_cast<zc*>(ws);
+    zc* r2 = r1 + d.region1();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 gy((d.Kz + MA - 1) / MA, d.Kx), gz((d.Kx + KB - 1) / KB, My), gx(Mz, My);
+    hipLaunchKernelGGL(k_ch_yback, gy, dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, By, r1, d.Kx, d.Kz,
+                       d.N, My, n);
+    hipLaunchKernelGGL(k_ch_zback, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, My,
+                       Mz, kz0);
+    hipLaunchKernelGGL(k_ch_xline, gx, dim3(256), lds_c, s, r2, static_cast<const zc*>(Wx), r1, d.Kx, Mz, Mx, kx0);
+    hipLaunchKernelGGL(k_ch_zfwd, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, Mz);
+    hipLaunchKernelGGL(k_ch_yfwd, gy, dim3(256), lds_e, s, r2, Ay, static_cast<const zc*>(base), static_cast<zc*>(F),
+                       d.Kx, d.Kz, d.N, My, n, R);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        cn::set_error(std::string("cn_ch3d_explicit: ") + hipGetErrorString(e));
+        return CN_EHIP;
+    }
+    return CN_OK;
+}
+
+}  // extern "C"

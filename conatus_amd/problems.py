@@ -1,4 +1,5 @@
-"""Fourier x Chebyshev problems of the IMEX path: 2D Rayleigh-Benard convection.
+"""Fourier x Chebyshev problems of the IMEX path: 2D Rayleigh-Benard convection and the 3D
+plane channel (Channel3D below).
 
 Equations (python/conatus/problems/convection.py:24-63, tests/ivp/ivp_cases.py:59-93), x periodic
 of length Lx, z in [z0, z1] between rigid walls, Boussinesq in thermal-diffusion units:
@@ -176,6 +177,12 @@ class RayleighBenard2D:
         L.sum_duplicates()
         return M, L
 
+    def identity(self) -> dict:
+        """What a checkpoint of this problem must match on restart."""
+        return {"problem": "rayleigh_benard_2d", "Nx": int(self.Nx), "Nz": int(self.Nz), "Lx": float(self.Lx),
+                "z0": float(self.z0), "z1": float(self.z1), "dealias": float(self.dealias), "Ra": float(self.Ra),
+                "Pr": float(self.Pr)}
+
     def b0(self) -> np.ndarray:
         """(K, n) constant right-hand side of the boundary rows (T = +1/2 at z0, -1/2 at z1)."""
         b = np.zeros((self.K, self.n), complex)
@@ -215,3 +222,258 @@ class RayleighBenard2D:
         rng = np.random.default_rng(seed)
         T = 0.5 - (z[None, :] - self.z0) / self.Lz + amplitude * rng.standard_normal((x.size, z.size)) * (1 - t[None, :] ** 2)
         return self.state_from_fields(T=T)
+
+
+# ------------------------------------------------------------------------------------------------
+# 3D plane channel
+# ------------------------------------------------------------------------------------------------
+
+UX, UY, UZ = 0, 1, 2                       # velocity components at column 1 + 4 n + f (P = 3)
+CH_TAUS = ("tu2x", "tu2y", "tu2z", "tu1x", "tu1y", "tu1z", "tp")
+
+
+@dataclass
+class Channel3D:
+    """Incompressible Navier-Stokes between walls y = -1, 1, periodic in x (length Lx) and z
+    (length Lz), primitive variables (u, p), driven by a constant pressure gradient G e_x
+    (tests/channel3d/ch3d_cases.py:20-78, the reference's channel case with
+    ``pressure_gradient=G``):
+
+        div(u) + ey . lift(tu2, -1) + tp = 0
+        dt(u) - lap(u)/Re + grad(p) + lift(tu1, -1) + lift(tu2, -2) = -u . grad(u) + G ex
+        u(-1) = u(1) = 0, integ(p) = 0   (mode (0, 0) gauge: tp)
+
+    G = 2/Re makes the laminar Poiseuille flow u = (1 - y^2) ex a steady state.
+
+    Discretization: one complex almost-banded system per Fourier mode exp(i(kx kx0 x + kz kz0 z)),
+    kx = 0..Kx-1 (Kx = Nx/2) and kz = 0..Nz/2-1, -(Nz/2-1)..-1 (Kz = Nz - 1 values in FFT order,
+    the Nyquist modes not represented); mode index m = kx * Kz + iz.  The fields are real, so the
+    kx = 0 modes hold conjugate pairs (kz, -kz).  Within a mode the layout is the one of
+    RayleighBenard2D with the wall-normal velocity v in the slot of w and the spanwise velocity w
+    in the slot of T: unknowns and equation rows interleaved by Chebyshev index, equations in C^(2)
+    (momentum) and C^(1) (continuity), taus lifted into T_{N-1}, T_{N-2}, border rows w(+-1),
+    u(+-1), v(+-1) and the gauge, pinned lowest modes u0, u1, w0, w1, v0, p0 in the first columns.
+    Every mode has n = 4 Ny + 7 and r = 7.
+    """
+    Nx: int
+    Ny: int
+    Nz: int
+    Re: float
+    Lx: float = 4 * np.pi
+    Lz: float = 2 * np.pi
+    G: float | None = None
+    dealias: float = 1.5
+    mats: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        if self.Nx % 2 or self.Nz % 2:
+            raise ValueError("Nx and Nz must be even (the Nyquist modes are not represented)")
+        if self.G is None:
+            self.G = 2.0 / self.Re
+        self.Kx, self.Kz = self.Nx // 2, self.Nz - 1
+        self.K = self.Kx * self.Kz
+        self.N = self.Ny
+        self.kx0 = 2 * np.pi / self.Lx
+        self.kz0 = 2 * np.pi / self.Lz
+        self.Mx = S.dealias_size(self.Nx, self.dealias)
+        self.My = S.dealias_size(self.Ny, self.dealias)
+        self.Mz = S.dealias_size(self.Nz, self.dealias)
+        self.n = 4 * self.N + 7
+        self.border = list(range(NBORDER))
+        N = self.N
+        self.S01 = S.conversion(N, 0)
+        self.S12 = S.conversion(N, 1)
+        self.S02 = S.conversion_chain(N, 0, 2)
+        self.D1 = S.differentiation(N, 0, 1)
+        self.D2 = S.differentiation(N, 0, 2)
+        self._terms()
+        self.mats = [self.mode_matrices(m) for m in range(self.K)]
+
+    # -- modes and layout --------------------------------------------------------------------------
+    def kz_of(self, iz):
+        iz = np.asarray(iz)
+        return np.where(iz < self.Nz // 2, iz, iz - self.Kz)
+
+    def mode(self, kx: int, kz: int) -> int:
+        return int(kx * self.Kz + (kz if kz >= 0 else kz + self.Kz))
+
+    def wavenumbers(self, m: int) -> tuple[int, int]:
+        return int(m // self.Kz), int(self.kz_of(m % self.Kz))
+
+    col = RayleighBenard2D.col
+    row = RayleighBenard2D.row
+    field_index = RayleighBenard2D.field_index
+    equation_rows = RayleighBenard2D.equation_rows
+
+    def tau_col(self, name):
+        if name == "tp":
+            return 0
+        return 1 + 4 * self.N + CH_TAUS.index(name)
+
+    # -- operators ---------------------------------------------------------------------------------
+    def _csr(self, R, C, V):
+        A = sp.csr_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(C))), shape=(self.n, self.n))
+        A.sum_duplicates()
+        return A
+
+    def _put(self, R, C, V, block, e, f):
+        B = sp.coo_matrix(block)
+        R.append(self.row(B.row, e))
+        C.append(self.col(B.col, f))
+        V.append(B.data.astype(complex))
+
+    def _put_col(self, R, C, V, vec, e, col):
+        nz = np.flatnonzero(vec)
+        R.append(self.row(nz, e))
+        C.append(np.full(nz.size, col))
+        V.append(np.asarray(vec[nz], dtype=complex))
+
+    def _terms(self):
+        """The mode-independent parts of L and the coefficients of kx^2 + kz^2, i kx and i kz."""
+        N = self.N
+        S02, S01, ReInv = self.S02, self.S01, 1.0 / self.Re
+        R, C, V = [], [], []
+        for e, f in ((XMOM, UX), (ZMOM, UY), (TEQ, UZ)):
+            self._put(R, C, V, S02, e, f)
+        self.M = self._csr(R, C, V)
+        # L0: -D2/Re, the wall-normal pressure gradient, continuity dv/dy, lifts, wall rows
+        R, C, V = [], [], []
+        for e, f in ((XMOM, UX), (ZMOM, UY), (TEQ, UZ)):
+            self._put(R, C, V, -ReInv * self.D2, e, f)
+        self._put(R, C, V, (self.S12 @ self.D1).tocsr(), ZMOM, P)
+        self._put(R, C, V, self.D1, CONT, UY)
+        S02d, S01d = S02.toarray(), S01.toarray()
+        for e, c in ((XMOM, "x"), (ZMOM, "y"), (TEQ, "z")):
+            self._put_col(R, C, V, S02d[:, N - 1], e, self.tau_col("tu1" + c))
+            self._put_col(R, C, V, S02d[:, N - 2], e, self.tau_col("tu2" + c))
+        self._put_col(R, C, V, S01d[:, N - 1], CONT, self.tau_col("tu2y"))
+        left, right = (-1.0) ** np.arange(N), np.ones(N)
+        for r, (vals, f) in enumerate(((left, UZ), (right, UZ), (left, UX), (right, UX), (left, UY), (right, UY))):
+            R.append(np.full(N, r))
+            C.append(self.col(np.arange(N), f))
+            V.append(vals.astype(complex))
+        self.L0 = self._csr(R, C, V)
+        # gauge row: integ(p) = 0 and the tp column of continuity at (0, 0); tp = 0 elsewhere
+        R, C, V = [], [], []
+        w = S.integration_weights(N, 2.0)
+        nz = np.flatnonzero(w)
+        R.append(np.full(nz.size, 6))
+        C.append(self.col(nz, P))
+        V.append(w[nz].astype(complex))
+        self._put_col(R, C, V, S01d[:, 0], CONT, self.tau_col("tp"))
+        self.Lg0 = self._csr(R, C, V)
+        self.Lg = self._csr([np.array([6])], [np.array([self.tau_col("tp")])], [np.array([1.0 + 0j])])
+        # k^2/Re S02 on the momentum rows; i kx: x pressure gradient + du/dx; i kz: z gradient + dw/dz
+        R, C, V = [], [], []
+        for e, f in ((XMOM, UX), (ZMOM, UY), (TEQ, UZ)):
+            self._put(R, C, V, ReInv * S02, e, f)
+        self.Lk2 = self._csr(R, C, V)
+        R, C, V = [], [], []
+        self._put(R, C, V, S02, XMOM, P)
+        self._put(R, C, V, S01, CONT, UX)
+        self.Lkx = self._csr(R, C, V)
+        R, C, V = [], [], []
+        self._put(R, C, V, S02, TEQ, P)
+        self._put(R, C, V, S01, CONT, UZ)
+        self.Lkz = self._csr(R, C, V)
+
+    def mode_matrices(self, m: int):
+        """(M, L) of mode m (complex CSR of size n; the border rows are rows 0..6):
+        L = L0 + (kx^2 + kz^2)/Re S02 + i kx (S02 p_x-row, S01 u-continuity) + i kz (...)."""
+        kx, kz = self.wavenumbers(m)
+        ax, az = kx * self.kx0, kz * self.kz0
+        L = self.L0 + (self.Lg0 if (kx, kz) == (0, 0) else self.Lg)
+        if kx or kz:
+            L = L + (ax * ax + az * az) * self.Lk2
+        if kx:
+            L = L + (1j * ax) * self.Lkx
+        if kz:
+            L = L + (1j * az) * self.Lkz
+        L = L.tocsr()
+        L.sum_duplicates()
+        L.eliminate_zeros()
+        return self.M, L
+
+    def b0(self) -> np.ndarray:
+        """(K, n) constant part of F: the driving pressure gradient G (T_0 coefficient of the x
+        momentum, converted to C^(2)) at mode (0, 0); the wall rows are homogeneous."""
+        b = np.zeros((self.K, self.n), complex)
+        b[0, self.equation_rows(XMOM)] = self.G * self.S02.toarray()[:, 0]
+        return b
+
+    def identity(self) -> dict:
+        """What a checkpoint of this problem must match on restart."""
+        return {"problem": "channel_3d", "Nx": int(self.Nx), "Ny": int(self.Ny), "Nz": int(self.Nz),
+                "Lx": float(self.Lx), "Lz": float(self.Lz), "dealias": float(self.dealias), "Re": float(self.Re),
+                "G": float(self.G)}
+
+    # -- states ------------------------------------------------------------------------------------
+    def velocity(self, X: np.ndarray) -> np.ndarray:
+        """(3, Kx, Kz, N) Chebyshev coefficients of u, v, w from a (K, n) state."""
+        return np.stack([X[:, self.field_index(f)].reshape(self.Kx, self.Kz, self.N) for f in (UX, UY, UZ)])
+
+    def state_from_velocity(self, V: np.ndarray) -> np.ndarray:
+        X = np.zeros((self.K, self.n), complex)
+        for f in (UX, UY, UZ):
+            X[:, self.field_index(f)] = V[f].reshape(self.K, self.N)
+        return X
+
+    def poiseuille(self) -> np.ndarray:
+        """(K, n) laminar state u = (G Re / 2)(1 - y^2) ex (p = 0, taus 0)."""
+        V = np.zeros((3, self.Kx, self.Kz, self.N), complex)
+        a = self.G * self.Re / 2
+        V[UX, 0, 0, 0] = a / 2                     # 1 - y^2 = T_0/2 - T_2/2
+        V[UX, 0, 0, 2] = -a / 2
+        return self.state_from_velocity(V)
+
+    def initial_state(self, seed: int = 0, energy: float = 1e-4, kmax: int = 4) -> np.ndarray:
+        """Laminar Poiseuille flow plus a seeded divergence-free perturbation u' = curl(psi),
+        psi = (1 - y^2)^2 times random complex amplitudes on the modes |kx|, |kz| <= kmax
+        ((0, 0) excluded; conjugate pairs at kx = 0), scaled to the volume-averaged kinetic
+        energy ``energy`` (1/2 <|u'|^2>).  (1 - y^2)^2 vanishes with its y-derivative at the
+        walls, so u' satisfies no-slip; the curl is taken on the coefficients, so div u' = 0."""
+        rng = np.random.default_rng(seed)
+        Kx, Kz, N = self.Kx, self.Kz, self.N
+        prof = np.zeros(N)
+        prof[:5] = np.array([3.0, 0.0, -4.0, 0.0, 1.0]) / 8.0          # (1 - y^2)^2 in T_n
+        psi = np.zeros((3, Kx, Kz, N), complex)
+        for kx in range(min(kmax, Kx - 1) + 1):
+            for kz in range(-min(kmax, self.Nz // 2 - 1), min(kmax, self.Nz // 2 - 1) + 1):
+                if (kx, kz) == (0, 0) or (kx == 0 and kz < 0):
+                    continue
+                a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+                psi[:, kx, self.mode(kx, kz) % Kz] = a[:, None] * prof
+                if kx == 0:
+                    psi[:, 0, self.mode(0, -kz) % Kz] = np.conj(a)[:, None] * prof
+        ikx = (1j * self.kx0 * np.arange(Kx))[:, None, None]
+        ikz = (1j * self.kz0 * self.kz_of(np.arange(Kz)))[None, :, None]
+        dy = lambda c: _cheb_diff_last(c)                              # noqa: E731
+        up = np.stack([dy(psi[2]) - ikz * psi[1], ikz * psi[0] - ikx * psi[2], ikx * psi[1] - dy(psi[0])])
+        e = self.kinetic_energy(up)
+        if e > 0:
+            up *= np.sqrt(energy / e)
+        return self.poiseuille() + self.state_from_velocity(up)
+
+    def kinetic_energy(self, V: np.ndarray) -> float:
+        """1/2 volume average of |u|^2 from (3, Kx, Kz, N) coefficients (Parseval in x, z,
+        Gauss-Chebyshev quadrature of the exact polynomial product in y)."""
+        My = 2 * self.N
+        B = S.cheb_backward_matrix(self.N, My)
+        vals = np.einsum("jn,fxzn->fxzj", B, V)
+        w = np.full(self.Kx, 2.0)
+        w[0] = 1.0
+        t = S.cheb_nodes(My)
+        q = np.pi / My * np.sqrt(1 - t * t)                             # Gauss-Chebyshev weights of dy
+        return 0.5 * float(np.einsum("x,fxzj,j->", w, np.abs(vals) ** 2, q)) / 2.0
+
+
+def _cheb_diff_last(c: np.ndarray) -> np.ndarray:
+    """d/dy on [-1, 1] of Chebyshev series along the last axis (backward recurrence)."""
+    N = c.shape[-1]
+    out = np.zeros_like(c)
+    if N > 1:
+        out[..., N - 2] = 2 * (N - 1) * c[..., N - 1]
+    for k in range(N - 3, -1, -1):
+        out[..., k] = out[..., k + 2] + 2 * (k + 1) * c[..., k + 1]
+    out[..., 0] *= 0.5
+    return out

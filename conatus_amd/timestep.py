@@ -15,10 +15,10 @@ import numpy as np
 
 from . import _lib
 from . import spectral as S
-from .problems import NBORDER, RayleighBenard2D
+from .problems import NBORDER, Channel3D, RayleighBenard2D
 from .runtime.imex import Imex
 
-__all__ = ["RB2DExplicit", "IVPSolver"]
+__all__ = ["RB2DExplicit", "Channel3DExplicit", "IVPSolver"]
 
 
 def _torch():
@@ -60,14 +60,58 @@ class RB2DExplicit:
         _lib.check(st, "RB2DExplicit")
 
 
-class IVPSolver:
-    """Device IVP solver of a Fourier x Chebyshev problem (2D Rayleigh-Benard)."""
+class Channel3DExplicit:
+    """F(X) of the 3D plane channel on the device (cn_ch3d_explicit: five transform launches,
+    one workspace).  The constant part (the driving pressure gradient) is added by the last
+    launch; the other rows of F keep base()."""
 
-    def __init__(self, problem: RayleighBenard2D, scheme: str, dt: float, pivot: bool = True,
-                 mode: str | None = None):
+    def __init__(self, pb: Channel3D):
+        torch = _torch()
+        self.pb = pb
+        N, Mx, My, Mz = pb.N, pb.Mx, pb.My, pb.Mz
+        dev = "cuda"
+        By = np.stack([S.cheb_backward_matrix(N, My, 0), S.cheb_backward_matrix(N, My, 1, 2.0)], axis=-1)
+        self.By = torch.from_numpy(np.ascontiguousarray(By)).to(dev)                       # (My, N, 2)
+        self.Wx = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mx) / Mx)).to(dev)
+        self.Wz = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mz) / Mz)).to(dev)
+        Ay = (pb.S02 @ S.cheb_forward_matrix(N, My)).T
+        self.Ay = torch.from_numpy(np.ascontiguousarray(Ay)).to(dev)                       # (My, N)
+        self._base = torch.from_numpy(pb.b0().ravel()).to(dev)
+        fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
+        self.fidx = torch.from_numpy(np.ascontiguousarray(fidx)).to(dev)
+        nbytes = _lib.lib().cn_ch3d_workspace_bytes(pb.Nx, pb.Ny, pb.Nz, Mx, My, Mz)
+        self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+
+    def base(self):
+        return self._base
+
+    def write(self, x, out) -> None:
+        pb = self.pb
+        v = ctypes.c_void_p
+        p = lambda t: v(t.data_ptr())   # noqa: E731
+        st = _lib.lib().cn_ch3d_explicit(p(x), p(self.fidx), p(self._base), p(out), p(self.ws), self.ws.numel(),
+                                         p(self.By), p(self.Wx), p(self.Wz), p(self.Ay), pb.Nx, pb.Ny, pb.Nz,
+                                         pb.Mx, pb.My, pb.Mz, pb.n, NBORDER, float(pb.kx0), float(pb.kz0),
+                                         v(_torch().cuda.current_stream().cuda_stream))
+        _lib.check(st, "Channel3DExplicit")
+
+
+def explicit_term(problem):
+    """The device explicit term of a problem."""
+    if isinstance(problem, Channel3D):
+        return Channel3DExplicit(problem)
+    if isinstance(problem, RayleighBenard2D):
+        return RB2DExplicit(problem)
+    raise TypeError(f"no device explicit term for {type(problem).__name__}")
+
+
+class IVPSolver:
+    """Device IVP solver of a Fourier x Chebyshev problem (2D Rayleigh-Benard, 3D channel)."""
+
+    def __init__(self, problem, scheme: str, dt: float, pivot: bool = True, mode: str | None = None):
         torch = _torch()
         self.problem = problem
-        self.explicit = RB2DExplicit(problem)
+        self.explicit = explicit_term(problem)
         Ms = [m[0] for m in problem.mats]
         Ls = [m[1] for m in problem.mats]
         self.imex = Imex(Ms, Ls, problem.border, scheme, dt, pivot=pivot, dtype=np.complex128,
@@ -114,6 +158,10 @@ class IVPSolver:
 
     def explain(self) -> str:
         pb = self.problem
+        if isinstance(pb, Channel3D):
+            return (f"IVPSolver(channel 3D {pb.Nx}x{pb.Ny}x{pb.Nz}, Re={pb.Re:g}, {self.scheme}, dt={self.dt:g}): "
+                    f"{pb.K} complex modes of n={pb.n} in one batched solve; F on the {pb.Mx}x{pb.My}x{pb.Mz} grid; "
+                    + self.imex.explain())
         return (f"IVPSolver(RB 2D {pb.Nx}x{pb.Nz}, Ra={pb.Ra:g}, Pr={pb.Pr:g}, {self.scheme}, dt={self.dt:g}): "
                 f"{pb.K} complex modes of n={pb.n} in one batched solve; F on the {pb.Mx}x{pb.Mz} grid; "
                 + self.imex.explain())

@@ -189,6 +189,22 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
                           const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, int members,
                           void* stream);
 
+/* ---- Fourier x Chebyshev x Fourier explicit term (3D plane channel) -------------------------
+ * F rows of the x, y and z momentum equations from the state X (Kx*Kz modes x n, complex;
+ * Kx = Nx/2, Kz = Nz-1, mode kx*Kz + iz with kz = iz for iz < Nz/2 else iz - Kz):
+ * F = base + S_{0->2} forward(-u.grad u) on the Mx x My x Mz grid.  Replaces the reference's 3D
+ * explicit evaluation (python/conatus/timestep/fused.py:994-1085 with the 3D transforms of
+ * python/conatus/timestep/xp_transforms.py:197-548).  Layout: coefficient m of velocity
+ * component f (u, v, w) of mode k at X[k*n + fidx[f*Ny + m]]; equation e rows at
+ * F[k*n + R + 4*m + e] (base: the constant part, same layout as F); other rows of F are left
+ * untouched.  Tables: By (My x Ny x 2: T_n(y_j), T_n'(y_j)), Wx, Wz (Mx, Mz complex twiddles
+ * exp(2 pi i q / M)), Ay (My x Ny: (S02 Fy)^T); kx0 = 2 pi / Lx, kz0 = 2 pi / Lz.  Workspace:
+ * ws of at least cn_ch3d_workspace_bytes(...) device bytes.  Five launches on `stream`. */
+size_t cn_ch3d_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz);
+int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
+                     const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
+                     int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

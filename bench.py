@@ -326,39 +326,55 @@ def ensemble_leg(runs: int, steps: int, dt: float) -> dict:
             "tuning": es.tuning}
 
 
-def served_leg(nreq: int, dt: float) -> dict:
-    """Served stream: Poisson arrivals (seeded, mean 50 requests/s) of RB 2D runs of the 128 x 64 leg's
-    physics (Ra = 1e5, Pr = 1) with Ny drawn from {32, 48, 64, 96, 128}, Nx = 2 Ny, seeded noise
-    initial states, 50-500 SBDF2 steps each, admitted by
-    HBM footprint up to 200 GB and continuously batched (conatus_amd.serving).  Reports p50/p99 of
-    the per-step latency every run saw and the aggregate run-steps/s, beside per-run sequential
-    GPU stepping of the same requests (single-run step times per size class times steps)."""
-    from conatus_amd.problems import RayleighBenard2D
+def served_leg(nreq: int, dt: float, channel_share: float = 0.1) -> dict:
+    """Served stream: Poisson arrivals (seeded, mean 50 requests/s) mixing RB 2D runs of the
+    128 x 64 leg's physics (Ra = 1e5, Pr = 1) with Ny drawn from {32, 48, 64, 96, 128}, Nx = 2 Ny,
+    seeded noise initial states, and (a seeded tenth of the requests) 3D channel runs at
+    64 x 65 x 64 (Re = 2000, Lx = 4 pi, Lz = 2 pi, Poiseuille + perturbation of energy 1e-4);
+    50-500 SBDF2 steps each, admitted by HBM footprint up to 200 GB and continuously batched
+    (conatus_amd.serving).  Reports p50/p99 of the per-step latency every run saw and the
+    aggregate run-steps/s, beside per-run sequential GPU stepping of the same requests
+    (single-run step times per size class times steps)."""
+    from conatus_amd.problems import Channel3D, RayleighBenard2D
     from conatus_amd.serving import Request, Server
     rng = np.random.default_rng(4)
     gaps = rng.exponential(1 / 50.0, size=nreq)
     arrivals = np.cumsum(gaps) - gaps[0]
     nys = rng.choice([32, 48, 64, 96, 128], size=nreq)
     steps = rng.integers(50, 501, size=nreq)
+    is_channel = rng.random(nreq) < channel_share
+
+    def problem(cls):
+        if cls == "channel":
+            return Channel3D(Nx=64, Ny=65, Nz=64, Re=2000.0, Lx=4 * np.pi, Lz=2 * np.pi)
+        return RayleighBenard2D(Nx=2 * cls, Nz=cls, Lx=4.0, Ra=1e5, Pr=1.0)
+
+    classes = ["channel" if is_channel[i] else int(nys[i]) for i in range(nreq)]
+    templates = {c: problem(c) for c in sorted(set(classes), key=str)}
     reqs = []
     for i in range(nreq):
-        pb = RayleighBenard2D(Nx=2 * int(nys[i]), Nz=int(nys[i]), Lx=4.0, Ra=1e5, Pr=1.0)
+        pb = templates[classes[i]]
         reqs.append(Request(pb, int(steps[i]), pb.initial_state(seed=2000 + i), arrival=float(arrivals[i])))
-    srv = Server("SBDF2", dt, hbm_budget=200e9, page_size=8)
+    srv = Server("SBDF2", dt, hbm_budget=200e9, page_size=8, channel_page_size=2)
     # every size class tuned, its pages and the single-run factors built before the clock starts
-    srv.prepare([RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0) for ny in sorted(set(int(v) for v in nys))],
-                pages=3)
+    rb = [pb for c, pb in templates.items() if c != "channel"]
+    srv.prepare(rb, pages=3)
+    if "channel" in templates:
+        srv.prepare([templates["channel"]], pages=8, startup_pages=3)
+    built = srv.pages_built
     quiet_heap()
     st = srv.serve(reqs, clock="wall")
     bad = sum(not np.all(np.isfinite(q.result)) for q in reqs)
     if bad:
         raise RuntimeError(f"served leg: {bad} results not finite")
-    per_class = {ny: single_run_ms(RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0), dt)
-                 for ny in sorted(set(int(v) for v in nys))}
-    seq_s = sum(per_class[int(nys[i])] * int(steps[i]) for i in range(nreq)) * 1e-3
-    st.update({"workload": f"{nreq} requests, Poisson arrivals mean 50/s (seeded), RB 2D Ra = 1e5, Pr = 1, "
-                           f"Ny in {{32..128}}, Nx = 2 Ny, 50-500 SBDF2 steps each, dt = {dt:g}, HBM budget 200 GB",
+    per_class = {c: single_run_ms(pb, dt) for c, pb in templates.items()}
+    seq_s = sum(per_class[classes[i]] * int(steps[i]) for i in range(nreq)) * 1e-3
+    st.update({"workload": f"{nreq} requests, Poisson arrivals mean 50/s (seeded): RB 2D Ra = 1e5, Pr = 1, "
+                           f"Ny in {{32..128}}, Nx = 2 Ny, and {int(is_channel.sum())} 3D channel runs 64x65x64 "
+                           f"(Re = 2000); 50-500 SBDF2 steps each, dt = {dt:g}, HBM budget 200 GB",
                "arrival_span_s": float(arrivals[-1]),
+               "channel_requests": int(is_channel.sum()),
+               "pages_built_while_serving": srv.pages_built - built,
                "sequential_single_run_steps_per_s": st["steps"] / seq_s,
                "speedup_vs_sequential_single_runs": (st["steps"] / st["wall_s"]) / (st["steps"] / seq_s),
                "single_run_ms_per_step": {str(k): v for k, v in per_class.items()}})

@@ -26,7 +26,7 @@ from .errors import OptionError
 from .problems import RayleighBenard2D
 from .runtime.banded import BatchHandle, csr_args
 from .runtime.imex import Imex, capture_graph, sbdf_coefficients
-from .timestep import IVPSolver, RB2DExplicit
+from .timestep import IVPSolver, RB2DExplicit, explicit_term
 
 __all__ = ["Ensemble", "EnsembleSet", "autotune_mode", "discretization_key"]
 
@@ -36,17 +36,18 @@ def _torch():
     return torch
 
 
-def discretization_key(pb: RayleighBenard2D) -> tuple:
-    """What members of one Ensemble must share: the grid and the dealiasing (not Ra, Pr)."""
-    return (pb.Nx, pb.Nz, float(pb.Lx), float(pb.z0), float(pb.z1), float(pb.dealias))
+def discretization_key(pb) -> tuple:
+    """What members of one Ensemble must share: the grid and the dealiasing (not the parameters
+    the implicit operators depend on: Ra, Pr of RB 2D, Re of the channel)."""
+    return pb.discretization_key()
 
 
 class _LoopExplicit:
     """F member by member (the reference's rhs='loop' path): one single-member launch pair per
     member on its slice of the stacked state; the same kernels as the batched path."""
 
-    def __init__(self, pb: RayleighBenard2D, members: int):
-        self.one = RB2DExplicit(pb, 1)
+    def __init__(self, pb, members: int):
+        self.one = explicit_term(pb)
         self.members = members
         self.size = pb.K * pb.n
         self._base = _torch().cat([self.one.base()] * members)
@@ -126,6 +127,8 @@ class Ensemble:
         self.problems = problems
         self.B, self.K, self.n = len(problems), p0.K, p0.n
         self.scheme, self.dt, self.pivot, self.mode = scheme, float(dt), pivot, mode
+        if not isinstance(p0, RayleighBenard2D):
+            rhs = "loop"                  # the channel's explicit term has no member dimension
         self.rhs_mode = rhs
         self.explicit = RB2DExplicit(p0, self.B) if rhs == "batched" else _LoopExplicit(p0, self.B)
         Ms = [M for pb in problems for (M, L) in pb.mats]
@@ -184,8 +187,8 @@ class Ensemble:
 
     def explain(self) -> str:
         pb = self.problems[0]
-        ras = sorted({p.Ra for p in self.problems})
-        return (f"Ensemble({self.B} member(s) of RB 2D {pb.Nx}x{pb.Nz}, Ra in [{ras[0]:g}, {ras[-1]:g}], "
+        ps = sorted({p.params() for p in self.problems})
+        return (f"Ensemble({self.B} member(s) of {pb.describe()}, parameters {ps[0]}..{ps[-1]}, "
                 f"{self.scheme}, dt={self.dt:g}, rhs={self.rhs_mode}): {self.B * self.K} mode systems of "
                 f"n={self.n} in one batched solve per stage; F in "
                 + ("one launch pair for all members" if self.rhs_mode == "batched" else "one launch pair per member")

@@ -177,6 +177,22 @@ class RayleighBenard2D:
         L.sum_duplicates()
         return M, L
 
+    kind = "rb2d"
+
+    def discretization_key(self) -> tuple:
+        """What members of one ensemble must share: the grid and the dealiasing (not Ra, Pr)."""
+        return (self.Nx, self.Nz, float(self.Lx), float(self.z0), float(self.z1), float(self.dealias))
+
+    def params(self) -> tuple:
+        """The parameters the implicit operators depend on."""
+        return (float(self.Ra), float(self.Pr))
+
+    def describe(self) -> str:
+        return f"RB 2D {self.Nx}x{self.Nz}, Ra={self.Ra:g}"
+
+    def explicit_scratch_bytes(self) -> int:
+        return 3 * self.K * self.Mz * 16
+
     def identity(self) -> dict:
         """What a checkpoint of this problem must match on restart."""
         return {"problem": "rayleigh_benard_2d", "Nx": int(self.Nx), "Nz": int(self.Nz), "Lx": float(self.Lx),
@@ -400,6 +416,28 @@ class Channel3D:
         b = np.zeros((self.K, self.n), complex)
         b[0, self.equation_rows(XMOM)] = self.G * self.S02.toarray()[:, 0]
         return b
+
+    kind = "channel3d"
+
+    def discretization_key(self) -> tuple:
+        """What members of one ensemble must share: the grid, the dealiasing and the driving
+        pressure gradient (the constant part of F); Re may differ."""
+        return ("channel3d", self.Nx, self.Ny, self.Nz, float(self.Lx), float(self.Lz), float(self.dealias),
+                float(self.G))
+
+    def params(self) -> tuple:
+        """The parameters the implicit operators depend on."""
+        return (float(self.Re),)
+
+    def describe(self) -> str:
+        return f"channel 3D {self.Nx}x{self.Ny}x{self.Nz}, Re={self.Re:g}"
+
+    def explicit_scratch_bytes(self) -> int:
+        """Workspace of the explicit term (the intermediate spectra between its launches)."""
+        Kx, Kz, Mx, My, Mz = self.Kx, self.Kz, self.Mx, self.My, self.Mz
+        r1 = max(Kx * My * 6 * Kz, My * Mz * 3 * Kx)
+        r2 = max(My * Mz * 9 * Kx, My * Kx * 3 * Kz)
+        return 16 * (r1 + r2)
 
     def identity(self) -> dict:
         """What a checkpoint of this problem must match on restart."""

@@ -1,11 +1,13 @@
-"""Serving of IVP runs on one MI355X: continuous batching of 2D Rayleigh-Benard requests.
+"""Serving of IVP runs on one MI355X: continuous batching of 2D Rayleigh-Benard and 3D channel
+requests.
 
 Requests (a problem, an initial state, a number of steps) arrive at any time.  The server keeps
 every admitted run's state, SBDF history and factored mode systems resident in HBM and advances
 all of them in ticks; one tick is one step of every active run.
 
-* Pages: every size class (grid) holds pages, resident ``Ensemble``s of ``page_size`` slots built
-  once (explicit-term tables, CSR, factors, captured graphs).  A stepping page runs at full order
+* Pages: every size class (grid) holds pages, resident ``Ensemble``s of ``page_size`` slots
+  (``channel_page_size`` for channels, whose steps are larger) built once (explicit-term tables,
+  CSR, factors, captured graphs).  A stepping page runs at full order
   and is replayed from its graphs (one batched explicit term, stage and solve for its slots); a
   startup page makes the lower-order startup steps of the runs placed in it.  Only pages that
   hold runs are stepped, each on its own stream, so that the pages' kernels overlap.  The kernel
@@ -15,7 +17,8 @@ all of them in ticks; one tick is one step of every active run.
   startup page of their class (state copied in).
 * Factors: a slot holds the factored operators of its run (one per SBDF order it steps at),
   copied from the run's single-run handles (``cn_bb_copy_systems``), which are built once per
-  (grid, Ra, Pr, order) and cached; a slot that already holds the same Ra, Pr copies nothing.
+  (grid, parameters, order) and cached; a slot that already holds the same parameters (Ra, Pr of
+  RB 2D, Re of a channel) copies nothing.
 * Joining: a run whose startup is complete moves into a free slot of a stepping page of its class
   (the fullest first) by device copies only: its state and its SBDF history levels (into the
   ring slots the page's next step reads).  From then on it shares the page's batched step.
@@ -50,7 +53,7 @@ def _torch():
 
 @dataclass
 class Request:
-    problem: RayleighBenard2D
+    problem: object                       # RayleighBenard2D or Channel3D
     steps: int
     X0: np.ndarray
     arrival: float = 0.0                  # seconds (or ticks) after the start of the serving loop
@@ -62,19 +65,26 @@ class Request:
     step_latencies: list = field(default_factory=list, repr=False)
 
 
-def footprint_bytes(pb: RayleighBenard2D, order: int = 2) -> int:
+def footprint_bytes(pb, order: int = 2) -> int:
     """Estimated HBM bytes of one resident run: state, SBDF history (M X and F per level), the
     explicit-term scratch and the factor storage of its K mode systems (band rows, U, L, W, V^T,
-    pivots; the largest band a RB 2D mode system has: kl = 12, ku = 18, r = 7)."""
+    pivots; the band of the RB 2D and channel mode systems: kl = 12, ku = 18, r = 7)."""
     K, n = pb.K, pb.n
     vec = K * n * 16
     kl, ku, r = 12, 18, 7
     factor = K * n * 16 * ((kl + ku + 1) + (kl + ku + 1) + kl + 2 * r) + K * n * 4
-    return vec * (1 + 2 * order + 3) + factor
+    return vec * (1 + 2 * order + 2) + pb.explicit_scratch_bytes() + factor
 
 
-def _params(pb: RayleighBenard2D) -> tuple:
-    return (float(pb.Ra), float(pb.Pr))
+def _params(pb) -> tuple:
+    return pb.params()
+
+
+def _grid(key: tuple) -> str:
+    """'NxxNz' of a RB 2D discretization key, 'channel NxxNyxNz' of a channel's."""
+    if key and key[0] == "channel3d":
+        return f"channel {key[1]}x{key[2]}x{key[3]}"
+    return f"{key[0]}x{key[1]}"
 
 
 def _alpha(order: int, dt: float) -> float:
@@ -89,7 +99,7 @@ class _Page:
     runs placed in it, eagerly.  ``facs`` maps an SBDF order to the page's factored operator of
     that order; ``slot_params[q][s]`` is the (Ra, Pr) whose factors slot s holds for order q."""
 
-    def __init__(self, template: RayleighBenard2D, P: int, scheme: str, dt: float, mode: str | None,
+    def __init__(self, template, P: int, scheme: str, dt: float, mode: str | None,
                  startup: bool):
         torch = _torch()
         self.P, self.startup, self.dt = P, startup, dt
@@ -128,16 +138,19 @@ class _Page:
 
 
 class Server:
-    """Continuous-batching server of 2D Rayleigh-Benard IVP requests (module doc)."""
+    """Continuous-batching server of 2D Rayleigh-Benard and 3D channel IVP requests (module doc)."""
 
     def __init__(self, scheme: str = "SBDF2", dt: float = 1e-4, hbm_budget: float = 200e9,
-                 autotune: bool = True, mode: str | None = None, page_size: int = 8):
+                 autotune: bool = True, mode: str | None = None, page_size: int = 8,
+                 channel_page_size: int = 2):
         if autotune and mode is not None:
             raise OptionError("Server: autotune and a fixed mode exclude each other")
         if scheme not in ("SBDF1", "SBDF2", "SBDF3", "SBDF4"):
             raise OptionError(f"Server: scheme {scheme!r} not served", fix="one of SBDF1..SBDF4")
-        if int(page_size) != page_size or page_size < 1:
-            raise OptionError("Server: page_size must be a whole number >= 1")
+        for name, v in (("page_size", page_size), ("channel_page_size", channel_page_size)):
+            if int(v) != v or v < 1:
+                raise OptionError(f"Server: {name} must be a whole number >= 1")
+        self.channel_page_size = int(channel_page_size)
         self.scheme, self.dt = scheme, float(dt)
         self.budget = float(hbm_budget)
         self.autotune, self.mode = autotune, mode
@@ -168,7 +181,7 @@ class Server:
         self.waiting.append(req)
         return req
 
-    def _mode_for(self, pb: RayleighBenard2D) -> str | None:
+    def _mode_for(self, pb) -> str | None:
         if not self.autotune:
             return self.mode
         key = discretization_key(pb)
@@ -176,9 +189,9 @@ class Server:
             self._modes[key] = autotune_mode([pb], self.scheme, self.dt)[0]
         return self._modes[key]
 
-    def _runfac(self, pb: RayleighBenard2D, q: int):
-        """Factored operator of one run's K mode systems for SBDF order q, cached per (grid, Ra,
-        Pr, q): the source of the factors copied into slots."""
+    def _runfac(self, pb, q: int):
+        """Factored operator of one run's K mode systems for SBDF order q, cached per (grid,
+        parameters, q): the source of the factors copied into slots."""
         key = (discretization_key(pb), _params(pb), q)
         f = self._runfacs.get(key)
         if f is None:
@@ -189,23 +202,25 @@ class Server:
             self._runfacs[key] = f
         return f
 
-    def _pages(self, pb: RayleighBenard2D, startup: bool) -> list:
+    def _pages(self, pb, startup: bool) -> list:
         return self.pages.setdefault((discretization_key(pb), startup), [])
 
-    def _new_page(self, pb: RayleighBenard2D, startup: bool) -> _Page:
-        pg = _Page(pb, self.page_size, self.scheme, self.dt, self._mode_for(pb), startup)
+    def _new_page(self, pb, startup: bool) -> _Page:
+        P = self.page_size if isinstance(pb, RayleighBenard2D) else self.channel_page_size
+        pg = _Page(pb, P, self.scheme, self.dt, self._mode_for(pb), startup)
         self._pages(pb, startup).append(pg)
         self.pages_built += 1
         return pg
 
-    def prepare(self, problems, pages: int = 2) -> None:
+    def prepare(self, problems, pages: int = 2, startup_pages: int = 1) -> None:
         """Tune the size classes of ``problems`` and build their pages (``pages`` stepping pages
-        and one startup page each) and the single-run factors of their parameters, ahead of the
-        requests."""
+        and ``startup_pages`` startup pages each) and the single-run factors of their parameters,
+        ahead of the requests (a page built while serving costs its tick the setup: CSR
+        operators, factorization and graph capture of all its slots)."""
         for pb in problems:
             while len(self._pages(pb, False)) < pages:
                 self._new_page(pb, False)
-            if self._order > 1 and not self._pages(pb, True):
+            while self._order > 1 and len(self._pages(pb, True)) < startup_pages:
                 self._new_page(pb, True)
             for q in range(1, self._order + 1):
                 self._runfac(pb, q)
@@ -348,5 +363,5 @@ class Server:
                 "p50_step_ms": float(np.percentile(lat, 50)) if steps else float("nan"),
                 "p99_step_ms": float(np.percentile(lat, 99)) if steps else float("nan"),
                 "ticks": len(self.ticks), "joins": self.joins, "pages_built": self.pages_built,
-                "pages": {f"{k[0][0]}x{k[0][1]}{' startup' if k[1] else ''}": len(v) for k, v in self.pages.items()},
-                "modes": {f"{k[0]}x{k[1]}": v for k, v in self._modes.items()}}
+                "pages": {f"{_grid(k[0])}{' startup' if k[1] else ''}": len(v) for k, v in self.pages.items()},
+                "modes": {_grid(k): v for k, v in self._modes.items()}}

@@ -37,3 +37,17 @@ def test_server_refuses_bad_requests():
         Server("SBDF2", 1e-3, hbm_budget=10.0, autotune=False).submit(Request(pb, 3, pb.initial_state()))
     with pytest.raises(OptionError):
         Server(autotune=True, mode="wave")
+
+
+def test_channel_keys_parameters_and_footprint():
+    from conatus_amd.ensembles import discretization_key
+    from conatus_amd.problems import Channel3D
+    from conatus_amd.serving import footprint_bytes
+    a = Channel3D(Nx=8, Ny=9, Nz=8, Re=500.0, G=1e-3)
+    b = Channel3D(Nx=8, Ny=9, Nz=8, Re=900.0, G=1e-3)
+    c = Channel3D(Nx=8, Ny=9, Nz=8, Re=900.0)
+    assert discretization_key(a) == discretization_key(b) != discretization_key(c)   # G is part of F
+    assert a.params() != b.params()
+    big = Channel3D(Nx=16, Ny=17, Nz=16, Re=500.0)
+    assert footprint_bytes(big) > 4 * footprint_bytes(a) > 0
+    assert footprint_bytes(a) > a.explicit_scratch_bytes() > 0

@@ -81,3 +81,20 @@ def test_admission_by_footprint_serializes_when_the_budget_is_small():
     for q in reqs:
         assert np.array_equal(q.result, _alone(q, 1e-3, "wave"))
     assert srv.resident == 0
+
+
+def test_channel_runs_in_the_mix_equal_their_runs_alone():
+    """3D channel requests served beside RB 2D ones (the served configuration mixes both): each
+    result bit-identical to its run alone."""
+    from conatus_amd.problems import Channel3D
+    from conatus_amd.serving import Request, Server
+    reqs = _requests([(16, 12, 2e4, 9, 0), (16, 12, 5e4, 6, 2)])
+    for i, (re_, steps, arrival) in enumerate([(800.0, 8, 0), (1500.0, 5, 1), (800.0, 6, 3)]):
+        pb = Channel3D(Nx=8, Ny=13, Nz=8, Re=re_, G=2.0 / 1000.0)
+        reqs.append(Request(pb, steps, pb.initial_state(seed=i, energy=1e-3), arrival=arrival))
+    srv = Server("SBDF2", 1e-3, autotune=False, mode="wave", channel_page_size=2)
+    st = srv.serve(reqs, clock="ticks")
+    print(st, flush=True)
+    assert st["requests"] == len(reqs) and "channel 8x13x8" in st["pages"]
+    for q in reqs:
+        assert np.array_equal(q.result, _alone(q, 1e-3, "wave")), q.rid

@@ -184,6 +184,207 @@ k_ch_xline(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict_
     }
 }
 
+// Even M: the sums of the pair (q, q + M/2) share their terms, e^{2 pi i k (q + M/2)/M} =
+// (-1)^k e^{2 pi i k q / M}: a backward transform accumulates the even-k and odd-k terms apart
+// (value at q: even + odd, at q + M/2: even - odd), a forward one sums g_q + g_{q+M/2} (even k)
+// or g_q - g_{q+M/2} (odd k) over q < M/2.  Half the multiply-adds of the direct sums.
+
+// one kz term of the z-backward sums of the three components (value, d/dz, d/dy), into acc
+__device__ __forceinline__ void zback_term(double (&acc)[18], const zc e, const zc* al, int Kz, int iz, double kz) {
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        const zc v = al[(2 * cc) * Kz + iz], d = al[(2 * cc + 1) * Kz + iz];
+        const double pr = fma(e.re, v.re, -(e.im * v.im)), pi = fma(e.re, v.im, e.im * v.re);
+        acc[6 * cc + 0] += pr;
+        acc[6 * cc + 1] += pi;
+        acc[6 * cc + 2] = fma(-kz, pi, acc[6 * cc + 2]);
+        acc[6 * cc + 3] = fma(kz, pr, acc[6 * cc + 3]);
+        acc[6 * cc + 4] += fma(e.re, d.re, -(e.im * d.im));
+        acc[6 * cc + 5] += fma(e.re, d.im, e.im * d.re);
+    }
+}
+
+// one kx term of the x-backward sums of the 12 factors (value, d/dx, d/dy, d/dz of u, v, w)
+__device__ __forceinline__ void xback_term(double (&f)[12], const zc e, const zc* zl, int Kx, int kx, double kk) {
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        const zc v = zl[(3 * cc) * Kx + kx], dz = zl[(3 * cc + 1) * Kx + kx], dy = zl[(3 * cc + 2) * Kx + kx];
+        const double pr = fma(e.re, v.re, -(e.im * v.im)), pi = fma(e.re, v.im, e.im * v.re);
+        f[4 * cc + 0] += pr;
+        f[4 * cc + 1] = fma(-kk, pi, f[4 * cc + 1]);
+        f[4 * cc + 2] += fma(e.re, dy.re, -(e.im * dy.im));
+        f[4 * cc + 3] += fma(e.re, dz.re, -(e.im * dz.im));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_zback2(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, int Kx, int Kz, int Nz,
+            int My, int Mz, double kz0) {
+    extern __shared__ __align__(16) char lds[];
+    zc* a = reinterpret_cast<zc*>(lds);              // [KB][6][Kz]
+    zc* W = a + (size_t)KB * 6 * Kz;                 // [Mz]
+    const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
+    for (int q = tid; q < KB * 6 * Kz; q += blockDim.x) {
+        const int l = q / (6 * Kz), r = q - l * 6 * Kz, kx = kxb + l;
+        a[q] = kx < Kx ? H1[(size_t)(kx * My + j) * 6 * Kz + r] : zc{0.0, 0.0};
+    }
+    for (int q = tid; q < Mz; q += blockDim.x) W[q] = Wz[q];
+    __syncthreads();
+    const int half = Nz / 2, Hz = Mz / 2;
+    const int neg0 = Mz - (Kz - half);
+    for (int p = tid; p < Hz * KB; p += blockDim.x) {
+        const int l = p % KB, zm = p / KB, kx = kxb + l;
+        const zc* al = a + (size_t)l * 6 * Kz;
+        double ev[18], od[18];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) {
+            ev[q] = 0.0;
+            od[q] = 0.0;
+        }
+        int idx = 0;
+        for (int iz = 0; iz < Kz; ++iz) {
+            if (iz == half) idx = (int)(((long long)neg0 * zm) % Mz);
+            const zc e = W[idx];
+            const int kzi = iz < half ? iz : iz - Kz;
+            const double kz = (double)kzi * kz0;
+            if (kzi & 1)                             // uniform across the workgroup
+                zback_term(od, e, al, Kz, iz, kz);
+            else
+                zback_term(ev, e, al, Kz, iz, kz);
+            idx += zm;
+            if (idx >= Mz) idx -= Mz;
+        }
+        if (kx < Kx) {
+            zc* h0 = H2 + (size_t)(j * Mz + zm) * 9 * Kx + kx;
+            zc* h1 = H2 + (size_t)(j * Mz + zm + Hz) * 9 * Kx + kx;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                h0[(size_t)q * Kx] = zc{ev[2 * q] + od[2 * q], ev[2 * q + 1] + od[2 * q + 1]};
+                h1[(size_t)q * Kx] = zc{ev[2 * q] - od[2 * q], ev[2 * q + 1] - od[2 * q + 1]};
+            }
+        }
+    }
+}
+
+constexpr int XL = 2;   // grid lines per workgroup of k_ch_xline2
+
+__global__ void __launch_bounds__(192)
+k_ch_xline2(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, int Kx, int Mz, int Mx,
+            double kx0) {
+    extern __shared__ __align__(16) char lds[];
+    zc* z = reinterpret_cast<zc*>(lds);              // [XL][9][Kx]
+    zc* W = z + XL * 9 * Kx;                         // [Mx]
+    double* G = reinterpret_cast<double*>(W + Mx);   // [XL][3][Mx]
+    const int zm0 = blockIdx.x * XL, j = blockIdx.y, tid = threadIdx.x, Hx = Mx / 2;
+    const zc* src = H2 + (size_t)(j * Mz + zm0) * 9 * Kx;   // the XL lines are consecutive
+    for (int q = tid; q < XL * 9 * Kx; q += blockDim.x) z[q] = src[q];
+    for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wx[q];
+    __syncthreads();
+    for (int p = tid; p < XL * Hx; p += blockDim.x) {
+        const int ln = p / Hx, i = p - ln * Hx;
+        const zc* zl = z + (size_t)ln * 9 * Kx;
+        double ev[12], od[12];                       // per component: value, d/dx, d/dy, d/dz
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            ev[q] = 0.0;
+            od[q] = 0.0;
+        }
+        // kx = 0 (weight 1) into the even sums, then pairs (odd, even) with weight 2
+        xback_term(ev, W[0], zl, Kx, 0, 0.0);
+        int m = i;                                   // kx i mod Mx at kx = 1
+        for (int kx = 1; kx < Kx; kx += 2) {
+            const zc t = W[m];
+            xback_term(od, zc{2.0 * t.re, 2.0 * t.im}, zl, Kx, kx, kx * kx0);
+            m += i;
+            if (m >= Mx) m -= Mx;
+            if (kx + 1 < Kx) {
+                const zc t2 = W[m];
+                xback_term(ev, zc{2.0 * t2.re, 2.0 * t2.im}, zl, Kx, kx + 1, (kx + 1) * kx0);
+                m += i;
+                if (m >= Mx) m -= Mx;
+            }
+        }
+        double* Gl = G + (size_t)ln * 3 * Mx;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            double f[12];
+#pragma unroll
+            for (int q = 0; q < 12; ++q) f[q] = side ? ev[q] - od[q] : ev[q] + od[q];
+            const double u = f[0], v = f[4], w = f[8];
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+                Gl[o * Mx + i + side * Hx] = -fma(u, f[4 * o + 1], fma(v, f[4 * o + 2], w * f[4 * o + 3]));
+        }
+    }
+    __syncthreads();
+    // sums and differences of the pairs (i, i + Mx/2), in place: [0, Hx) sums, [Hx, Mx) differences
+    for (int p = tid; p < XL * 3 * Hx; p += blockDim.x) {
+        const int lo = p / Hx, i = p - lo * Hx;
+        double* g = G + (size_t)lo * Mx;
+        const double a0 = g[i], a1 = g[i + Hx];
+        g[i] = a0 + a1;
+        g[i + Hx] = a0 - a1;
+    }
+    __syncthreads();
+    const double inv = 1.0 / Mx;
+    for (int p = tid; p < XL * 3 * Kx; p += blockDim.x) {
+        const int lo = p / Kx, kx = p - lo * Kx, ln = lo / 3;
+        const double* g = G + (size_t)lo * Mx + ((kx & 1) ? Hx : 0);
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int i = 0; i < Hx; ++i) {
+            const zc t = W[m];
+            const double gi = g[i] * inv;
+            re = fma(t.re, gi, re);
+            im = fma(-t.im, gi, im);
+            m += kx;
+            if (m >= Mx) m -= Mx;
+        }
+        H3[(size_t)(j * Mz + zm0 + ln) * 3 * Kx + (lo - 3 * ln) * Kx + kx] = zc{re, im};
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ch_zfwd2(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz,
+           int Mz) {
+    extern __shared__ __align__(16) char lds[];
+    zc* g = reinterpret_cast<zc*>(lds);              // [KB][3][Mz]: sums in [0, Hz), differences in [Hz, Mz)
+    zc* W = g + (size_t)KB * 3 * Mz;                 // [Mz]
+    const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x, Hz = Mz / 2;
+    for (int q = tid; q < KB * 3 * Hz; q += blockDim.x) {
+        const int l = q / (3 * Hz), r = q - l * 3 * Hz, o = r / Hz, zm = r - o * Hz, kx = kxb + l;
+        zc a0{0.0, 0.0}, a1{0.0, 0.0};
+        if (kx < Kx) {
+            a0 = H3[((size_t)(j * Mz + zm) * 3 + o) * Kx + kx];
+            a1 = H3[((size_t)(j * Mz + zm + Hz) * 3 + o) * Kx + kx];
+        }
+        zc* gl = g + (size_t)(l * 3 + o) * Mz;
+        gl[zm] = zc{a0.re + a1.re, a0.im + a1.im};
+        gl[zm + Hz] = zc{a0.re - a1.re, a0.im - a1.im};
+    }
+    for (int q = tid; q < Mz; q += blockDim.x) W[q] = Wz[q];
+    __syncthreads();
+    const int half = Nz / 2;
+    const double inv = 1.0 / Mz;
+    for (int p = tid; p < Kz * KB * 3; p += blockDim.x) {
+        const int iz = p % Kz, lo = p / Kz, l = lo / 3, o = lo - 3 * l, kx = kxb + l;
+        const int kzi = iz < half ? iz : iz - Kz;
+        const int kzm = iz < half ? iz : iz - Kz + Mz;
+        const zc* gl = g + (size_t)(l * 3 + o) * Mz + ((kzi & 1) ? Hz : 0);
+        double re = 0.0, im = 0.0;
+        int idx = 0;
+        for (int zm = 0; zm < Hz; ++zm) {
+            const zc t = W[idx];
+            const zc v = gl[zm];
+            re += fma(t.re, v.re, t.im * v.im);
+            im += fma(t.re, v.im, -(t.im * v.re));
+            idx += kzm;
+            if (idx >= Mz) idx -= Mz;
+        }
+        if (kx < Kx) H4[((size_t)(j * Kx + kx) * 3 + o) * Kz + iz] = zc{re * inv, im * inv};
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz, int Mz) {
     extern __shared__ __align__(16) char lds[];
@@ -275,8 +476,10 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
     const size_t lds_c = ((size_t)9 * d.Kx + Mx) * sizeof(zc) + (size_t)3 * Mx * sizeof(double);
     const size_t lds_d = ((size_t)KB * 3 * Mz + Mz) * sizeof(zc);
     const size_t lds_e = (size_t)My * 3 * MA * sizeof(zc);
+    const bool split_x = Mx % 2 == 0 && Mz % XL == 0, split_z = Mz % 2 == 0;
+    const size_t lds_c2 = ((size_t)XL * 9 * d.Kx + Mx) * sizeof(zc) + (size_t)XL * 3 * Mx * sizeof(double);
     const size_t cap = 160 * 1024;
-    if (lds_a > cap || lds_b > cap || lds_c > cap || lds_d > cap || lds_e > cap) {
+    if (lds_a > cap || lds_b > cap || lds_c > cap || lds_d > cap || lds_e > cap || (split_x && lds_c2 > cap)) {
         cn::set_error("cn_ch3d_explicit: grid too large for the transform kernels (LDS)");
         return CN_EUNSUPPORTED;
     }
@@ -286,10 +489,22 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
     const dim3 gy((d.Kz + MA - 1) / MA, d.Kx), gz((d.Kx + KB - 1) / KB, My), gx(Mz, My);
     hipLaunchKernelGGL(k_ch_yback, gy, dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, By, r1, d.Kx, d.Kz,
                        d.N, My, n);
-    hipLaunchKernelGGL(k_ch_zback, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, My,
-                       Mz, kz0);
-    hipLaunchKernelGGL(k_ch_xline, gx, dim3(256), lds_c, s, r2, static_cast<const zc*>(Wx), r1, d.Kx, Mz, Mx, kx0);
-    hipLaunchKernelGGL(k_ch_zfwd, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, Mz);
+    if (split_z)
+        hipLaunchKernelGGL(k_ch_zback2, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
+                           My, Mz, kz0);
+    else
+        hipLaunchKernelGGL(k_ch_zback, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
+                           My, Mz, kz0);
+    if (split_x)
+        hipLaunchKernelGGL(k_ch_xline2, dim3(Mz / XL, My), dim3(192), lds_c2, s, r2, static_cast<const zc*>(Wx), r1,
+                           d.Kx, Mz, Mx, kx0);
+    else
+        hipLaunchKernelGGL(k_ch_xline, gx, dim3(256), lds_c, s, r2, static_cast<const zc*>(Wx), r1, d.Kx, Mz, Mx, kx0);
+    if (split_z)
+        hipLaunchKernelGGL(k_ch_zfwd2, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
+                           Mz);
+    else
+        hipLaunchKernelGGL(k_ch_zfwd, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, Mz);
     hipLaunchKernelGGL(k_ch_yfwd, gy, dim3(256), lds_e, s, r2, Ay, static_cast<const zc*>(base), static_cast<zc*>(F),
                        d.Kx, d.Kz, d.N, My, n, R);
     hipError_t e = hipGetLastError();

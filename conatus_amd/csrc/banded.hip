@@ -312,9 +312,12 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
 }
 
 // wave mode: one staging buffer, and the whole LDS of a wavefront ([x: n entries][2 buffers])
-int wave_buf(const cn_bb& h) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz); }
-size_t wave_lds_bytes(const cn_bb& h) {
-    return 2 * (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h);
+// staged block of the wave-mode sweeps for a launch of `waves` wavefronts: the longer block while
+// the batch leaves CUs idle at two workgroups per CU, the shorter one (more resident) beyond
+int wave_steps(size_t waves) { return waves > 512 ? SPW_MANY : SPW_FEW; }
+int wave_buf(const cn_bb& h, int sp) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz, sp); }
+size_t wave_lds_bytes(const cn_bb& h, int sp) {
+    return 2 * (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h, sp);
 }
 
 // Block length of the streamed sweeps: 16 steps when two buffers of buf16 bytes (the largest sweep
@@ -369,16 +372,18 @@ void run_factor(cn_bb& h, hipStream_t s) {
     if (h.r == 0) return;
     // W = U^-1 (L^-1 P_B U) in place and S = I + V^T W, then the LU of S
     if (h.mode == 1) {
-        const size_t lds = wave_lds_bytes(h);
+        const int sp = wave_steps((size_t)h.B * h.r);
+        const size_t lds = wave_lds_bytes(h, sp);
         if (h.dtype == CN_C128)
-            hipLaunchKernelGGL((k_border_w<zc>), dim3(h.B, h.r), dim3(64), lds, s, static_cast<const zc*>(h.Uf.p),
-                               static_cast<const zc*>(h.Vt.p), static_cast<zc*>(h.W.p), static_cast<zc*>(h.S.p), h.n,
-                               h.r, h.M, h.US, wave_buf(h));
+            hipLaunchKernelGGL((sp == SPW_MANY ? k_border_w<zc, SPW_MANY> : k_border_w<zc, SPW_FEW>), dim3(h.B, h.r),
+                               dim3(64), lds, s, static_cast<const zc*>(h.Uf.p), static_cast<const zc*>(h.Vt.p),
+                               static_cast<zc*>(h.W.p), static_cast<zc*>(h.S.p), h.n, h.r, h.M, h.US,
+                               wave_buf(h, sp));
         else
-            hipLaunchKernelGGL((k_border_w<double>), dim3(h.B, h.r), dim3(64), lds, s,
-                               static_cast<const double*>(h.Uf.p), static_cast<const double*>(h.Vt.p),
-                               static_cast<double*>(h.W.p), static_cast<double*>(h.S.p), h.n, h.r, h.M, h.US,
-                               wave_buf(h));
+            hipLaunchKernelGGL((sp == SPW_MANY ? k_border_w<double, SPW_MANY> : k_border_w<double, SPW_FEW>),
+                               dim3(h.B, h.r), dim3(64), lds, s, static_cast<const double*>(h.Uf.p),
+                               static_cast<const double*>(h.Vt.p), static_cast<double*>(h.W.p),
+                               static_cast<double*>(h.S.p), h.n, h.r, h.M, h.US, wave_buf(h, sp));
         hip_check(hipGetLastError(), "k_border_w launch");
     }
     int spx = 0;
@@ -548,7 +553,8 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
         const int lane_buf = std::max({fwd_bytes(8, h.KLP, e), bwd_bytes(8, h.US, rb, e), cor_bytes(8, rb, e)});
         const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf + 2 * stage_bytes(8, e) <= LDS_CAP;
         const bool wave_ok =
-            2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e) <= (size_t)LDS_CAP;
+            2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e, SPW_FEW) <=
+                (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
         const std::string want = g_mode_request == 1 ? "lane" : g_mode_request == 2 ? "wave" : em ? em : "";
         if (want == "wave") h.mode = wave_ok ? 1 : 0;
@@ -648,20 +654,22 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
         reserve_impl(h, nrhs);
     }
     if (h.mode == 1) {
-        const size_t lds = wave_lds_bytes(h);
+        const int sp = wave_steps((size_t)h.B * nrhs);
+        const size_t lds = wave_lds_bytes(h, sp);
         if (h.dtype == CN_C128)
-            hipLaunchKernelGGL((k_solve_w<zc>), dim3(h.B, nrhs), dim3(64), lds, s, static_cast<const zc*>(h.Uf.p),
-                               static_cast<const zc*>(h.Lf.p), static_cast<const int*>(h.piv.p),
-                               static_cast<const zc*>(h.Vt.p), static_cast<const zc*>(h.W.p),
-                               static_cast<const zc*>(h.S.p), static_cast<const int*>(h.spiv.p),
-                               static_cast<const int*>(h.perm.p), io, h.n, h.r, h.KLP, h.M, h.US, wave_buf(h));
-        else
-            hipLaunchKernelGGL((k_solve_w<double>), dim3(h.B, nrhs), dim3(64), lds, s,
-                               static_cast<const double*>(h.Uf.p), static_cast<const double*>(h.Lf.p),
-                               static_cast<const int*>(h.piv.p), static_cast<const double*>(h.Vt.p),
-                               static_cast<const double*>(h.W.p), static_cast<const double*>(h.S.p),
+            hipLaunchKernelGGL((sp == SPW_MANY ? k_solve_w<zc, SPW_MANY> : k_solve_w<zc, SPW_FEW>), dim3(h.B, nrhs),
+                               dim3(64), lds, s, static_cast<const zc*>(h.Uf.p), static_cast<const zc*>(h.Lf.p),
+                               static_cast<const int*>(h.piv.p), static_cast<const zc*>(h.Vt.p),
+                               static_cast<const zc*>(h.W.p), static_cast<const zc*>(h.S.p),
                                static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), io, h.n, h.r,
-                               h.KLP, h.M, h.US, wave_buf(h));
+                               h.KLP, h.M, h.US, wave_buf(h, sp));
+        else
+            hipLaunchKernelGGL((sp == SPW_MANY ? k_solve_w<double, SPW_MANY> : k_solve_w<double, SPW_FEW>),
+                               dim3(h.B, nrhs), dim3(64), lds, s, static_cast<const double*>(h.Uf.p),
+                               static_cast<const double*>(h.Lf.p), static_cast<const int*>(h.piv.p),
+                               static_cast<const double*>(h.Vt.p), static_cast<const double*>(h.W.p),
+                               static_cast<const double*>(h.S.p), static_cast<const int*>(h.spiv.p),
+                               static_cast<const int*>(h.perm.p), io, h.n, h.r, h.KLP, h.M, h.US, wave_buf(h, sp));
         hip_check(hipGetLastError(), "k_solve_w launch");
         return;
     }

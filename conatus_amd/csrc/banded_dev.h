@@ -1073,11 +1073,14 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
 //   forward (w = kl+1 lanes): the gbtrs column order, pivot exchange of lanes 0 and p;
 //   backward (w = kl+ku+1 lanes): the column-oriented order of LAPACK's banded triangular solve,
 //   with the division by U(j, j) applied when the finished rows are flushed.
-// The L rows + pivots (forward) and U columns (backward) of the next block of SPW steps are
+// The L rows + pivots (forward) and U columns (backward) of the next block of SP steps are
 // copied HBM -> LDS while the current block is computed.  t = V^T y, z = S^-1 t and
 // x = y - W z are parallel over the entries of x.
 // ------------------------------------------------------------------------------------------
-constexpr int SPW = 32;   // steps per staged block in wave mode
+// Steps per staged block in wave mode, a template parameter SP of the sweeps: 32 for few systems
+// (fewer block hand-overs per sweep), 16 when the batch fills the chip (three buffers of 16 steps
+// leave LDS for 4 one-wavefront workgroups per CU at kl = 12, ku = 18 instead of 2).
+constexpr int SPW_FEW = 32, SPW_MANY = 16;
 
 // Ordering point of a one-wavefront workgroup: LDS operations of a wavefront execute in order,
 // and the staging copies are waited for explicitly (vm_wait_le), so only the compiler has to be
@@ -1135,9 +1138,9 @@ __host__ __device__ constexpr int align16(int b) { return (b + 15) / 16 * 16; }
 // LDS after the two vectors: zero slot, capacitance LU (MAXR^2 entries) and its pivots
 constexpr int WAVE_SMALL = 16 + MAXR * MAXR * 16 + MAXR * 4 + 16;
 // LDS bytes of one staging buffer of the wave-mode sweeps
-__host__ __device__ constexpr int wave_buf_bytes(int KL, int US, int esz) {
-    return align16(SPW * KL * esz + SPW * 4) > align16(SPW * US * esz) ? align16(SPW * KL * esz + SPW * 4)
-                                                                        : align16(SPW * US * esz);
+__host__ __device__ constexpr int wave_buf_bytes(int KL, int US, int esz, int SP) {
+    return align16(SP * KL * esz + SP * 4) > align16(SP * US * esz) ? align16(SP * KL * esz + SP * 4)
+                                                                    : align16(SP * US * esz);
 }
 
 // number of copy instructions dmaT / dma4 issue for `count` entries / bytes
@@ -1149,17 +1152,17 @@ __device__ __forceinline__ int dma4_count(int nbytes) { return (nbytes + 255) / 
 
 
 // Staging copies of one block (returns the number of copy instructions issued)
-template <class T>
+template <class T, int SP>
 __device__ __forceinline__ int wave_stage_fwd(const T* __restrict__ L0, const int* __restrict__ P0, int n, int KL,
                                               int blk, char* buf, int lane) {
-    const int j0 = blk * SPW, cnt = min(SPW, n - j0);
+    const int j0 = blk * SP, cnt = min(SP, n - j0);
     dmaT<T>(L0 + (size_t)j0 * KL, buf, cnt * KL, lane);
-    dma4(P0 + j0, buf + SPW * KL * (int)sizeof(T), cnt * 4, lane);
+    dma4(P0 + j0, buf + SP * KL * (int)sizeof(T), cnt * 4, lane);
     return dmaT_count<T>(cnt * KL) + dma4_count(cnt * 4);
 }
-template <class T>
+template <class T, int SP>
 __device__ __forceinline__ int wave_stage_bwd(const T* __restrict__ Uc0, int n, int US, int blk, char* buf, int lane) {
-    const int jhi = n - 1 - blk * SPW, jlo = max(0, jhi - SPW + 1);
+    const int jhi = n - 1 - blk * SP, jlo = max(0, jhi - SP + 1);
     dmaT<T>(Uc0 + (size_t)jlo * US, buf, (jhi - jlo + 1) * US, lane);
     return dmaT_count<T>((jhi - jlo + 1) * US);
 }
@@ -1179,7 +1182,7 @@ __device__ __forceinline__ zc rol1(zc v) { return zc{rol1(v.re), rol1(v.im)}; }
 // lanes read a zero slot), and one wave rotation, which moves the next row into the window and the
 // finished row to lane 63.  Steps run in branch-free groups of SB <= 64 - w; after a group the
 // lanes 64-SB .. 63 hold its finished rows: they are stored, and the lanes above the window are
-// reloaded with the next rows of c.  The multipliers of a block of SPW steps are read from LDS
+// reloaded with the next rows of c.  The multipliers of a block of SP steps are read from LDS
 // into registers before the block.  A partial last block runs step by step (SB = 0: every block).
 template <class T>
 __device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
@@ -1190,36 +1193,36 @@ __device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
     y = rol1(y);
 }
 
-template <class T, int SB>
+template <class T, int SB, int SP>
 __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int* __restrict__ P0, T* yv,
                                              const char* zslot, char* bufs, int bufbytes, int n, int KL, int lane) {
     constexpr int ES = (int)sizeof(T);
     const int w = KL + 1;
     const int R = 64 - w;
-    const int nblk = (n + SPW - 1) / SPW;
+    const int nblk = (n + SP - 1) / SP;
     const bool upd = lane >= 1 && lane <= KL;
     const bool above = lane >= w;
     T y = lane < n ? yv[lane] : cn::zero<T>();
     int t = 0, jf = 0;            // steps since the last flush, first row finished since then
-    const int Poff = SPW * KL * ES;
+    const int Poff = SP * KL * ES;
     // three staging buffers, copies issued two blocks ahead
-    wave_stage_fwd<T>(L0, P0, n, KL, 0, bufs, lane);
-    const int c1 = nblk > 1 ? wave_stage_fwd<T>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
+    wave_stage_fwd<T, SP>(L0, P0, n, KL, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_fwd<T, SP>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
     vm_wait_le(c1);
     wave_sync();
     for (int b = 0; b < nblk; ++b) {
         char* cur = bufs + (b % 3) * bufbytes;
-        const int c2 = b + 2 < nblk ? wave_stage_fwd<T>(L0, P0, n, KL, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
-        const int j0 = b * SPW, cnt = min(SPW, n - j0);
+        const int c2 = b + 2 < nblk ? wave_stage_fwd<T, SP>(L0, P0, n, KL, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int j0 = b * SP, cnt = min(SP, n - j0);
         const int pv = lane < cnt ? lds_at<int>(cur + Poff + lane * 4) : 0;   // pivot of step lane
         const char* lb = upd ? cur + (lane - 1) * ES : zslot;
         const int lst = upd ? KL * ES : 0;
-        if (SB > 0 && cnt == SPW) {
-            T lv[SPW];
+        if (SB > 0 && cnt == SP) {
+            T lv[SP];
 #pragma unroll
-            for (int sidx = 0; sidx < SPW; ++sidx) lv[sidx] = lds_at<T>(lb + sidx * lst);
+            for (int sidx = 0; sidx < SP; ++sidx) lv[sidx] = lds_at<T>(lb + sidx * lst);
 #pragma unroll
-            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+            for (int q = 0; q < (SB > 0 ? SP / SB : 1); ++q) {
 #pragma unroll
                 for (int u = 0; u < SB; ++u) fwd_step<T>(y, lv[q * SB + u], __builtin_amdgcn_readlane(pv, q * SB + u), lane);
                 const int jn = j0 + (q + 1) * SB;     // first row of the new window
@@ -1262,34 +1265,34 @@ __device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
     y = rol1(y);
 }
 
-template <class T, int SB>
+template <class T, int SB, int SP>
 __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot, char* bufs,
                                               int bufbytes, int n, int w, int US, int lane) {
     constexpr int ES = (int)sizeof(T);
     const int R = 64 - w;
-    const int nblk = (n + SPW - 1) / SPW;
+    const int nblk = (n + SP - 1) / SP;
     const bool upd = lane >= 1 && lane < w;
     const bool above = lane >= w;
     wave_load<T>(dv, Uc0, US, n, lane);
     T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
     int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
-    wave_stage_bwd<T>(Uc0, n, US, 0, bufs, lane);
-    const int c1 = nblk > 1 ? wave_stage_bwd<T>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
+    wave_stage_bwd<T, SP>(Uc0, n, US, 0, bufs, lane);
+    const int c1 = nblk > 1 ? wave_stage_bwd<T, SP>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
     vm_wait_le(c1);
     wave_sync();
     for (int b = 0; b < nblk; ++b) {
         char* cur = bufs + (b % 3) * bufbytes;
-        const int c2 = b + 2 < nblk ? wave_stage_bwd<T>(Uc0, n, US, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
-        const int jhi = n - 1 - b * SPW, jlo = max(0, jhi - SPW + 1);
+        const int c2 = b + 2 < nblk ? wave_stage_bwd<T, SP>(Uc0, n, US, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        const int jhi = n - 1 - b * SP, jlo = max(0, jhi - SP + 1);
         const int cnt = jhi - jlo + 1;
         const char* ub = upd ? cur + lane * ES : zslot;
         const int ust = upd ? US * ES : 0;
-        if (SB > 0 && cnt == SPW) {
-            T uv[SPW];
+        if (SB > 0 && cnt == SP) {
+            T uv[SP];
 #pragma unroll
-            for (int sidx = 0; sidx < SPW; ++sidx) uv[sidx] = lds_at<T>(ub + (SPW - 1 - sidx) * ust);
+            for (int sidx = 0; sidx < SP; ++sidx) uv[sidx] = lds_at<T>(ub + (SP - 1 - sidx) * ust);
 #pragma unroll
-            for (int q = 0; q < (SB > 0 ? SPW / SB : 1); ++q) {
+            for (int q = 0; q < (SB > 0 ? SP / SB : 1); ++q) {
 #pragma unroll
                 for (int u = 0; u < SB; ++u) bwd_step<T>(y, uv[q * SB + u], lane);
                 const int jn = jhi - (q + 1) * SB;    // row of lane 0 in the new window
@@ -1326,16 +1329,21 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
 }
 
 // group size of the backward sweep for a window of w lanes (0: step by step)
-__host__ __device__ constexpr int bwd_group(int w) { return w <= 32 ? 32 : w <= 48 ? 16 : w <= 56 ? 8 : 0; }
+__host__ __device__ constexpr int bwd_group(int w, int SP) {
+    return (SP >= 32 && w <= 32) ? 32 : w <= 48 ? 16 : w <= 56 ? 8 : 0;
+}
+static_assert(SPW_FEW % 32 == 0 && SPW_MANY % 16 == 0, "the step groups divide a staged block");
 
-template <class T>
+template <class T, int SP>
 __device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot,
                                                   char* bufs, int bufbytes, int n, int w, int US, int lane) {
-    switch (bwd_group(w)) {
-        case 32: wave_backward<T, 32>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
-        case 16: wave_backward<T, 16>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
-        case 8: wave_backward<T, 8>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
-        default: wave_backward<T, 0>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+    switch (bwd_group(w, SP)) {
+        case 32:
+            if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane);
+            break;
+        case 16: wave_backward<T, 16, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 8: wave_backward<T, 8, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        default: wave_backward<T, 0, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
     }
 }
 
@@ -1376,7 +1384,7 @@ __device__ __forceinline__ void wave_vty(const T* __restrict__ V0, const T* yv, 
 
 // Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][dv: n entries][zero
 // slot, S, pivots of S][3 staging buffers].
-template <class T>
+template <class T, int SP>
 __global__ void __launch_bounds__(64)
 k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
@@ -1404,8 +1412,8 @@ k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
     }
     if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
     wave_sync();
-    wave_forward<T, 32>(Lf + mn * KL, piv + mn, yv, zslot, bufs, bufbytes, n, KL, lane);   // kl + 1 <= 17 lanes
-    wave_backward_any<T>(Uc + mn * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    wave_forward<T, (SP < 32 ? SP : 32), SP>(Lf + mn * KL, piv + mn, yv, zslot, bufs, bufbytes, n, KL, lane);   // kl + 1 <= 17 lanes
+    wave_backward_any<T, SP>(Uc + mn * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
     if (r == 0) {
         for (int j = lane; j < n; j += 64) io_store<T>(io, mn, k, j, yv[j]);
         return;
@@ -1439,7 +1447,7 @@ k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
 
 // Border completion in wave mode: grid (B, r); W row a = U^-1 (its forward half), in place, and
 // column a of S = I + V^T W.
-template <class T>
+template <class T, int SP>
 __global__ void __launch_bounds__(64)
 k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W, T* __restrict__ S, int n, int r,
            int w2, int US, int bufbytes) {
@@ -1454,7 +1462,7 @@ k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W
     wave_load<T>(yv, Wa, 1, n, lane);
     if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
     wave_sync();
-    wave_backward_any<T>(Uc + (size_t)m * n * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    wave_backward_any<T, SP>(Uc + (size_t)m * n * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
     for (int j = lane; j < n; j += 64) Wa[j] = yv[j];
     T t[MAXR];
     wave_vty<T>(Vt + (size_t)m * r * n, yv, n, r, lane, t);

@@ -4,7 +4,8 @@
 // 3 outputs, derivative chains, backward transforms, products, forward transforms, C^(lam)
 // conversion into the stacked rows).
 //
-// Two launches (grid y: ensemble member, the members' states (K, n) one after another):
+// Two launches (grid y: ensemble member, the members' states (K, n) one after another; for even
+// Mx, k_fc_line2 is A with the Fourier sums split into even and odd modes):
 //   A  one workgroup per grid line z_j (Mz workgroups per member), everything in LDS:
 //      z-backward of the 6 factor columns of the modes (u, u_z, w, w_z, T, T_z; the z-derivative
 //      is folded into the matrix Bz_1 = T_n'(z_j)), x-backward of the 8 factors u, u_x, u_z, w,
@@ -135,6 +136,121 @@ k_fc_line(const zc* __restrict__ X, const int* __restrict__ fidx, const double* 
     }
 }
 
+// one mode's terms of the 8 factors at x_i (e = weight times twiddle): u, u_x, u_z, w, w_x, w_z,
+// T_x, T_z; p = e z, Re(p) the value, -k kappa Im(p) the x-derivative
+__device__ __forceinline__ void fc_term(double (&f)[8], const zc e, const zc* Zj, int K, int k, double kk) {
+    const zc zu = Zj[k], zuz = Zj[K + k], zw = Zj[2 * K + k], zwz = Zj[3 * K + k], zT = Zj[4 * K + k],
+             zTz = Zj[5 * K + k];
+    const double pu_r = fma(e.re, zu.re, -(e.im * zu.im)), pu_i = fma(e.re, zu.im, e.im * zu.re);
+    const double pw_r = fma(e.re, zw.re, -(e.im * zw.im)), pw_i = fma(e.re, zw.im, e.im * zw.re);
+    const double pT_i = fma(e.re, zT.im, e.im * zT.re);
+    f[0] += pu_r;
+    f[1] = fma(-kk, pu_i, f[1]);
+    f[2] += fma(e.re, zuz.re, -(e.im * zuz.im));
+    f[3] += pw_r;
+    f[4] = fma(-kk, pw_i, f[4]);
+    f[5] += fma(e.re, zwz.re, -(e.im * zwz.im));
+    f[6] = fma(-kk, pT_i, f[6]);
+    f[7] += fma(e.re, zTz.re, -(e.im * zTz.im));
+}
+
+// k_fc_line for even Mx: the points x_i and x_{i+Mx/2} from one pass over the modes
+// (e^{i k kappa x_{i+Mx/2}} = (-1)^k e^{i k kappa x_i}: even-k and odd-k terms summed apart), and
+// the x-forward over i < Mx/2 of G_i + G_{i+Mx/2} (even k) or G_i - G_{i+Mx/2} (odd k).
+__global__ void __launch_bounds__(256)
+k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
+           const zc* __restrict__ Wt, zc* __restrict__ Gh, int K, int n, int N, int Mz, int Mx, double kappa) {
+    extern __shared__ __align__(16) char lds[];
+    zc* Zj = reinterpret_cast<zc*>(lds);            // [6][K]
+    zc* W = Zj + 6 * K;                              // [Mx]
+    double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx]
+    double* b = G + 3 * Mx;                          // [2][N]
+    const int j = blockIdx.x, tid = threadIdx.x, Hx = Mx / 2;
+    X += (size_t)blockIdx.y * K * n;
+    Gh += (size_t)blockIdx.y * 3 * K * Mz;
+    for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wt[q];
+    for (int q = tid; q < 2 * N; q += blockDim.x) {
+        const int v = q / N, nn = q - v * N;
+        b[q] = Bz[((size_t)v * N + nn) * Mz + j];
+    }
+    __syncthreads();
+    for (int p = tid; p < 3 * K; p += blockDim.x) {
+        const int f = p / K, k = p - f * K;
+        const zc* Xk = X + (size_t)k * n;
+        const int* pos = fidx + (size_t)f * N;
+        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 8
+        for (int nn = 0; nn < N; ++nn) {
+            const zc x = Xk[pos[nn]];
+            const double b0 = b[nn], b1 = b[N + nn];
+            r0 = fma(b0, x.re, r0);
+            i0 = fma(b0, x.im, i0);
+            r1 = fma(b1, x.re, r1);
+            i1 = fma(b1, x.im, i1);
+        }
+        Zj[(2 * f) * K + k] = zc{r0, i0};
+        Zj[(2 * f + 1) * K + k] = zc{r1, i1};
+    }
+    __syncthreads();
+    for (int i = tid; i < Hx; i += blockDim.x) {
+        double ev[8], od[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            ev[q] = 0.0;
+            od[q] = 0.0;
+        }
+        fc_term(ev, W[0], Zj, K, 0, 0.0);
+        int m = i;                                   // k i mod Mx at k = 1
+        for (int k = 1; k < K; k += 2) {
+            const zc t = W[m];
+            fc_term(od, zc{2.0 * t.re, 2.0 * t.im}, Zj, K, k, k * kappa);
+            m += i;
+            if (m >= Mx) m -= Mx;
+            if (k + 1 < K) {
+                const zc t2 = W[m];
+                fc_term(ev, zc{2.0 * t2.re, 2.0 * t2.im}, Zj, K, k + 1, (k + 1) * kappa);
+                m += i;
+                if (m >= Mx) m -= Mx;
+            }
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            double f[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) f[q] = side ? ev[q] - od[q] : ev[q] + od[q];
+            const int at = i + side * Hx;
+            G[at] = -fma(f[0], f[1], f[3] * f[2]);
+            G[Mx + at] = -fma(f[0], f[4], f[3] * f[5]);
+            G[2 * Mx + at] = -fma(f[0], f[6], f[3] * f[7]);
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < 3 * Hx; p += blockDim.x) {   // sums in [0, Hx), differences in [Hx, Mx)
+        const int o = p / Hx, i = p - o * Hx;
+        double* g = G + (size_t)o * Mx;
+        const double a0 = g[i], a1 = g[i + Hx];
+        g[i] = a0 + a1;
+        g[i + Hx] = a0 - a1;
+    }
+    __syncthreads();
+    const double inv = 1.0 / Mx;
+    for (int p = tid; p < 3 * K; p += blockDim.x) {
+        const int o = p / K, k = p - o * K;
+        const double* g = G + (size_t)o * Mx + ((k & 1) ? Hx : 0);
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int i = 0; i < Hx; ++i) {
+            const zc t = W[m];
+            const double gi = g[i] * inv;
+            re = fma(t.re, gi, re);
+            im = fma(-t.im, gi, im);
+            m += k;
+            if (m >= Mx) m -= Mx;
+        }
+        Gh[((size_t)o * K + k) * Mz + j] = zc{re, im};
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_fc_mode_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __restrict__ F, int K, int Mz, int N,
                int n, int R) {
@@ -184,8 +300,12 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
         return CN_EUNSUPPORTED;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
-                       static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
+    if (Mx % 2 == 0)
+        hipLaunchKernelGGL(k_fc_line2, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
+                           static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
+    else
+        hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
+                           static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
     hipLaunchKernelGGL(k_fc_mode_rows, dim3(K, members), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
                        static_cast<zc*>(F), K, Mz, Nz, n, R);
     hipError_t e = hipGetLastError();

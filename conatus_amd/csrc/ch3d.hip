@@ -7,7 +7,8 @@
 //
 // State X: (Kx * Kz, n) complex, mode m = kx * Kz + iz, kz = iz (iz < Nz/2) or iz - Kz, Kz = Nz - 1;
 // fidx[f][nn] = column of Chebyshev coefficient nn of velocity component f (u, v, w).
-// Five launches, each a direct transform along one axis with its operands staged in LDS:
+// Five launches, each a transform along one axis with its operands staged in LDS (sums over the
+// modes, split by parity: symmetric Chebyshev nodes, and for even M the Fourier pairs q, q + M/2):
 //   A  y-backward (per 8 modes):    H1[kx][j][6][iz]  = (u, u_y, v, v_y, w, w_y) at y_j
 //   B  z-backward (per 4 kx, y_j):  H2[j][zm][9][kx]  = per component: value, d/dz, d/dy at z_m
 //   C  x-line (per y_j, z_m):       x-backward of the 12 factors (d/dx = i kx kx0 on the modes),
@@ -42,6 +43,9 @@ struct ChDims {
     size_t bytes() const { return (region1() + region2()) * sizeof(zc); }
 };
 
+// y-backward of 8 modes.  The Gauss-Chebyshev nodes are symmetric, y_{My-1-j} = -y_j, and
+// T_n(-y) = (-1)^n T_n(y), T_n'(-y) = (-1)^(n+1) T_n'(y): the even-n and odd-n sums at y_j give
+// the values at both nodes of the pair (the middle node, y = 0 for odd My, on its own).
 __global__ void __launch_bounds__(256)
 k_ch_yback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ By,
            zc* __restrict__ H1, int Kx, int Kz, int N, int My, int n) {
@@ -54,25 +58,37 @@ k_ch_yback(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
         c[q] = iz < Kz ? X[(size_t)(kx * Kz + iz) * n + fidx[fn]] : zc{0.0, 0.0};
     }
     __syncthreads();
-    const int l = tid % MA, iz = iz0 + l;
-    for (int p = tid / MA; p < 3 * My; p += blockDim.x / MA) {
-        const int f = p / My, j = p - f * My;
+    const int l = tid % MA, iz = iz0 + l, P = (My + 1) / 2;
+    for (int p = tid / MA; p < 3 * P; p += blockDim.x / MA) {
+        const int f = p / P, j = p - f * P, jr = My - 1 - j;
         const double* b = By + (size_t)j * N * 2;   // (T_n(y_j), T_n'(y_j)) pairs
         const zc* cf = c + (size_t)f * N * MA + l;
-        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
-#pragma unroll 4
-        for (int nn = 0; nn < N; ++nn) {
+        double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // even: v.re v.im d.re d.im, odd: same
+        for (int nn = 0; nn < N; nn += 2) {
             const double b0 = b[2 * nn], b1 = b[2 * nn + 1];
             const zc x = cf[nn * MA];
-            r0 = fma(b0, x.re, r0);
-            i0 = fma(b0, x.im, i0);
-            r1 = fma(b1, x.re, r1);
-            i1 = fma(b1, x.im, i1);
+            s[0] = fma(b0, x.re, s[0]);
+            s[1] = fma(b0, x.im, s[1]);
+            s[2] = fma(b1, x.re, s[2]);
+            s[3] = fma(b1, x.im, s[3]);
+            if (nn + 1 < N) {
+                const double c0 = b[2 * nn + 2], c1 = b[2 * nn + 3];
+                const zc y = cf[(nn + 1) * MA];
+                s[4] = fma(c0, y.re, s[4]);
+                s[5] = fma(c0, y.im, s[5]);
+                s[6] = fma(c1, y.re, s[6]);
+                s[7] = fma(c1, y.im, s[7]);
+            }
         }
         if (iz < Kz) {
             zc* h = H1 + ((size_t)(kx * My + j) * 6 + 2 * f) * Kz + iz;
-            h[0] = zc{r0, i0};
-            h[Kz] = zc{r1, i1};
+            h[0] = zc{s[0] + s[4], s[1] + s[5]};
+            h[Kz] = zc{s[2] + s[6], s[3] + s[7]};
+            if (jr != j) {
+                zc* hr = H1 + ((size_t)(kx * My + jr) * 6 + 2 * f) * Kz + iz;
+                hr[0] = zc{s[0] - s[4], s[1] - s[5]};
+                hr[Kz] = zc{s[6] - s[2], s[7] - s[3]};
+            }
         }
     }
 }
@@ -417,24 +433,42 @@ k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__
     }
 }
 
+// y-forward + conversion of 8 modes: Ay[My-1-j][n] = (-1)^n Ay[j][n] (S_{0->2} keeps the parity
+// of n), so the sums run over j < My/2 on g_j + g_{My-1-j} (even n) or g_j - g_{My-1-j} (odd n),
+// plus the middle node for odd My.
 __global__ void __launch_bounds__(256)
 k_ch_yfwd(const zc* __restrict__ H4, const double* __restrict__ Ay, const zc* __restrict__ base, zc* __restrict__ F,
           int Kx, int Kz, int N, int My, int n, int R) {
     extern __shared__ __align__(16) char lds[];
-    zc* g = reinterpret_cast<zc*>(lds);              // [My][3][MA]
-    const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x;
+    zc* g = reinterpret_cast<zc*>(lds);              // [My][3][MA]; after the fold: sums at j < Hy, differences at My-1-j
+    const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x, Hy = My / 2;
     for (int q = tid; q < My * 3 * MA; q += blockDim.x) {
         const int l = q % MA, r = q / MA, jj = r / 3, o = r - 3 * jj, iz = iz0 + l;
         g[q] = iz < Kz ? H4[((size_t)(jj * Kx + kx) * 3 + o) * Kz + iz] : zc{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int q = tid; q < Hy * 3 * MA; q += blockDim.x) {
+        const int l = q % MA, r = q / MA, jj = r / 3, o = r - 3 * jj;
+        zc* a = g + (size_t)(jj * 3 + o) * MA + l;
+        zc* b = g + (size_t)((My - 1 - jj) * 3 + o) * MA + l;
+        const zc a0 = *a, a1 = *b;
+        *a = zc{a0.re + a1.re, a0.im + a1.im};
+        *b = zc{a0.re - a1.re, a0.im - a1.im};
     }
     __syncthreads();
     const int l = tid % MA, iz = iz0 + l;
     for (int p = tid / MA; p < 3 * N; p += blockDim.x / MA) {
         const int e = p / N, nn = p - e * N;
         double re = 0.0, im = 0.0;
-        for (int jj = 0; jj < My; ++jj) {
+        for (int jj = 0; jj < Hy; ++jj) {
             const double a = Ay[(size_t)jj * N + nn];
-            const zc v = g[(jj * 3 + e) * MA + l];
+            const zc v = g[(((nn & 1) ? My - 1 - jj : jj) * 3 + e) * MA + l];
+            re = fma(a, v.re, re);
+            im = fma(a, v.im, im);
+        }
+        if (My & 1) {
+            const double a = Ay[(size_t)Hy * N + nn];
+            const zc v = g[(Hy * 3 + e) * MA + l];
             re = fma(a, v.re, re);
             im = fma(a, v.im, im);
         }

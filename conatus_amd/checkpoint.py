@@ -35,6 +35,10 @@ def _identity(pb) -> dict:
 def payload(solver) -> tuple[dict, dict]:
     """(meta, host arrays) of an IVPSolver: state, multistep history (newest first) and clock."""
     im = solver.imex
+    if getattr(im, "cn", None) is not None:
+        raise OptionError(f"checkpoint: {solver.scheme} runs carry an L X history that the "
+                          f"{FORMAT} keys (X, MX<i>, F<i>) do not hold",
+                          fix="checkpoint SBDF or RK runs")
     meta = {"format": FORMAT, "scheme": solver.scheme, "dt": float(solver.dt), "pivot": bool(solver.pivot),
             "iteration": int(im.iteration), "t": float(solver.t), "n": int(solver.problem.n),
             "K": int(solver.problem.K), "nhist": int(im.history_length), "identity": _identity(solver.problem)}

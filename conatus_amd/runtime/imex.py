@@ -3,8 +3,11 @@
 The device twin of the reference's ``cuda.Imex`` (python/conatus/runtime/cuda.py:267-308,
 core/src/cuda/imex.cu:56-244) and of the host ``ImexIntegrator`` step arithmetic
 (core/include/conatus/imex.hpp:73-219): SBDF1..4 with the lower-order startup, RK111 / RK222 /
-RK443, factorizations cached by their LHS coefficients (RK443: one), the continuation rule of
-SBDF histories, and F = g * X + h (pointwise) or an external F writer.
+RK443 and RKSMR / RKGFY (python/conatus/timestep/schemes.py:122-137), the Crank-Nicolson
+multistep family CNAB1 / CNAB2 / MCNAB2 / CNLF2 (schemes.py:91-103, step of
+python/conatus/timestep/integrator.py:576-602, CNAB1 until the history is full), factorizations
+cached by their LHS coefficients (RK443: one), the continuation rule of multistep histories, and
+F = g * X + h (pointwise) or an external F writer.
 
 One SBDF step is two launches on the current stream: the fused stage kernel (M X, F, history
 level and right-hand side in one pass) and the out-of-place batched solve into X.  History
@@ -23,9 +26,10 @@ from .. import _lib
 from ..errors import OptionError
 from .banded import BatchHandle, _factor_dtype, csr_args, solve_operand
 
-__all__ = ["Imex", "DeviceCsr", "SCHEMES", "sbdf_coefficients", "rk_tableau"]
+__all__ = ["Imex", "DeviceCsr", "SCHEMES", "CN_SCHEMES", "sbdf_coefficients", "rk_tableau", "cn_coefficients"]
 
-SCHEMES = ("SBDF1", "SBDF2", "SBDF3", "SBDF4", "RK111", "RK222", "RK443")
+CN_SCHEMES = ("CNAB1", "CNAB2", "MCNAB2", "CNLF2")
+SCHEMES = ("SBDF1", "SBDF2", "SBDF3", "SBDF4", "RK111", "RK222", "RK443", "RKSMR", "RKGFY") + CN_SCHEMES
 
 
 def sbdf_coefficients(order: int):
@@ -39,8 +43,21 @@ def sbdf_coefficients(order: int):
     return table[order]
 
 
+def cn_coefficients(name: str):
+    """(a, b, c), a in units of 1/dt, of a CN-family multistep scheme (schemes.py:91-103):
+    sum_j (a_j/dt) M X^{n+1-j} + sum_j b_j L X^{n+1-j} = sum_{j>=1} c_j F^{n+1-j}."""
+    table = {"CNAB1": ((1.0, -1.0), (0.5, 0.5), (0.0, 1.0)),
+             "CNAB2": ((1.0, -1.0, 0.0), (0.5, 0.5, 0.0), (0.0, 1.5, -0.5)),
+             "MCNAB2": ((1.0, -1.0, 0.0), (9 / 16, 6 / 16, 1 / 16), (0.0, 1.5, -0.5)),
+             "CNLF2": ((0.5, 0.0, -0.5), (0.5, 0.0, 0.5), (0.0, 1.0, 0.0))}
+    if name not in table:
+        raise OptionError(f"unknown CN scheme: {name}", fix=f"one of {CN_SCHEMES}")
+    return table[name]
+
+
 def rk_tableau(name: str):
-    """(A, H, c) of the ARS IMEX tableaux RK111, RK222, RK443 (imex.hpp:31-51)."""
+    """(A, H, c) of the IMEX tableaux RK111, RK222, RK443 (imex.hpp:31-51) and RKSMR, RKGFY
+    (schemes.py:122-137), in the cumulative form M X_i = M X_0 + dt sum (A_ij F_j - H_ij L X_j)."""
     if name == "RK111":
         return ((0.0, 0.0), (1.0, 0.0)), ((0.0, 0.0), (0.0, 1.0)), (0.0, 1.0)
     if name == "RK222":
@@ -54,6 +71,18 @@ def rk_tableau(name: str):
                 ((0, 0, 0, 0, 0), (0, 1 / 2, 0, 0, 0), (0, 1 / 6, 1 / 2, 0, 0),
                  (0, -1 / 2, 1 / 2, 1 / 2, 0), (0, 3 / 2, -3 / 2, 1 / 2, 1 / 2)),
                 (0, 1 / 2, 2 / 3, 1 / 2, 1))
+    if name == "RKSMR":
+        # Spalart-Moser-Rogers substeps X_i = X_{i-1} + dt [g_i F_{i-1} + z_i F_{i-2} - a_i L X_{i-1}
+        # - b_i L X_i] summed into cumulative rows
+        al, be = (29 / 96, -3 / 40, 1 / 6), (37 / 160, 5 / 24, 1 / 6)
+        ga, ze = (8 / 15, 5 / 12, 3 / 4), (-17 / 60, -5 / 12)
+        return (((0, 0, 0, 0), (ga[0], 0, 0, 0), (ga[0] + ze[0], ga[1], 0, 0),
+                 (ga[0] + ze[0], ga[1] + ze[1], ga[2], 0)),
+                ((0, 0, 0, 0), (al[0], be[0], 0, 0), (al[0], be[0] + al[1], be[1], 0),
+                 (al[0], be[0] + al[1], be[1] + al[2], be[2])), (0, 8 / 15, 2 / 3, 1))
+    if name == "RKGFY":                  # Heun on F, Crank-Nicolson on L
+        return (((0, 0, 0), (1, 0, 0), (1 / 2, 1 / 2, 0)), ((0, 0, 0), (1 / 2, 1 / 2, 0), (1 / 2, 0, 1 / 2)),
+                (0, 1, 1))
     raise OptionError(f"unknown RK scheme: {name}", fix=f"one of {SCHEMES}")
 
 
@@ -171,8 +200,12 @@ class Imex:
         self._explicit = explicit
         if explicit is not None and (g is not None or h is not None):
             raise OptionError("Imex: pass either the pointwise g, h or an explicit term, not both")
+        self.cn = cn_coefficients(scheme) if scheme in CN_SCHEMES else None
         if scheme.startswith("SBDF"):
             self.order = int(scheme[4])
+            self.tab = None
+        elif self.cn is not None:
+            self.order = 0
             self.tab = None
         else:
             self.order = 0
@@ -184,6 +217,9 @@ class Imex:
         if self.order:
             self._ring = [(z(), z()) for _ in range(self.order)]     # (MX, F) levels
             self._fnew = fz() if explicit is not None else None
+        elif self.cn is not None:
+            s = len(self.cn[0]) - 1
+            self._ring = [(z(), fz(), z()) for _ in range(s)]     # (MX, F, LX) levels
         else:
             s = len(self.tab[2]) - 1
             self._X = [z() for _ in range(s)]
@@ -206,7 +242,7 @@ class Imex:
 
     @property
     def history_length(self) -> int:
-        return len(self._levels) if self.order else 0
+        return len(self._levels) if (self.order or self.cn is not None) else 0
 
     @property
     def n_factorizations(self) -> int:
@@ -253,6 +289,52 @@ class Imex:
                                       v(_stream()))
         _lib.check(st, "Imex (SBDF stage)")
         fac = self._factor(a[0] / self.dt, 1.0)
+        fac.solve_mapped_ptr(_ptr(self._rhs), 0, _ptr(x), 0, _stream())
+
+    def _cn_step(self, x) -> None:
+        """One CN-family step: the new level (M X, F(X), L X) of the current state, then
+        (M + dt b_0/a_0 L) X^{n+1} = M X_j* + dt/a_0 sum_{j>=1} (c_j F_j - b_j L X_j), the
+        reference's (a_0/dt M + b_0 L) X^{n+1} = sum_j (c_j F_j - (a_j/dt) M X_j - b_j L X_j)
+        divided by a_0/dt: every scheme of the family has one M X history term, a_j* = -a_0
+        (CNAB1, CNAB2, MCNAB2: j* = 1; CNLF2: j* = 2).  The right-hand side is the RK stage
+        kernel's M X_0 + sum dA F - sum dH L X."""
+        s = len(self.cn[0]) - 1
+        if len(self._levels) < s:
+            used = set(self._levels)
+            slot = next(i for i in range(s) if i not in used)
+            levels = [slot] + self._levels
+        else:
+            levels = [self._levels[-1]] + self._levels[:-1]
+        mx, f, lx = self._ring[levels[0]]
+        N = self.batch * self.n
+        code = _lib.CN_C128 if self.dtype == np.complex128 else _lib.CN_F64
+        v = ctypes.c_void_p
+        self._M.matvec(x, mx)
+        self._L.matvec(x, lx)
+        if self._explicit is not None:
+            self._explicit.write(x, f)
+        else:
+            _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(f)), v(_ptr(x)), v(_ptr(self._g)),
+                                               v(_ptr(self._hh)), v(_stream())), "Imex (F)")
+        self._levels = levels
+        a, b, c = self.cn if len(levels) >= s else cn_coefficients("CNAB1")
+        q = len(a) - 1
+        jstar = [j for j in range(1, q + 1) if a[j] != 0]
+        if len(jstar) != 1 or a[jstar[0]] != -a[0]:
+            raise OptionError(f"Imex: {self.scheme} is not of the one-M-history CN form")
+        P = ctypes.c_void_p * 4
+        dA = [self.dt * c[j] / a[0] for j in range(1, q + 1)]
+        dH = [self.dt * b[j] / a[0] for j in range(1, q + 1)]
+        st = _lib.lib().cn_rk_rhs(code, N, v(_ptr(self._ring[levels[jstar[0] - 1]][0])),
+                                  P(*([_ptr(self._ring[levels[k]][1]) if dA[k] != 0 else 0 for k in range(q)]
+                                      + [0] * (4 - q))),
+                                  P(*([_ptr(self._ring[levels[k]][2]) if dH[k] != 0 else 0 for k in range(q)]
+                                      + [0] * (4 - q))),
+                                  (ctypes.c_double * 4)(*(dA + [0.0] * (4 - q))),
+                                  (ctypes.c_double * 4)(*(dH + [0.0] * (4 - q))), q, v(_ptr(self._rhs)),
+                                  v(_stream()))
+        _lib.check(st, "Imex (CN right-hand side)")
+        fac = self._factor(1.0, self.dt * b[0] / a[0])
         fac.solve_mapped_ptr(_ptr(self._rhs), 0, _ptr(x), 0, _stream())
 
     def _rk_step(self, x) -> None:
@@ -346,6 +428,8 @@ class Imex:
         for _ in range(int(nsteps)):
             if self.order:
                 self._sbdf_step(d)
+            elif self.cn is not None:
+                self._cn_step(d)
             else:
                 self._rk_step(d)
             t += self.dt
@@ -357,7 +441,9 @@ class Imex:
         """As ``run``, with every full-order step replayed from captured HIP graphs.  Whole chunks
         of steps (``chunk`` rounded up to a multiple of the SBDF ring period) are one graph each,
         so that the host launches one graph per chunk; the remaining steps use one graph per ring
-        phase (one graph for RK).  Startup steps run eagerly."""
+        phase (one graph for RK).  Startup steps run eagerly; CN-family schemes step eagerly."""
+        if self.cn is not None:
+            return self.run(d, t0, nsteps)
         self._check_state(d)
         self._continuation(d, t0)
         t, done = self._graph_startup(d, float(t0), nsteps)
@@ -404,6 +490,8 @@ class Imex:
         """One full-order step of ``d`` from the captured graphs, for an owner that keeps the
         trajectory itself (no continuation check, no host synchronisation): the history must be
         complete (startup done, or levels adopted from other integrators)."""
+        if self.cn is not None:
+            raise OptionError("Imex.replay_step: CN-family schemes step eagerly (run)")
         if self.order and self.history_length < self.order - 1:
             raise OptionError("Imex.replay_step: the SBDF history is not complete")
         if self.order:

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import spectral as S
 from .banded import BorderedBanded
-from .imex import rk_tableau, sbdf_coefficients
+from .imex import CN_SCHEMES, cn_coefficients, cn_rhs, rk_tableau, sbdf_coefficients
 
 
 def to_pairs(c: np.ndarray) -> np.ndarray:
@@ -56,7 +56,9 @@ class RB2DOracle:
     def __init__(self, pb, scheme: str = "SBDF2", dt: float = 1e-3):
         self.pb = pb
         self.order = int(scheme[4]) if scheme.startswith("SBDF") else 0
-        self.tab = None if self.order else rk_tableau(scheme)
+        self.cn = cn_coefficients(scheme) if scheme in CN_SCHEMES else None
+        self.tab = None if (self.order or self.cn is not None) else rk_tableau(scheme)
+        self.LXh = []
         self.dt = dt
         self.M = [m[0] for m in pb.mats]
         self.L = [m[1] for m in pb.mats]
@@ -128,7 +130,27 @@ class RB2DOracle:
             Xs[i] = np.stack([self.factor(kk, 1.0, dt * H[i][i]).solve(rhs[kk]) for kk in range(K)])
         return Xs[s]
 
+    def step_cn(self, X: np.ndarray) -> np.ndarray:
+        """CN-family multistep step per mode (oracle.imex.ImexIntegrator.step_cn)."""
+        K = self.pb.K
+        s = len(self.cn[0]) - 1
+        self.MXh.insert(0, np.stack([self.M[k] @ X[k] for k in range(K)]))
+        self.LXh.insert(0, np.stack([self.L[k] @ X[k] for k in range(K)]))
+        self.Fh.insert(0, self.explicit(X))
+        while len(self.MXh) > s:
+            self.MXh.pop()
+            self.LXh.pop()
+            self.Fh.pop()
+        a, b, c = self.cn if len(self.MXh) >= s else cn_coefficients("CNAB1")
+        rhs = cn_rhs(a, b, c, self.dt, self.MXh, self.LXh, self.Fh)
+        return np.stack([self.factor(k, a[0] / self.dt, b[0]).solve(rhs[k]) for k in range(K)])
+
     def run(self, X: np.ndarray, steps: int) -> np.ndarray:
         for _ in range(steps):
-            X = self.step(X) if self.order else self.step_rk(X)
+            if self.order:
+                X = self.step(X)
+            elif self.cn is not None:
+                X = self.step_cn(X)
+            else:
+                X = self.step_rk(X)
         return X

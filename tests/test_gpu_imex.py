@@ -16,7 +16,8 @@ from tests.imex_cases import heat_system
 
 pytestmark = pytest.mark.gpu
 
-SCHEMES = ["SBDF1", "SBDF2", "SBDF3", "SBDF4", "RK111", "RK222", "RK443"]
+SCHEMES = ["SBDF1", "SBDF2", "SBDF3", "SBDF4", "RK111", "RK222", "RK443", "RKSMR", "RKGFY",
+           "CNAB1", "CNAB2", "MCNAB2", "CNLF2"]
 
 
 def rel(a, b):

@@ -66,10 +66,12 @@ def test_trajectory_matches_oracle(Nx, Nz, dt, steps):
     assert abs(s.t - steps * dt) < 1e-12
 
 
-@pytest.mark.parametrize("scheme", ["RK443", "RK222"])
+@pytest.mark.parametrize("scheme", ["RK443", "RK222", "RKSMR", "RKGFY", "CNAB2", "MCNAB2", "CNLF2"])
 def test_rk_trajectory_matches_oracle(scheme):
     """The reference's published RB 2D timings use RK443 (four stages: F of the first three
-    stages, L X of the stages the later ones use, one solve of M + dt H_ii L per stage)."""
+    stages, L X of the stages the later ones use, one solve of M + dt H_ii L per stage); the
+    other IMEX RK tableaux and the Crank-Nicolson multistep family (CNAB1 startup, then the
+    scheme; eager steps under run_graph) against the oracle's reference-order arithmetic."""
     from conatus_amd.timestep import IVPSolver
     pb = RayleighBenard2D(Nx=32, Nz=16, Lx=4.0, Ra=1e5, Pr=1.0)
     X0 = pb.initial_state(seed=2)
@@ -80,7 +82,10 @@ def test_rk_trajectory_matches_oracle(scheme):
     assert np.all(np.isfinite(Xo))
     r = rel(s.state(), Xo)
     print(f"{scheme}: relative difference after 15 steps {r:.3e}", flush=True)
-    assert r < 1e-11
+    # CNLF2's leapfrog part has a neutral computational mode that keeps rounding differences:
+    # the oracle with its right-hand sides perturbed by 1e-16 (relative) moves 3.7e-11 after these
+    # 15 steps (CNAB2: 9e-15), so its bar is that floor's scale
+    assert r < (2e-10 if scheme == "CNLF2" else 1e-11)
     g = IVPSolver(pb, scheme, 1e-3)
     g.set_state(X0)
     g.run_graph(15)

@@ -21,7 +21,8 @@ def error(scheme, nsteps, T=0.5, phi="cos"):
 
 @pytest.mark.parametrize("scheme,order,phi", [
     ("SBDF1", 1, "cos"), ("SBDF2", 2, "cos"), ("SBDF3", 3, "flat"), ("SBDF4", 4, "flat"),
-    ("RK111", 1, "cos"), ("RK222", 2, "cos"), ("RK443", 3, "cos")])
+    ("RK111", 1, "cos"), ("RK222", 2, "cos"), ("RK443", 3, "cos"), ("RKSMR", 2, "cos"), ("RKGFY", 2, "cos"),
+    ("CNAB1", 1, "cos"), ("CNAB2", 2, "cos"), ("MCNAB2", 2, "cos"), ("CNLF2", 2, "cos")])
 def test_observed_order(scheme, order, phi):
     e1, _ = error(scheme, 40, phi=phi)
     e2, _ = error(scheme, 80, phi=phi)

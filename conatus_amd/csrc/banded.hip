@@ -404,7 +404,8 @@ void run_factor(cn_bb& h, hipStream_t s) {
         fd(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Vt.p, h.W.p, h.S.p, h.B, h.n, h.r, h.US);
         hip_check(hipGetLastError(), "k_border launch");
     }
-    hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.S.p),
+    hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + CAP_LANES - 1) / CAP_LANES), dim3(CAP_LANES), cap_lds_bytes<T>(), s,
+                       static_cast<T*>(h.S.p),
                        static_cast<int*>(h.spiv.p), static_cast<double*>(h.cond.p), static_cast<int*>(h.info.p),
                        h.B, h.r);
     hip_check(hipGetLastError(), "k_cap_lu launch");
@@ -425,7 +426,8 @@ void run_prepare_t(cn_bb& h, hipStream_t s) {
         hipLaunchKernelGGL((k_build_st<T>), dim3((l2 + 255) / 256), dim3(256), 0, s, static_cast<const T*>(h.Wt.p),
                            static_cast<T*>(h.St.p), static_cast<int*>(h.info_t.p), h.B, h.n, h.r, h.il);
         hip_check(hipGetLastError(), "k_build_st launch");
-        hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + 255) / 256), dim3(256), 0, s, static_cast<T*>(h.St.p),
+        hipLaunchKernelGGL((k_cap_lu<T>), dim3((h.B + CAP_LANES - 1) / CAP_LANES), dim3(CAP_LANES), cap_lds_bytes<T>(),
+                           s, static_cast<T*>(h.St.p),
                            static_cast<int*>(h.stpiv.p), static_cast<double*>(h.cond_t.p),
                            static_cast<int*>(h.info_t.p), h.B, h.r);
         hip_check(hipGetLastError(), "k_cap_lu (transposed) launch");

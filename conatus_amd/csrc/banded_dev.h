@@ -145,41 +145,61 @@ __device__ __forceinline__ double neg(double a) { return -a; }
 
 // One lane per system: LU of S in place (column-major, partial pivoting by |re|+|im|), and
 // cond_1(S) = |S|_1 |S^-1|_1 with the inverse from the factors (banded.hpp:403-432).
+// one lane per system; the r x r matrix, its pivots and the inverse column of each lane live in
+// LDS, entry-major ([entry][lane], CAP_LANES lanes per workgroup), not in private arrays: a
+// dynamically indexed private array is scratch memory, which the runtime backs for every wave
+// the device can hold (about 0.8 GB at 1.2 KB per lane).
+constexpr int CAP_LANES = 64;
 template <class T>
-__global__ void __launch_bounds__(256)
+__host__ __device__ constexpr size_t cap_lds_bytes() {
+    return (size_t)(MAXR * MAXR + MAXR) * CAP_LANES * sizeof(T) + (size_t)MAXR * CAP_LANES * sizeof(int);
+}
+
+template <class T>
+__global__ void __launch_bounds__(CAP_LANES)
 k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, int* __restrict__ info,
          int B, int r) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x * CAP_LANES + lane;
     if (m >= B) return;
-    T a[MAXR * MAXR];
+    T* La = reinterpret_cast<T*>(lds);                         // [MAXR * MAXR][CAP_LANES]
+    T* Lx = La + MAXR * MAXR * CAP_LANES;                      // [MAXR][CAP_LANES]
+    int* Lp = reinterpret_cast<int*>(Lx + MAXR * CAP_LANES);   // [MAXR][CAP_LANES]
+#define CAP_A(q) La[(q) * CAP_LANES + lane]
+#define CAP_X(i) Lx[(i) * CAP_LANES + lane]
+#define CAP_P(i) Lp[(i) * CAP_LANES + lane]
     T* Sm = S + (size_t)m * r * r;
-    for (int q = 0; q < r * r; ++q) a[q] = Sm[q];
+    for (int q = 0; q < r * r; ++q) CAP_A(q) = Sm[q];
     double nS = 0.0;
     for (int b = 0; b < r; ++b) {
         double s = 0.0;
-        for (int i = 0; i < r; ++i) s += absv(a[b * r + i]);
+        for (int i = 0; i < r; ++i) s += absv(CAP_A(b * r + i));
         if (!(s <= nS)) nS = s;
     }
     int bad = 0;
-    int p[MAXR];
     for (int jc = 0; jc < r; ++jc) {
         int ip = jc;
-        double best = cn::cabs1(a[jc * r + jc]);
+        double best = cn::cabs1(CAP_A(jc * r + jc));
         for (int i = jc + 1; i < r; ++i) {
-            const double v = cn::cabs1(a[jc * r + i]);
+            const double v = cn::cabs1(CAP_A(jc * r + i));
             if (v > best) { best = v; ip = i; }
         }
-        p[jc] = ip;
+        CAP_P(jc) = ip;
         if (best == 0.0) { bad = 1; continue; }
         if (ip != jc)
-            for (int q = 0; q < r; ++q) { T tmp = a[q * r + jc]; a[q * r + jc] = a[q * r + ip]; a[q * r + ip] = tmp; }
-        const T rc = cn::recip(a[jc * r + jc]);
-        for (int i = jc + 1; i < r; ++i) a[jc * r + i] = cn::mul(a[jc * r + i], rc);
+            for (int q = 0; q < r; ++q) {
+                T tmp = CAP_A(q * r + jc);
+                CAP_A(q * r + jc) = CAP_A(q * r + ip);
+                CAP_A(q * r + ip) = tmp;
+            }
+        const T rc = cn::recip(CAP_A(jc * r + jc));
+        for (int i = jc + 1; i < r; ++i) CAP_A(jc * r + i) = cn::mul(CAP_A(jc * r + i), rc);
         for (int q = jc + 1; q < r; ++q)
-            for (int i = jc + 1; i < r; ++i) a[q * r + i] = cn::fnms(a[jc * r + i], a[q * r + jc], a[q * r + i]);
+            for (int i = jc + 1; i < r; ++i) CAP_A(q * r + i) = cn::fnms(CAP_A(jc * r + i), CAP_A(q * r + jc), CAP_A(q * r + i));
     }
-    for (int q = 0; q < r * r; ++q) Sm[q] = a[q];
-    for (int q = 0; q < r; ++q) spiv[(size_t)m * r + q] = p[q];
+    for (int q = 0; q < r * r; ++q) Sm[q] = CAP_A(q);
+    for (int q = 0; q < r; ++q) spiv[(size_t)m * r + q] = CAP_P(q);
     double c = 1.0;
     if (bad) {
         if (info[m] == 0) info[m] = -1;
@@ -188,23 +208,27 @@ k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, i
         // inverse column by column: solve S x = e_b with the factors
         double ninv = 0.0;
         for (int b = 0; b < r; ++b) {
-            T x[MAXR];
-            for (int i = 0; i < r; ++i) x[i] = (i == b) ? cn::one<T>() : cn::zero<T>();
+            for (int i = 0; i < r; ++i) CAP_X(i) = (i == b) ? cn::one<T>() : cn::zero<T>();
+            for (int i = 0; i < r; ++i) {
+                const int pi = CAP_P(i);
+                if (pi != i) { T tmp = CAP_X(i); CAP_X(i) = CAP_X(pi); CAP_X(pi) = tmp; }
+            }
             for (int i = 0; i < r; ++i)
-                if (p[i] != i) { T tmp = x[i]; x[i] = x[p[i]]; x[p[i]] = tmp; }
-            for (int i = 0; i < r; ++i)
-                for (int q = 0; q < i; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
+                for (int q = 0; q < i; ++q) CAP_X(i) = cn::fnms(CAP_A(q * r + i), CAP_X(q), CAP_X(i));
             for (int i = r - 1; i >= 0; --i) {
-                for (int q = i + 1; q < r; ++q) x[i] = cn::fnms(a[q * r + i], x[q], x[i]);
-                x[i] = cn::mul(x[i], cn::recip(a[i * r + i]));
+                for (int q = i + 1; q < r; ++q) CAP_X(i) = cn::fnms(CAP_A(q * r + i), CAP_X(q), CAP_X(i));
+                CAP_X(i) = cn::mul(CAP_X(i), cn::recip(CAP_A(i * r + i)));
             }
             double s = 0.0;
-            for (int i = 0; i < r; ++i) s += absv(x[i]);
+            for (int i = 0; i < r; ++i) s += absv(CAP_X(i));
             if (!(s <= ninv)) ninv = s;
         }
         c = nS * ninv;
     }
     cond[m] = c;
+#undef CAP_A
+#undef CAP_X
+#undef CAP_P
 }
 
 // acc0 - sum_{d=1..WU} u(d) xw[d-1], the row step of the back substitution with U.  The terms

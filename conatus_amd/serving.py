@@ -66,14 +66,11 @@ class Request:
 
 
 def footprint_bytes(pb, order: int = 2) -> int:
-    """Estimated HBM bytes of one resident run: state, SBDF history (M X and F per level), the
-    explicit-term scratch and the factor storage of its K mode systems (band rows, U, L, W, V^T,
-    pivots; the band of the RB 2D and channel mode systems: kl = 12, ku = 18, r = 7)."""
-    K, n = pb.K, pb.n
-    vec = K * n * 16
-    kl, ku, r = 12, 18, 7
-    factor = K * n * 16 * ((kl + ku + 1) + (kl + ku + 1) + kl + 2 * r) + K * n * 4
-    return vec * (1 + 2 * order + 2) + pb.explicit_scratch_bytes() + factor
+    """HBM bytes of one resident run under SBDF-``order``: the device-memory plan of its solver
+    (conatus_amd.memory.plan: state, history, operators, the factor handles of every startup
+    order, explicit-term tables and workspace, the measured runtime share)."""
+    from .memory import plan
+    return plan(pb, f"SBDF{int(order)}").total
 
 
 def _params(pb) -> tuple:

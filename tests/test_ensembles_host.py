@@ -49,5 +49,22 @@ def test_channel_keys_parameters_and_footprint():
     assert discretization_key(a) == discretization_key(b) != discretization_key(c)   # G is part of F
     assert a.params() != b.params()
     big = Channel3D(Nx=16, Ny=17, Nz=16, Re=500.0)
-    assert footprint_bytes(big) > 4 * footprint_bytes(a) > 0
+    assert footprint_bytes(big) > 2 * footprint_bytes(a) > 0       # systems padded to 64 per handle
     assert footprint_bytes(a) > a.explicit_scratch_bytes() > 0
+
+
+def test_memory_plan_handle_layout():
+    """A factor handle's bytes by hand for B = 100 systems, n = 50, kl = 3 (kernel width 4),
+    ku = 5 (kl + ku = 9 padded to the kernel width 16, U rows of 17), r = 2, complex: systems
+    padded to 128."""
+    from conatus_amd.memory import handle_bytes, plan
+    from conatus_amd.problems import RayleighBenard2D
+    Bp, n, es = 128, 50, 16
+    band = Bp * n * (4 + 5 + 1) * es + Bp * n * 17 * es + Bp * n * 4 * es + Bp * n * 4
+    border = 3 * Bp * 2 * n * es + 2 * (Bp * 4 * es + Bp * 2 * 4 + Bp * 8 + Bp * 4) + n * 4
+    assert handle_bytes(100, n, 3, 5, 2) == band + border + 100 * n * es
+    pb = RayleighBenard2D(Nx=64, Nz=32, Lx=4.0, Ra=1e5, Pr=1.0)
+    p2, p3 = plan(pb, "SBDF2"), plan(pb, "SBDF3")
+    assert p3.lines["factors"] == 3 * p2.lines["factors"] // 2   # one handle per startup order
+    assert plan(pb, "RK443").lines["factors"] == p2.lines["factors"] // 2
+    assert plan(pb, "MCNAB2").lines["factors"] == p2.lines["factors"]   # CNAB1 startup + MCNAB2

@@ -25,7 +25,9 @@ print(free)
 
 
 def _measure(build):
+    import gc
     import torch
+    gc.collect()                          # buffers of earlier tests' runs released first
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     free0, _ = torch.cuda.mem_get_info()
@@ -69,8 +71,10 @@ def test_plan_matches_the_device(case):
 def test_process_floor():
     """Free memory before a fresh process minus free memory once it has loaded torch and the
     library and run one solve (this process's own buffers stay where they are meanwhile)."""
+    import gc
     import torch
     from conatus_amd.memory import PROCESS_FLOOR_BYTES
+    gc.collect()
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     free0, _ = torch.cuda.mem_get_info()

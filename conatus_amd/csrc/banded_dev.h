@@ -1102,9 +1102,9 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
 // x = y - W z are parallel over the entries of x.
 // ------------------------------------------------------------------------------------------
 // Steps per staged block in wave mode, a template parameter SP of the sweeps: 32 for few systems
-// (fewer block hand-overs per sweep), 16 when the batch fills the chip (three buffers of 16 steps
-// leave LDS for 4 one-wavefront workgroups per CU at kl = 12, ku = 18 instead of 2).
-constexpr int SPW_FEW = 32, SPW_MANY = 16;
+// (fewer block hand-overs per sweep), 8 when the batch fills the chip (three buffers of 8 steps
+// leave LDS for 6 one-wavefront workgroups per CU at kl = 12, ku = 18, n = 395, instead of 2).
+constexpr int SPW_FEW = 32, SPW_MANY = 8;
 
 // Ordering point of a one-wavefront workgroup: LDS operations of a wavefront execute in order,
 // and the staging copies are waited for explicitly (vm_wait_le), so only the compiler has to be
@@ -1354,9 +1354,9 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
 
 // group size of the backward sweep for a window of w lanes (0: step by step)
 __host__ __device__ constexpr int bwd_group(int w, int SP) {
-    return (SP >= 32 && w <= 32) ? 32 : w <= 48 ? 16 : w <= 56 ? 8 : 0;
+    return (SP >= 32 && w <= 32) ? 32 : (SP >= 16 && w <= 48) ? 16 : w <= 56 ? 8 : 0;
 }
-static_assert(SPW_FEW % 32 == 0 && SPW_MANY % 16 == 0, "the step groups divide a staged block");
+static_assert(SPW_FEW % 32 == 0 && SPW_MANY % 8 == 0, "the step groups divide a staged block");
 
 template <class T, int SP>
 __device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot,
@@ -1365,7 +1365,9 @@ __device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* 
         case 32:
             if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane);
             break;
-        case 16: wave_backward<T, 16, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 16:
+            if constexpr (SP >= 16) wave_backward<T, 16, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane);
+            break;
         case 8: wave_backward<T, 8, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
         default: wave_backward<T, 0, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
     }

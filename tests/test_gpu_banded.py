@@ -366,3 +366,25 @@ def test_auto_mode_selection(rng, monkeypatch, solve_mode):
     many = BandedBatch([A] * 1024, [0, 1])
     assert many.solve_mode == "lane"
     assert "lane per (system, rhs)" in many.explain() and "wavefront per (system, rhs)" in few.explain()
+
+
+@pytest.mark.parametrize("kl,ku,r", [(12, 18, 7), (4, 4, 2), (16, 16, 3)])
+def test_many_wide_band_systems(kl, ku, r, rng, solve_mode):
+    """Enough systems that the wave-mode sweeps take their short staged blocks (more than 512
+    wavefronts per launch: solve and border completion) at the band of the RB 2D and channel mode
+    systems; the 1e-13 backward-error bar on a sample checked against the oracle."""
+    from conatus_amd.runtime.hip import BandedBatch, to_device
+    if solve_mode != "wave":
+        pytest.skip("the wave-mode block length is what this case exercises")
+    n, B = 120, 640
+    print(f"case kl={kl} ku={ku} r={r} B={B}", flush=True)
+    base = [conditioned(rng, n, kl, ku, r, shift=8.0 + kl + ku, dtype=complex) for _ in range(8)]
+    mats = [base[k % 8] for k in range(B)]
+    bb = BandedBatch(mats, list(range(r)))
+    assert bb.solve_mode == "wave"
+    b = _rhs(rng, (B, n), np.complex128)
+    d = to_device(b)
+    bb.solve(d)
+    x = d.cpu().numpy()
+    sample = sorted(set(rng.choice(B, size=12, replace=False).tolist()) | {0, B - 1})
+    _check_against_oracle([mats[k] for k in sample], list(range(r)), b[sample], x[sample], np.complex128)

@@ -5,7 +5,9 @@
 // conversion into the stacked rows).
 //
 // Two launches (grid y: ensemble member, the members' states (K, n) one after another; for even
-// Mx, k_fc_line2 is A with the Fourier sums split into even and odd modes):
+// Mx, k_fc_line2 is A with the Fourier sums split into even and odd modes; when members x K >= 4096
+// a third launch, k_fc_zback, does A's z-backward once per mode instead of once per line, which
+// would otherwise read every member's whole state once per grid line):
 //   A  one workgroup per grid line z_j (Mz workgroups per member), everything in LDS:
 //      z-backward of the 6 factor columns of the modes (u, u_z, w, w_z, T, T_z; the z-derivative
 //      is folded into the matrix Bz_1 = T_n'(z_j)), x-backward of the 8 factors u, u_x, u_z, w,
@@ -157,9 +159,43 @@ __device__ __forceinline__ void fc_term(double (&f)[8], const zc e, const zc* Zj
 // k_fc_line for even Mx: the points x_i and x_{i+Mx/2} from one pass over the modes
 // (e^{i k kappa x_{i+Mx/2}} = (-1)^k e^{i k kappa x_i}: even-k and odd-k terms summed apart), and
 // the x-forward over i < Mx/2 of G_i + G_{i+Mx/2} (even k) or G_i - G_{i+Mx/2} (odd k).
+// z-backward of every grid line at once for large batches: one workgroup per (mode, member) reads
+// the mode's coefficients once, Zg[member][j][2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)] (the
+// same sums, in the same order, as the fused form inside k_fc_line2).
+__global__ void __launch_bounds__(256)
+k_fc_zback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
+           zc* __restrict__ Zg, int K, int n, int N, int Mz) {
+    extern __shared__ __align__(16) char lds[];
+    zc* c = reinterpret_cast<zc*>(lds);             // [3][N]
+    const int k = blockIdx.x, tid = threadIdx.x;
+    const zc* Xk = X + ((size_t)blockIdx.y * K + k) * n;
+    Zg += (size_t)blockIdx.y * Mz * 6 * K;
+    for (int q = tid; q < 3 * N; q += blockDim.x) c[q] = Xk[fidx[q]];
+    __syncthreads();
+    for (int p = tid; p < 3 * Mz; p += blockDim.x) {
+        const int f = p / Mz, j = p - f * Mz;
+        const zc* cf = c + (size_t)f * N;
+        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 8
+        for (int nn = 0; nn < N; ++nn) {
+            const zc x = cf[nn];
+            const double b0 = Bz[(size_t)nn * Mz + j], b1 = Bz[((size_t)N + nn) * Mz + j];
+            r0 = fma(b0, x.re, r0);
+            i0 = fma(b0, x.im, i0);
+            r1 = fma(b1, x.re, r1);
+            i1 = fma(b1, x.im, i1);
+        }
+        zc* z = Zg + ((size_t)j * 6 + 2 * f) * K + k;
+        z[0] = zc{r0, i0};
+        z[K] = zc{r1, i1};
+    }
+}
+
+template <bool PRE>
 __global__ void __launch_bounds__(256)
 k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
-           const zc* __restrict__ Wt, zc* __restrict__ Gh, int K, int n, int N, int Mz, int Mx, double kappa) {
+           const zc* __restrict__ Wt, zc* __restrict__ Gh, const zc* __restrict__ Zg, int K, int n, int N, int Mz,
+           int Mx, double kappa) {
     extern __shared__ __align__(16) char lds[];
     zc* Zj = reinterpret_cast<zc*>(lds);            // [6][K]
     zc* W = Zj + 6 * K;                              // [Mx]
@@ -169,29 +205,35 @@ k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
     X += (size_t)blockIdx.y * K * n;
     Gh += (size_t)blockIdx.y * 3 * K * Mz;
     for (int q = tid; q < Mx; q += blockDim.x) W[q] = Wt[q];
-    for (int q = tid; q < 2 * N; q += blockDim.x) {
-        const int v = q / N, nn = q - v * N;
-        b[q] = Bz[((size_t)v * N + nn) * Mz + j];
-    }
-    __syncthreads();
-    for (int p = tid; p < 3 * K; p += blockDim.x) {
-        const int f = p / K, k = p - f * K;
-        const zc* Xk = X + (size_t)k * n;
-        const int* pos = fidx + (size_t)f * N;
-        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
-#pragma unroll 8
-        for (int nn = 0; nn < N; ++nn) {
-            const zc x = Xk[pos[nn]];
-            const double b0 = b[nn], b1 = b[N + nn];
-            r0 = fma(b0, x.re, r0);
-            i0 = fma(b0, x.im, i0);
-            r1 = fma(b1, x.re, r1);
-            i1 = fma(b1, x.im, i1);
+    if constexpr (PRE) {
+        const zc* z = Zg + ((size_t)blockIdx.y * Mz + j) * 6 * K;
+        for (int q = tid; q < 6 * K; q += blockDim.x) Zj[q] = z[q];
+        __syncthreads();
+    } else {
+        for (int q = tid; q < 2 * N; q += blockDim.x) {
+            const int v = q / N, nn = q - v * N;
+            b[q] = Bz[((size_t)v * N + nn) * Mz + j];
         }
-        Zj[(2 * f) * K + k] = zc{r0, i0};
-        Zj[(2 * f + 1) * K + k] = zc{r1, i1};
+        __syncthreads();
+        for (int p = tid; p < 3 * K; p += blockDim.x) {
+            const int f = p / K, k = p - f * K;
+            const zc* Xk = X + (size_t)k * n;
+            const int* pos = fidx + (size_t)f * N;
+            double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll 8
+            for (int nn = 0; nn < N; ++nn) {
+                const zc x = Xk[pos[nn]];
+                const double b0 = b[nn], b1 = b[N + nn];
+                r0 = fma(b0, x.re, r0);
+                i0 = fma(b0, x.im, i0);
+                r1 = fma(b1, x.re, r1);
+                i1 = fma(b1, x.im, i1);
+            }
+            Zj[(2 * f) * K + k] = zc{r0, i0};
+            Zj[(2 * f + 1) * K + k] = zc{r1, i1};
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int i = tid; i < Hx; i += blockDim.x) {
         double ev[8], od[8];
 #pragma unroll
@@ -300,9 +342,20 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
         return CN_EUNSUPPORTED;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (Mx % 2 == 0)
-        hipLaunchKernelGGL(k_fc_line2, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
-                           static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
+    // many modes: the z-backward of each mode once (k_fc_zback) into the scratch after the line
+    // products, then the lines read their values; few modes: fused into the line kernel
+    zc* Zg = static_cast<zc*>(Gh) + (size_t)members * 3 * K * Mz;
+    const bool pre = Mx % 2 == 0 && (long long)members * K >= 4096 && (size_t)3 * Nz * sizeof(zc) <= 64 * 1024;
+    if (pre) {
+        hipLaunchKernelGGL(k_fc_zback, dim3(K, members), dim3(256), (size_t)3 * Nz * sizeof(zc), s,
+                           static_cast<const zc*>(X), fidx, Bz, Zg, K, n, Nz, Mz);
+        hipLaunchKernelGGL(k_fc_line2<true>, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx,
+                           Bz, static_cast<const zc*>(Wx), static_cast<zc*>(Gh), static_cast<const zc*>(Zg), K, n, Nz,
+                           Mz, Mx, kappa);
+    } else if (Mx % 2 == 0)
+        hipLaunchKernelGGL(k_fc_line2<false>, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx,
+                           Bz, static_cast<const zc*>(Wx), static_cast<zc*>(Gh), static_cast<const zc*>(Zg), K, n, Nz,
+                           Mz, Mx, kappa);
     else
         hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
                            static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);

@@ -191,7 +191,7 @@ class RayleighBenard2D:
         return f"RB 2D {self.Nx}x{self.Nz}, Ra={self.Ra:g}"
 
     def explicit_scratch_bytes(self) -> int:
-        return 3 * self.K * self.Mz * 16
+        return 9 * self.K * self.Mz * 16
 
     def identity(self) -> dict:
         """What a checkpoint of this problem must match on restart."""

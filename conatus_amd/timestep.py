@@ -42,7 +42,8 @@ class RB2DExplicit:
         self.Wx = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mx) / Mx)).to(dev)        # twiddles
         Az = (pb.S02 @ S.cheb_forward_matrix(N, Mz)).T
         self.Az = torch.from_numpy(np.ascontiguousarray(Az)).to(dev)                       # (Mz, N)
-        self.Gh = torch.empty(self.members * 3 * K * Mz, dtype=torch.complex128, device=dev)   # scratch
+        # scratch: 3 K Mz line products and 6 K Mz line values per member
+        self.Gh = torch.empty(self.members * 9 * K * Mz, dtype=torch.complex128, device=dev)
         self._base = torch.from_numpy(np.tile(pb.b0().ravel(), self.members)).to(dev)
         fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
         self.fidx = torch.from_numpy(np.ascontiguousarray(fidx)).to(dev)

@@ -175,14 +175,14 @@ int cn_pointwise(int dtype, int64_t N, void* out, const void* x, const void* g, 
  * pipeline FourierChebyshev2DRHS.evaluate (python/conatus/timestep/fc2d.py:567-643) and the fused
  * product evaluation (python/conatus/timestep/fused.py:994-1085).  Layout: coefficient m of field f (u, w, T) of
  * mode k at X[k*n + fidx[f*Nz + m]] (fidx: 3 x Nz device ints), equation e rows at F[k*n + R + 4*m + e]; other rows of F are left
- * untouched.  Scratch: Gh (3 K Mz complex).  Tables: Bz (2 x Nz x Mz: T_n, T_n' at the nodes),
+ * untouched.  Scratch: Gh (9 K Mz complex: 3 K Mz line products, 6 K Mz line values).  Tables: Bz (2 x Nz x Mz: T_n, T_n' at the nodes),
  * Wx (Mx complex twiddles exp(2 pi i m / Mx)), Az (Mz x Nz: (S02 Fz)^T).  Needs
  * (6 K + Mx) 16 + (3 Mx + 2 Nz) 8 <= 65536 bytes of LDS (CN_EUNSUPPORTED otherwise). */
 int cn_rb2d_explicit(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,
                      const double* Az, int K, int Nz, int Mx, int Mz, int n, int R, double kappa, void* stream);
 
 /* The same for `members` states of one discretization stored one after another (X and F:
- * members x K x n, Gh: members x 3 K Mz): the explicit term of an ensemble in one launch pair
+ * members x K x n, Gh: members x 9 K Mz): the explicit term of an ensemble in one launch pair
  * (the reference evaluates an ensemble's F through BatchedRHS, python/conatus/ensembles/batched.py:116-440).
  * 1 <= members <= 65535. */
 int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, const double* Bz, const void* Wx,

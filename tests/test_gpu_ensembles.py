@@ -89,3 +89,19 @@ def test_uneven_set_in_one_graph_equals_single_runs():
         gi, _ = es.where[i]
         ref = _single(pb, X0[i], 18, 1e-3, es.groups[gi].mode, graph=False)
         assert np.array_equal(got[i], ref), (i, np.max(abs(got[i] - ref)))
+
+
+def test_large_ensemble_members_equal_single_runs():
+    """130 members x 32 modes: the explicit term takes its separate z-backward launch (one pass per
+    mode instead of one per grid line); members stay bit-identical to their single runs."""
+    from conatus_amd.ensembles import Ensemble
+    pb = _pb(64, 16, 5e4)
+    P = 130
+    e = Ensemble.from_problems([pb] * P, "SBDF2", 1e-3, mode="wave")
+    X0 = np.stack([pb.initial_state(seed=200 + b) for b in range(P)])
+    e.set_states(X0)
+    e.run(6)
+    got = e.states()
+    for b in (0, 77, P - 1):
+        ref = _single(pb, X0[b], 6, 1e-3, "wave", graph=False)
+        assert np.array_equal(got[b], ref), (b, np.max(abs(got[b] - ref)))

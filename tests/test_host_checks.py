@@ -41,3 +41,10 @@ def test_rhs_shape_check():
 
 def test_threads_rule():
     assert threads_for(64, 262, 8) == 4
+
+
+def test_empty_batch_is_refused_before_the_library():
+    # runtime/banded.py:174-176: an empty list is a type error of the call, raised before the core
+    from conatus_amd.runtime.banded import BorderedBandedBatch
+    with pytest.raises(InputTypeError, match="no matrix"):
+        BorderedBandedBatch([], [])

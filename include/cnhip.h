@@ -80,8 +80,8 @@ int cn_bb_create(cn_bb_t* out, int dtype, int batch, int n,
                  const void* const* data, const int64_t* nnz,
                  const int32_t* border, int r, int pivot, int col_border);
 
-/* Re-run the device factorization from the stored unfactored layout (the factor kernel alone,
- * for timing and for re-use after cn_bb_set_values).  Asynchronous on `stream` when check = 0;
+/* Re-run the device factorization from the stored unfactored layout (the factor kernels alone,
+ * for timing them; the stored matrices do not change).  Asynchronous on `stream` when check = 0;
  * check = 1 synchronises and reports singular systems like cn_bb_create. */
 int cn_bb_factor(cn_bb_t h, void* stream, int check);
 

@@ -1,6 +1,6 @@
 """GPU parity of the gfx950 bordered-banded factor/solve against the CPU oracle (oracle/banded.py).
 
-Bar: normwise backward error <= 1e-13 (north-star tolerance for the solve), the residual within
+Bar: normwise backward error <= 1e-13, the residual within
 10x the oracle's own residual plus n eps (the reference's cross-backend rule,
 tests/cross_backend/test_banded_level.py:71-76), solutions within a conditioning-scaled distance of
 the oracle's, fused gather/scatter and split-complex maps exact, bitwise determinism.  Every test

@@ -1,6 +1,6 @@
 """GPU parity of the 2D Rayleigh-Benard IMEX step against the CPU oracle (oracle/rb2d.py).
 
-Bar (north star): the state after k steps within 1e-11 relative of the FP64 CPU oracle; the
+Bar: the state after k steps within 1e-11 relative of the FP64 CPU oracle; the
 explicit term within 1e-13 of the oracle's FFT-based evaluation; graph replays equal eager steps
 bit for bit; the linear growth rate of the onset roll at Ra = 2500, k = pi, Pr = 1 equals the
 reference's validation value 5.5425059 (python/conatus/problems/convection.py:38-40) to the

@@ -5,8 +5,8 @@ python/conatus/io/checkpoint.py:1-118): a ``meta`` JSON record (scheme, dt, iter
 length, what the state is a state of) and the arrays ``X`` (state), ``MX<i>`` and ``F<i>``
 (multistep history, newest first: M X^{n-i} and F(X^{n-i})).  This module writes the same keys
 in the reference's NumPy format (``.npz``, written to a temporary file and renamed, so that a job
-killed while writing never leaves a truncated checkpoint); the state is this build's (K, n) mode
-layout.  The HDF5 variant needs h5py, which is not available here: ``.h5`` paths are refused.
+killed while writing never leaves a truncated checkpoint); the state is stored in this package's (K, n)
+mode layout.  The HDF5 variant needs h5py, which is not installed: ``.h5`` paths are refused.
 
 Restart compares the numerics (scheme, dt, pivoting) and the identity of the problem (grid,
 walls, dealiasing, Ra, Pr) and refuses a mismatch; the arrays are restored without conversion,

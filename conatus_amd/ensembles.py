@@ -42,6 +42,20 @@ def discretization_key(pb) -> tuple:
     return pb.discretization_key()
 
 
+def _member_indices(member_of, nsolvers: int) -> np.ndarray:
+    """The solver index of every member: whole numbers (bools are not indices) in [0, nsolvers)."""
+    vals = list(member_of) if not isinstance(member_of, np.ndarray) else member_of.tolist()
+    if isinstance(member_of, np.ndarray) and member_of.ndim != 1:
+        vals = None
+    ok = vals is not None and all(isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_))
+                                  and 0 <= int(v) < nsolvers for v in vals)
+    if not ok:
+        raise OptionError(f"Ensemble.from_solvers: member_of={member_of!r} is not a list of solver indices "
+                          f"(whole numbers from 0 to {nsolvers - 1})",
+                          fix="member_of=[0, 0, 1, 1]: members 0 and 1 use solvers[0], 2 and 3 solvers[1]")
+    return np.asarray([int(v) for v in vals], dtype=int)
+
+
 class _LoopExplicit:
     """F member by member (the reference's rhs='loop' path): one single-member launch pair per
     member on its slice of the stacked state; the same kernels as the batched path."""
@@ -75,16 +89,7 @@ class Ensemble:
         """Parameter ensemble: member b uses ``solvers[member_of[b]]`` (default: one member per
         solver).  The solvers must share the discretization, scheme, dt, pivoting and kernel mode."""
         self = cls.__new__(cls)
-        if member_of is None:
-            mo = np.arange(len(solvers))
-        else:
-            raw = np.asarray(member_of)
-            mo = raw.astype(int)
-            if raw.dtype == bool or not np.array_equal(mo, raw) or raw.ndim != 1 \
-                    or (mo.size and (mo.min() < 0 or mo.max() >= len(solvers))):
-                raise OptionError(f"Ensemble.from_solvers: member_of={member_of!r} must be a 1-D "
-                                  f"sequence of integer indices in [0, {len(solvers)})",
-                                  fix="member_of=[0, 0, 1, 1], for example")
+        mo = np.arange(len(solvers)) if member_of is None else _member_indices(member_of, len(solvers))
         self._setup(list(solvers), mo, rhs)
         return self
 

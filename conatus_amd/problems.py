@@ -103,6 +103,20 @@ class RayleighBenard2D:
     def equation_rows(self, e) -> np.ndarray:
         return NBORDER + 4 * np.arange(self.N) + e
 
+    field_names = ("u", "w", "T", "p")
+    tau_names = TAUS
+
+    def unknown_name(self, j: int) -> str:
+        """Name of unknown j of a mode's state vector: a field coefficient ('T[5]') or a tau."""
+        for f, name in enumerate(self.field_names):
+            hit = np.flatnonzero(self.col(np.arange(self.N), f) == j)
+            if hit.size:
+                return f"{name}[{int(hit[0])}]"
+        for t in self.tau_names:
+            if self.tau_col(t) == j:
+                return t
+        return f"unknown {j}"
+
     # -- operators ---------------------------------------------------------------------------------
     def mode_matrices(self, k: int):
         """(M, L) of mode k, complex CSR of size n; the border rows are rows 0..6."""
@@ -320,6 +334,9 @@ class Channel3D:
     row = RayleighBenard2D.row
     field_index = RayleighBenard2D.field_index
     equation_rows = RayleighBenard2D.equation_rows
+    unknown_name = RayleighBenard2D.unknown_name
+    field_names = ("u", "v", "w", "p")
+    tau_names = CH_TAUS
 
     def tau_col(self, name):
         if name == "tp":

@@ -15,6 +15,7 @@ import scipy.sparse as sp
 
 from ..errors import BackendError, OptionError
 from .banded import BatchHandle, _factor_dtype, csr_args
+from .checks import check_index_map, check_io_overlap
 
 __all__ = ["available", "device_name", "synchronize", "to_device", "BandedBatch"]
 
@@ -150,39 +151,18 @@ class BandedBatch:
 
     def solve_mapped(self, rhs, rows, x, cols) -> None:
         """Out-of-place solve with the gather/scatter fused in the kernel: b_m[i] = rhs[rows[m, i]],
-        x[cols[m, i]] = x_m[i]; ``rows``/``cols`` int64 (batch, n) device tensors or None.
-        Checks as python/conatus/runtime/cuda.py:186-232."""
-        torch = _torch()
-        for name, a in (("rhs", rhs), ("x", x)):
-            self._check_tensor(a, name)
-        for name, a, full in (("rows", rows, rhs), ("cols", cols, x)):
-            if a is None:
-                if full.numel() < self.batch * self.n:
-                    raise ValueError(f"BandedBatch.solve_mapped: {name}=None needs at least "
-                                     f"batch * n = {self.batch * self.n} entries")
-            elif a.dtype != torch.int64 or tuple(a.shape) != (self.batch, self.n) or not a.is_contiguous():
-                raise ValueError(f"BandedBatch.solve_mapped: {name} must be a contiguous int64 "
-                                 f"({self.batch}, {self.n}) tensor (got {a.dtype} {tuple(a.shape)})")
-            elif a.numel():
-                lo, hi = int(a.min()), int(a.max())
-                if lo < 0 or hi >= full.numel():
-                    raise ValueError(f"BandedBatch.solve_mapped: {name} holds indices out of range "
-                                     f"[{lo}, {hi}] for an array of {full.numel()} entries")
-                if name == "cols" and int(torch.unique(a).numel()) != a.numel():
-                    raise ValueError("BandedBatch.solve_mapped: cols has repeated indices (two "
-                                     "systems would write one entry of x)")
-        r0, x0 = rhs.data_ptr(), x.data_ptr()
-        rb, xb = rhs.numel() * rhs.element_size(), x.numel() * x.element_size()
-        if r0 < x0 + xb and x0 < r0 + rb:
-            same = r0 == x0 and rb == xb and (
-                (rows is None and cols is None)
-                or (rows is not None and cols is not None and bool(torch.equal(rows, cols))))
-            if not same:
-                raise ValueError("BandedBatch.solve_mapped: rhs and x overlap in memory with different "
-                                 "maps (systems would read entries others already wrote); pass a "
-                                 "separate x, or the same map for rows and cols")
-        p = lambda a: 0 if a is None else a.data_ptr()   # noqa: E731
-        self._h.solve_mapped_ptr(p(rhs), p(rows), p(x), p(cols), _stream())
+        x[cols[m, i]] = x_m[i]; ``rows``/``cols`` int64 (batch, n) device tensors or None (the
+        contiguous layout).  The kernel reads raw pointers, so the operands, the maps' ranges, a
+        repeated scatter index and overlapping rhs/x are refused here (the refusals of
+        python/conatus/runtime/cuda.py:186-232)."""
+        where = "BandedBatch.solve_mapped"
+        self._check_tensor(rhs, "rhs")
+        self._check_tensor(x, "x")
+        check_index_map("rows", rows, rhs.numel(), self.batch, self.n, where)
+        check_index_map("cols", cols, x.numel(), self.batch, self.n, where, unique=True)
+        check_io_overlap(rhs, rows, x, cols, where)
+        self._h.solve_mapped_ptr(rhs.data_ptr(), 0 if rows is None else rows.data_ptr(), x.data_ptr(),
+                                 0 if cols is None else cols.data_ptr(), _stream())
 
     def solve_mapped_ptr(self, rhs: int, rows: int, x: int, cols: int, stream: int = 0) -> None:
         if not rhs or not x:
@@ -190,20 +170,21 @@ class BandedBatch:
         self._h.solve_mapped_ptr(rhs, rows, x, cols, stream)
 
     def solve_split(self, re, rows_c, rows_s, x, cols_c, cols_s) -> None:
-        """Complex systems of real data: b = re[rows_c] + i re[rows_s], x[cols_c] = Re, x[cols_s] = Im."""
+        """Complex systems of real data: b = re[rows_c] + i re[rows_s], x[cols_c] = Re, x[cols_s] = Im
+        (four int64 (batch, n) maps)."""
         torch = _torch()
+        where = "BandedBatch.solve_split"
         if self.dtype != np.complex128:
-            raise OptionError("BandedBatch.solve_split: complex128 factors required")
+            raise OptionError(f"{where}: complex128 factors required")
         for name, a in (("re", re), ("x", x)):
-            if a.dtype != torch.float64 or not a.is_contiguous() or a.device.type != "cuda":
-                raise ValueError(f"BandedBatch.solve_split: {name} must be a contiguous float64 device tensor")
-        for name, a, full in (("rows_c", rows_c, re), ("rows_s", rows_s, re), ("cols_c", cols_c, x),
-                              ("cols_s", cols_s, x)):
-            if a.dtype != torch.int64 or tuple(a.shape) != (self.batch, self.n) or not a.is_contiguous():
-                raise ValueError(f"BandedBatch.solve_split: {name} must be a contiguous int64 "
-                                 f"({self.batch}, {self.n}) tensor")
-            if a.numel() and (int(a.min()) < 0 or int(a.max()) >= full.numel()):
-                raise ValueError(f"BandedBatch.solve_split: {name} holds indices out of range")
+            if not (isinstance(a, torch.Tensor) and a.device.type == "cuda" and a.dtype == torch.float64
+                    and a.is_contiguous()):
+                raise ValueError(f"{where}: {name} must be a contiguous float64 device tensor")
+        for name, idx, target in (("rows_c", rows_c, re), ("rows_s", rows_s, re), ("cols_c", cols_c, x),
+                                  ("cols_s", cols_s, x)):
+            if idx is None:
+                raise ValueError(f"{where}: {name} is required (an int64 ({self.batch}, {self.n}) tensor)")
+            check_index_map(name, idx, target.numel(), self.batch, self.n, where)
         self._h.solve_split_ptr(re.data_ptr(), rows_c.data_ptr(), rows_s.data_ptr(), x.data_ptr(),
                                 cols_c.data_ptr(), cols_s.data_ptr(), _stream())
 

@@ -388,8 +388,18 @@ class Imex:
                                   "or pass the returned (x, t) to continue")
 
     def _remember(self, d, t) -> None:
-        if self.order:
+        """Keep the (state, time) a call returned, which the next call must continue from while
+        a multistep history (SBDF levels, or the CN family's M X, F, L X levels) is held."""
+        if self.order or self.cn is not None:
             self._last = (d.clone(), float(t))
+        else:
+            self._last = None
+
+    def _next_t(self, t: float, clock) -> float:
+        """The time after the step just committed: ``clock(iteration)`` when the owner keeps a
+        time origin (t = t_origin + (iteration - it_origin) dt, no accumulated round-off), else
+        one addition of dt per step, as imex.hpp:112-116 does."""
+        return float(clock(self.iteration)) if clock is not None else t + self.dt
 
     def adopt_history(self, sources: Sequence, d, t: float) -> None:
         """Continue the trajectories of other integrators' systems: ``sources`` is a list of
@@ -417,11 +427,13 @@ class Imex:
         self.iteration = max(src.iteration for src, _ in sources)
         self._remember(d, t)
 
-    def run(self, d, t0: float, nsteps: int) -> float:
+    def run(self, d, t0: float, nsteps: int, clock=None) -> float:
         """Advance the (batch, n) device tensor ``d`` in place by ``nsteps``; returns the time
-        t0 + nsteps dt computed as the reference does (one addition per step, imex.hpp:112-116)."""
+        after the last step: t0 plus one addition of dt per step, as the reference's integrator
+        computes it (imex.hpp:112-116), or ``clock(iteration)`` when given (the IVP solver's
+        t = t_origin + (iteration - it_origin) dt, integrator.py:680)."""
         self._check_state(d)
-        if int(nsteps) != nsteps or nsteps < 0:
+        if isinstance(nsteps, bool) or int(nsteps) != nsteps or nsteps < 0:
             raise OptionError("Imex.run: nsteps must be a whole number >= 0")
         self._continuation(d, t0)
         t = float(t0)
@@ -432,21 +444,21 @@ class Imex:
                 self._cn_step(d)
             else:
                 self._rk_step(d)
-            t += self.dt
             self.iteration += 1
+            t = self._next_t(t, clock)
         self._remember(d, t)
         return t
 
-    def run_graph(self, d, t0: float, nsteps: int, chunk: int = 8) -> float:
+    def run_graph(self, d, t0: float, nsteps: int, chunk: int = 8, clock=None) -> float:
         """As ``run``, with every full-order step replayed from captured HIP graphs.  Whole chunks
         of steps (``chunk`` rounded up to a multiple of the SBDF ring period) are one graph each,
         so that the host launches one graph per chunk; the remaining steps use one graph per ring
         phase (one graph for RK).  Startup steps run eagerly; CN-family schemes step eagerly."""
         if self.cn is not None:
-            return self.run(d, t0, nsteps)
+            return self.run(d, t0, nsteps, clock)
         self._check_state(d)
         self._continuation(d, t0)
-        t, done = self._graph_startup(d, float(t0), nsteps)
+        t, done = self._graph_startup(d, float(t0), nsteps, clock)
         period = max(self.order, 1)
         chunk = period * max(1, -(-int(chunk) // period))
         while done < nsteps:
@@ -459,7 +471,7 @@ class Imex:
                 g.replay()
                 for _ in range(chunk):
                     self._graph_step(d)[2]()
-                    t += self.dt
+                    t = self._next_t(t, clock)
                 done += chunk
                 continue
             gk, fn, commit = self._graph_step(d)
@@ -469,7 +481,7 @@ class Imex:
                 self._graphs[gk] = g
             g.replay()
             commit()
-            t += self.dt
+            t = self._next_t(t, clock)
             done += 1
         self._remember(d, t)
         return t
@@ -506,15 +518,15 @@ class Imex:
         commit()
 
     # -- graph steps (shared with ensemble sets that capture several integrators in one graph) --
-    def _graph_startup(self, d, t: float, nsteps: int):
+    def _graph_startup(self, d, t: float, nsteps: int, clock=None):
         """Eager lower-order startup steps (SBDF), and every factorization the full-order steps use
         built before any capture; returns (t, steps done)."""
         done = 0
         if self.order:
             while self.history_length < self.order - 1 and done < nsteps:
                 self._sbdf_step(d)
-                t += self.dt
                 self.iteration += 1
+                t = self._next_t(t, clock)
                 done += 1
             if done < nsteps:
                 a, _ = sbdf_coefficients(self.order)

@@ -282,6 +282,10 @@ class Server:
             else:
                 pg.ens._flat()[s * K:(s + 1) * K].copy_(torch.from_numpy(np.ascontiguousarray(q.X0)).to("cuda"))
             pg.set_factors(s, q, self._runfac)
+        if src is not None:
+            # the startup page may be reset and refilled in this same tick (admission writes its
+            # slots on its own stream): that work waits until the copies above have read them
+            src.stream.wait_stream(pg.stream)
         pg.members[s] = q
         self.joins += 1
 

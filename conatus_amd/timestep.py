@@ -16,6 +16,7 @@ import numpy as np
 from . import _lib
 from . import spectral as S
 from .problems import NBORDER, Channel3D, RayleighBenard2D
+from .errors import NumericalError, OptionError
 from .runtime.imex import Imex
 
 __all__ = ["RB2DExplicit", "Channel3DExplicit", "IVPSolver"]
@@ -106,12 +107,39 @@ def explicit_term(problem):
     raise TypeError(f"no device explicit term for {type(problem).__name__}")
 
 
-class IVPSolver:
-    """Device IVP solver of a Fourier x Chebyshev problem (2D Rayleigh-Benard, 3D channel)."""
+#: Steps between two finite-state checks of an IVP run (the reference's FINITE_CHECK_EVERY,
+#: python/conatus/timestep/run_checks.py:14-20): one device reduction and host synchronisation.
+FINITE_CHECK_EVERY = 100
 
-    def __init__(self, problem, scheme: str, dt: float, pivot: bool = True, mode: str | None = None):
+
+def finite_cadence(check_finite, where: str) -> int:
+    """True -> every FINITE_CHECK_EVERY steps, False -> off (0), a positive int -> that cadence."""
+    if check_finite is True:
+        return FINITE_CHECK_EVERY
+    if check_finite is False:
+        return 0
+    if isinstance(check_finite, (int, np.integer)) and not isinstance(check_finite, bool) and check_finite > 0:
+        return int(check_finite)
+    raise OptionError(f"{where}: check_finite={check_finite!r}",
+                      why=f"the check is on (True: every {FINITE_CHECK_EVERY} steps), off (False) or "
+                          "every n steps (n > 0)",
+                      fix="check_finite=True (default), check_finite=10 or check_finite=False")
+
+
+class IVPSolver:
+    """Device IVP solver of a Fourier x Chebyshev problem (2D Rayleigh-Benard, 3D channel).
+
+    The clock follows the reference's rule t = t_origin + (iteration - it_origin) dt
+    (integrator.py:680): the origin is the (t, iteration) of ``set_state`` or of a restart, so no
+    round-off accumulates over a long run.  ``check_finite`` (default every 100 steps) tests the
+    state for NaN/inf at multiples of the cadence and at the end of every ``run``/``run_graph``
+    call and raises ``NumericalError`` naming the iteration, the time and the unknown."""
+
+    def __init__(self, problem, scheme: str, dt: float, pivot: bool = True, mode: str | None = None,
+                 check_finite=True):
         torch = _torch()
         self.problem = problem
+        self._finite_every = finite_cadence(check_finite, f"IVPSolver(problem, {scheme!r}, ...)")
         self.explicit = explicit_term(problem)
         Ms = [m[0] for m in problem.mats]
         Ls = [m[1] for m in problem.mats]
@@ -119,31 +147,83 @@ class IVPSolver:
                          explicit=self.explicit, mode=mode)
         self.pivot, self.mode = pivot, mode
         self.X = torch.zeros((problem.K, problem.n), dtype=torch.complex128, device="cuda")
-        self.t = 0.0
         self.scheme, self.dt = scheme, float(dt)
+        self.t = 0.0
+        self.t_origin, self.it_origin = 0.0, 0
+        self._finite_ok = None
 
     @property
     def iteration(self) -> int:
         return self.imex.iteration
 
+    def _clock(self, iteration: int) -> float:
+        return self.t_origin + (iteration - self.it_origin) * self.dt
+
     def set_state(self, X: np.ndarray, t: float = 0.0) -> None:
+        """Start a new trajectory from the (K, n) state X at time t (history and iteration count
+        reset; the time origin becomes (t, 0))."""
         self.X.copy_(_torch().from_numpy(np.ascontiguousarray(X, dtype=np.complex128)))
-        self.t = float(t)
         self.imex.reset()
+        self.t = float(t)
+        self.t_origin, self.it_origin = float(t), 0
+        self._finite_ok = None
 
     def state(self) -> np.ndarray:
         return self.X.cpu().numpy()
 
+    def _steps_arg(self, steps, where: str) -> int:
+        if isinstance(steps, bool) or not isinstance(steps, (int, np.integer)) or steps < 0:
+            raise OptionError(f"{where}: steps={steps!r} is not a number of steps",
+                              fix="an integer >= 0, e.g. steps=100")
+        return int(steps)
+
     def step(self) -> None:
-        self.t = self.imex.run(self.X, self.t, 1)
+        self.t = self.imex.run(self.X, self.t, 1, self._clock)
+        if self._finite_every and self.iteration % self._finite_every == 0:
+            self.check_finite("IVPSolver.step")
 
     def run(self, steps: int) -> float:
-        self.t = self.imex.run(self.X, self.t, steps)
+        """Advance ``steps`` steps; the state is checked finite at every multiple of the cadence
+        and at the end of the call."""
+        steps = self._steps_arg(steps, "IVPSolver.run")
+        fe, left = self._finite_every, steps
+        while left:
+            k = min(left, fe - self.iteration % fe) if fe else left
+            self.t = self.imex.run(self.X, self.t, k, self._clock)
+            left -= k
+            if fe and self.iteration % fe == 0 and left:
+                self.check_finite("IVPSolver.run")
+        if fe and steps:
+            self.check_finite("IVPSolver.run (end of the call)")
         return self.t
 
     def run_graph(self, steps: int) -> float:
-        self.t = self.imex.run_graph(self.X, self.t, steps)
+        """As ``run``, full-order steps replayed from captured HIP graphs; one finite-state check
+        at the end of the call (no host synchronisation between replays)."""
+        steps = self._steps_arg(steps, "IVPSolver.run_graph")
+        self.t = self.imex.run_graph(self.X, self.t, steps, clock=self._clock)
+        if self._finite_every and steps:
+            self.check_finite("IVPSolver.run_graph (end of the call)")
         return self.t
+
+    def check_finite(self, where: str = "IVPSolver.check_finite") -> None:
+        """Raise NumericalError if the state holds a NaN or an inf (one device reduction):
+        names the iteration, the time, the first non-finite unknown and the last iteration at
+        which the state was finite (python/conatus/timestep/run_checks.py:64-97)."""
+        torch = _torch()
+        if bool(torch.isfinite(self.X).all()):
+            self._finite_ok = (self.iteration, self.t)
+            return
+        bad = ~torch.isfinite(self.X).cpu().numpy()
+        name, count = self.problem.unknown_name(int(np.flatnonzero(bad.any(axis=0))[0])), int(bad.sum())
+        last = self._finite_ok
+        since = f"; last checked finite at iteration {last[0]} (t = {last[1]:.10g})" if last else ""
+        raise NumericalError(
+            f"non-finite state (NaN or inf) at iteration {self.iteration}, t = {self.t:.10g}: "
+            f"unknown {name!r} ({count} non-finite coefficient(s)){since}", where=where,
+            why="the run has blown up (a numerical or physical instability the resolution cannot follow)",
+            fix=f"reduce dt (now {self.dt:g}), raise the resolution, or check the initial state; "
+                "check_finite=False turns this check off")
 
     def save_checkpoint(self, path):
         """State + SBDF history + clock (``conatus.checkpoint/1``, ``.npz``): a restart with

@@ -63,7 +63,7 @@ def test_trajectory_matches_oracle(Nx, Nz, dt, steps):
     r = rel(Xg, Xo)
     print(f"relative difference after {steps} steps: {r:.3e}", flush=True)
     assert r < 1e-11
-    assert abs(s.t - steps * dt) < 1e-12
+    assert s.t == steps * dt             # t = t_origin + (iteration - it_origin) dt, no accumulation
 
 
 @pytest.mark.parametrize("scheme", ["RK443", "RK222", "RKSMR", "RKGFY", "CNAB2", "MCNAB2", "CNLF2"])
@@ -90,6 +90,55 @@ def test_rk_trajectory_matches_oracle(scheme):
     g.set_state(X0)
     g.run_graph(15)
     assert np.array_equal(g.state(), s.state())
+
+
+@pytest.mark.parametrize("scheme", ["CNAB1", "CNAB2", "MCNAB2", "CNLF2", "SBDF3", "RK443"])
+def test_continued_calls_equal_one_call(scheme):
+    """A run split over several calls (run(10) + run(5), fifteen step() calls, run_graph pieces)
+    is the one-call run bit for bit, the CN family included (its L X history is kept between
+    calls), and the clock is exactly t_origin + iteration dt."""
+    from conatus_amd.timestep import IVPSolver
+    pb = RayleighBenard2D(Nx=32, Nz=16, Lx=4.0, Ra=1e5, Pr=1.0)
+    X0 = pb.initial_state(seed=4)
+    one = IVPSolver(pb, scheme, 1e-3)
+    one.set_state(X0, t=0.25)
+    one.run(15)
+    two = IVPSolver(pb, scheme, 1e-3)
+    two.set_state(X0, t=0.25)
+    two.run(10)
+    two.run(5)
+    each = IVPSolver(pb, scheme, 1e-3)
+    each.set_state(X0, t=0.25)
+    for _ in range(15):
+        each.step()
+    graph = IVPSolver(pb, scheme, 1e-3)
+    graph.set_state(X0, t=0.25)
+    graph.run_graph(7)
+    graph.run_graph(8)
+    for s in (two, each, graph):
+        assert np.array_equal(s.state(), one.state())
+        assert s.t == one.t == 0.25 + 15 * 1e-3 and s.iteration == 15
+
+
+def test_finite_state_check_names_the_step():
+    """A state that holds a NaN stops the run with NumericalError at the next check, naming the
+    iteration and the unknown (python/conatus/timestep/run_checks.py:64-97)."""
+    import torch
+    from conatus_amd.errors import NumericalError
+    from conatus_amd.timestep import IVPSolver
+    pb = RayleighBenard2D(Nx=16, Nz=12, Lx=4.0, Ra=1e5, Pr=1.0)
+    s = IVPSolver(pb, "SBDF2", 1e-3, check_finite=5)
+    s.set_state(pb.initial_state(seed=1))
+    s.run(5)
+    s.X[3, pb.field_index(2)[4]] = complex(float("nan"), 0.0)
+    torch.cuda.synchronize()
+    with pytest.raises(NumericalError, match="iteration 10"):
+        s.run(7)
+    assert s.iteration == 10
+    off = IVPSolver(pb, "SBDF2", 1e-3, check_finite=False)
+    off.set_state(pb.initial_state(seed=1))
+    off.X[0, 0] = complex(float("inf"), 0.0)
+    off.run(2)                            # no check: the run goes on
 
 
 def test_graph_replay_equals_eager_steps():

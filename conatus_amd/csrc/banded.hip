@@ -51,6 +51,7 @@ using cn::zc;
 
 using namespace cnk;
 
+
 namespace {
 
 thread_local std::string g_err;
@@ -277,6 +278,7 @@ struct cn_bb {
     DevBuf Wt, St, stpiv, cond_t, info_t;   // transposed Woodbury data (built on demand)
     bool tr_ready = false;
     std::vector<double> hcond;
+    std::vector<int> hperm;   // host copy of the row permutation (border rows first)
 };
 
 namespace {
@@ -595,6 +597,7 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     hip_check(hipMemcpy(h.Ab.p, Ab.data(), nAb * es, hipMemcpyHostToDevice), "hipMemcpy Ab");
     if (nV) hip_check(hipMemcpy(h.Vt.p, Vt.data(), nV * es, hipMemcpyHostToDevice), "hipMemcpy Vt");
     hip_check(hipMemcpy(h.perm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy perm");
+    h.hperm.assign(perm.begin(), perm.begin() + n);
     hip_check(hipMemset(h.Uf.p, 0, nU * es), "hipMemset U");
     hip_check(hipMemset(h.Lf.p, 0, nL * es), "hipMemset L");
     hip_check(hipMemset(h.info.p, 0, (size_t)h.Bp * sizeof(int)), "hipMemset info");
@@ -603,11 +606,17 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     hip_check(hipMemset(h.cond.p, 0, (size_t)h.Bp * sizeof(double)), "hipMemset cond");
 }
 
+// bytes of the lane work space of a solve of nrhs right-hand sides (one entry per system, rhs, row)
+size_t solve_ws_bytes(const cn_bb& h, int nrhs) {
+    return checked_mul({(size_t)h.B, (size_t)nrhs, (size_t)h.n, h.esz});
+}
+
 void reserve_impl(cn_bb& h, int nrhs) {
     if (nrhs <= h.nrhs_cap) return;
-    h.ws.alloc(checked_mul({(size_t)h.B, (size_t)nrhs, (size_t)h.n, h.esz}));
+    h.ws.alloc(solve_ws_bytes(h, nrhs));
     h.nrhs_cap = nrhs;
 }
+
 
 void prepare_t(cn_bb& h, hipStream_t s) {
     if (h.tr_ready) return;
@@ -646,8 +655,10 @@ int streamed_bufbytes(const cn_bb& h, const SolveIO& io, int& spx) {
                : -1;
 }
 
-void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
-    if (nrhs > h.nrhs_cap) {
+// ws: the lane work space of the solve (nullptr: the handle's own, grown to nrhs right-hand
+// sides here when needed; else caller-provided memory of at least solve_ws_bytes(h, nrhs) bytes)
+void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s, void* ws = nullptr) {
+    if (ws == nullptr && nrhs > h.nrhs_cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
         if (cs != hipStreamCaptureStatusNone)
@@ -655,6 +666,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
                                 " right-hand sides not reserved before graph capture (call cn_bb_reserve)");
         reserve_impl(h, nrhs);
     }
+    if (ws == nullptr) ws = h.ws.p;
     if (h.mode == 1) {
         const int sp = wave_steps((size_t)h.B * nrhs);
         const size_t lds = wave_lds_bytes(h, sp);
@@ -689,7 +701,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
             const size_t off = (size_t)k0 * (size_t)rs * h.esz;
             fs(dim3((h.B + IL - 1) / IL), dim3(192), streamed_lds(bufbytes, spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
                static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p),
-               static_cast<const int*>(h.perm.p), h.ws.p, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
+               static_cast<const int*>(h.perm.p), ws, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
             hip_check(hipGetLastError(), "k_solve_s launch");
             k0 += nr;
         }
@@ -699,7 +711,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s) {
     if (!f) fail(CN_ELOGIC, "no solve kernel for (kl, wu) = (" + std::to_string(h.KLP) + ", " + std::to_string(h.WU) + ")");
     const int lanes = h.B * nrhs;
     f(dim3((lanes + 255) / 256), dim3(256), s, h.Uf.p, h.Lf.p, static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p,
-      h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), h.ws.p, io, h.B, h.n, h.r, h.US,
+      h.S.p, static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p), ws, io, h.B, h.n, h.r, h.US,
       nrhs);
     hip_check(hipGetLastError(), "k_solve launch");
 }
@@ -884,12 +896,9 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
         if (count < 0 || src_first < 0 || dst_first < 0 || src_first + count > S_.B || dst_first + count > D.B)
             fail(CN_EINVAL, "cn_bb_copy_systems: system range out of bounds");
         if (count == 0) return;
-        {   // the row permutation (border rows first) must be the same
-            std::vector<int> a(S_.n), b(D.n);
-            hip_check(hipMemcpy(a.data(), S_.perm.p, sizeof(int) * S_.n, hipMemcpyDeviceToHost), "hipMemcpy perm");
-            hip_check(hipMemcpy(b.data(), D.perm.p, sizeof(int) * D.n, hipMemcpyDeviceToHost), "hipMemcpy perm");
-            if (a != b) fail(CN_EINVAL, "cn_bb_copy_systems: the handles have different border rows");
-        }
+        // the row permutation (border rows first) must be the same: compared on the host copies, so
+        // that the call stays asynchronous
+        if (S_.hperm != D.hperm) fail(CN_EINVAL, "cn_bb_copy_systems: the handles have different border rows");
         hipStream_t st = static_cast<hipStream_t>(stream);
         const size_t il = D.il, es = D.esz;
         // interleaved arrays: entry e of system m at ((m / il) * len + e) * il + m % il

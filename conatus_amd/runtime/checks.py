@@ -150,7 +150,7 @@ def check_index_map(name: str, idx, length: int, batch: int, n: int, where: str,
     lo, hi = torch.aminmax(idx)
     lo, hi = int(lo), int(hi)
     if lo < 0 or hi >= length:
-        raise ValueError(f"{where}: {name} points outside the array ([{lo}, {hi}] for {length} entries)")
+        raise ValueError(f"{where}: {name} has indices out of range: [{lo}, {hi}] against {length} entries")
     if unique and torch.sort(idx.reshape(-1)).values.diff().eq(0).any():
         raise ValueError(f"{where}: {name} repeats an index, so two systems would write one entry")
 

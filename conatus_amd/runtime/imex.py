@@ -227,11 +227,29 @@ class Imex:
             self._LX = [z() for _ in range(s)]
             self._MX0 = z()
         self._graphs = {}
+        self._t_step = 0.0            # time of the state being stepped (for a time-aware F)
         self.reset()
         # the continuation check compares states byte for byte (as imex.cu:145-151 does); run it
         # once here so that its device code is loaded before any timed step
         zz = torch.zeros((self.batch, self.n), dtype=self._tdt, device="cuda")
         torch.equal(zz.view(torch.uint8), zz.clone().view(torch.uint8))
+
+    def set_pointwise(self, g=None, h=None) -> None:
+        """F = g * X + h from (batch, n) host arrays (None: absent), copied to the device once (the
+        reference's set_pointwise_rhs, core/src/cuda/imex.cu:113-119); factors and history are kept."""
+        if self._explicit is not None:
+            raise OptionError("Imex.set_pointwise: this integrator has an explicit term writer")
+        torch = _torch()
+        N = self.batch * self.n
+
+        def conv(a):
+            if a is None:
+                return None
+            a = solve_operand(a, self.dtype, "Imex (g, h)").ravel()
+            if a.size != N:
+                raise OptionError("pointwise rhs: expected batch*n values")
+            return torch.from_numpy(a).to("cuda")
+        self._g, self._hh = conv(g), conv(h)
 
     # -- state ---------------------------------------------------------------------------------
     def reset(self) -> None:
@@ -257,6 +275,15 @@ class Imex:
         return f
 
     # -- steps ---------------------------------------------------------------------------------
+    def _write_F(self, x, out, t: float) -> None:
+        """F(x) into out: the explicit writer of the integrator; one that declares ``wants_time``
+        (a host callback F(x, t)) is given the time of the state, t + c_j dt for RK stage j
+        (imex.hpp:186)."""
+        if getattr(self._explicit, "wants_time", False):
+            self._explicit.write(x, out, t)
+        else:
+            self._explicit.write(x, out)
+
     def _sbdf_step(self, x) -> None:
         # the slot of the new level: a free slot, or the oldest level's
         if len(self._levels) < self.order:
@@ -282,7 +309,7 @@ class Imex:
         v = ctypes.c_void_p
         fext = None
         if self._explicit is not None:
-            self._explicit.write(x, self._fnew)
+            self._write_F(x, self._fnew, self._t_step)
             fext = v(_ptr(self._fnew))
         st = _lib.lib().cn_sbdf_stage(self._M._h, v(_ptr(x)), v(_ptr(self._g)), v(_ptr(self._hh)), fext,
                                       v(_ptr(mx0)), v(_ptr(f0)), mxh, fh, q, cc, adt, v(_ptr(self._rhs)),
@@ -312,7 +339,7 @@ class Imex:
         self._M.matvec(x, mx)
         self._L.matvec(x, lx)
         if self._explicit is not None:
-            self._explicit.write(x, f)
+            self._write_F(x, f, self._t_step)
         else:
             _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(f)), v(_ptr(x)), v(_ptr(self._g)),
                                                v(_ptr(self._hh)), v(_stream())), "Imex (F)")
@@ -351,7 +378,7 @@ class Imex:
             j = i - 1
             if needF(j):
                 if self._explicit is not None:
-                    self._explicit.write(X[j], self._F[j])
+                    self._write_F(X[j], self._F[j], self._t_step + float(cc[j]) * self.dt)
                 else:
                     _lib.check(_lib.lib().cn_pointwise(code, N, v(_ptr(self._F[j])), v(_ptr(X[j])),
                                                        v(_ptr(self._g)), v(_ptr(self._hh)), v(_stream())),
@@ -438,6 +465,7 @@ class Imex:
         self._continuation(d, t0)
         t = float(t0)
         for _ in range(int(nsteps)):
+            self._t_step = t
             if self.order:
                 self._sbdf_step(d)
             elif self.cn is not None:
@@ -456,6 +484,9 @@ class Imex:
         phase (one graph for RK).  Startup steps run eagerly; CN-family schemes step eagerly."""
         if self.cn is not None:
             return self.run(d, t0, nsteps, clock)
+        if getattr(self._explicit, "wants_time", False):
+            raise OptionError("Imex.run_graph: the explicit term depends on t (a host callback), which a "
+                              "captured graph would freeze", fix="use run()")
         self._check_state(d)
         self._continuation(d, t0)
         t, done = self._graph_startup(d, float(t0), nsteps, clock)
@@ -524,6 +555,7 @@ class Imex:
         done = 0
         if self.order:
             while self.history_length < self.order - 1 and done < nsteps:
+                self._t_step = t
                 self._sbdf_step(d)
                 self.iteration += 1
                 t = self._next_t(t, clock)

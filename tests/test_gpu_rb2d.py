@@ -120,25 +120,31 @@ def test_continued_calls_equal_one_call(scheme):
         assert s.t == one.t == 0.25 + 15 * 1e-3 and s.iteration == 15
 
 
-def test_finite_state_check_names_the_step():
-    """A state that holds a NaN stops the run with NumericalError at the next check, naming the
-    iteration and the unknown (python/conatus/timestep/run_checks.py:64-97)."""
-    import torch
+def test_finite_state_check_names_the_step(tmp_path):
+    """A state that holds a NaN stops the run with NumericalError at the next check (every
+    `check_finite` steps, and at the end of each call), naming the iteration and the unknown
+    (python/conatus/timestep/run_checks.py:64-97); no checkpoint of it is written."""
     from conatus_amd.errors import NumericalError
     from conatus_amd.timestep import IVPSolver
     pb = RayleighBenard2D(Nx=16, Nz=12, Lx=4.0, Ra=1e5, Pr=1.0)
+    X = pb.initial_state(seed=1)
+    X[3, pb.field_index(2)[4]] = complex(float("nan"), 0.0)
     s = IVPSolver(pb, "SBDF2", 1e-3, check_finite=5)
-    s.set_state(pb.initial_state(seed=1))
-    s.run(5)
-    s.X[3, pb.field_index(2)[4]] = complex(float("nan"), 0.0)
-    torch.cuda.synchronize()
-    with pytest.raises(NumericalError, match="iteration 10"):
+    s.set_state(X)
+    with pytest.raises(NumericalError, match="iteration 5"):
         s.run(7)
-    assert s.iteration == 10
+    assert s.iteration == 5
+    e = IVPSolver(pb, "SBDF2", 1e-3)                  # default cadence 100: the end of the call
+    e.set_state(X)
+    with pytest.raises(NumericalError, match=r"iteration 3, .*unknown '"):
+        e.run_graph(3)
+    with pytest.raises(NumericalError):
+        e.save_checkpoint(tmp_path / "nan.npz")
+    assert not (tmp_path / "nan.npz").exists()
     off = IVPSolver(pb, "SBDF2", 1e-3, check_finite=False)
-    off.set_state(pb.initial_state(seed=1))
-    off.X[0, 0] = complex(float("inf"), 0.0)
+    off.set_state(X)
     off.run(2)                            # no check: the run goes on
+    assert off.iteration == 2
 
 
 def test_graph_replay_equals_eager_steps():

@@ -22,7 +22,8 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
            "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
            "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode",
-           "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit")
+           "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit",
+           "cn_bb_solve_noalloc")
 
 
 class Props(ctypes.Structure):
@@ -56,6 +57,7 @@ def _declare(lib):
         "cn_bb_reserve": (ctypes.c_int, [_vp, ctypes.c_int]),
         "cn_bb_solve": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _vp]),
         "cn_bb_solve_mapped": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "cn_bb_solve_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int64, _vp]),
         "cn_bb_solve_split": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "cn_bb_prepare_transpose": (ctypes.c_int, [_vp]),
         "cn_bb_props_get": (ctypes.c_int, [_vp, ctypes.POINTER(Props)]),

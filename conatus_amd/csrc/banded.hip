@@ -39,6 +39,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -51,6 +52,10 @@ using cn::zc;
 
 using namespace cnk;
 
+// Lane work space of cn_bb_solve_noalloc (see scratch_for_device below): 64 MiB, enough for the
+// four right-hand sides of 4096 complex systems of n = 256.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
 namespace {
 
@@ -617,6 +622,26 @@ void reserve_impl(cn_bb& h, int nrhs) {
     h.nrhs_cap = nrhs;
 }
 
+// ---- the allocation-free solve: its lane work space is a fixed device array of this library ----
+// Every device the library runs on holds one copy of `scratch` for the life of the process; its
+// address is looked up once per device (hipGetSymbolAddress, with that device current) and kept.
+// The kernels write every work-space entry before they read it, so nothing needs clearing.
+// Calls that overlap in time on one device share the array and race.
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+void* scratch_for_device(int dev) {
+    if (dev < 0 || dev >= SCRATCH_DEVICES) fail(CN_EUNSUPPORTED, "cn_bb_solve_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) fail(CN_EHIP, "cn_bb_solve_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
 
 void prepare_t(cn_bb& h, hipStream_t s) {
     if (h.tr_ready) return;
@@ -810,6 +835,32 @@ int cn_bb_solve(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, int transpose,
         io.x = x;
         io.rhs_stride = rhs_stride;
         solve_impl(*h, io, nrhs, static_cast<hipStream_t>(stream));
+    });
+}
+
+int cn_bb_solve_noalloc(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, void* stream) {
+    return guarded([&] {
+        if (!h) fail(CN_EINVAL, "cn_bb_solve_noalloc: null handle");
+        if (!x) fail(CN_EINVAL, "cn_bb_solve_noalloc: null device pointer");
+        if (nrhs < 1) fail(CN_EINVAL, "cn_bb_solve_noalloc: nrhs must be >= 1");
+        if (nrhs > 1 && rhs_stride < (int64_t)h->B * h->n)
+            fail(CN_EINVAL, "cn_bb_solve_noalloc: rhs_stride must be >= batch * n");
+        const size_t need = solve_ws_bytes(*h, nrhs);
+        if (need > SCRATCH_BYTES)
+            fail(CN_EINVAL, "cn_bb_solve_noalloc: the work space of " + std::to_string(nrhs) + " right-hand side(s) of " +
+                                std::to_string(h->B) + " systems of n = " + std::to_string(h->n) + " needs " +
+                                std::to_string(need) + " bytes, more than the library's fixed " +
+                                std::to_string(SCRATCH_BYTES) + " (use cn_bb_solve, or fewer right-hand sides per call)");
+        int dev = -1;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        if (dev != h->device)
+            fail(CN_EINVAL, "cn_bb_solve_noalloc: called with device " + std::to_string(dev) +
+                                " current, the handle lives on device " + std::to_string(h->device));
+        SolveIO io{};
+        io.mode = 0;
+        io.x = x;
+        io.rhs_stride = rhs_stride;
+        solve_impl(*h, io, nrhs, static_cast<hipStream_t>(stream), scratch_for_device(dev));
     });
 }
 

@@ -97,6 +97,15 @@ int cn_bb_reserve(cn_bb_t h, int nrhs);
  * (core/bindings/module.cpp:181-206). */
 int cn_bb_solve(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, int transpose, void* stream);
 
+/* In-place batched solve that allocates nothing: cn_bb_solve (transpose = 0) with the lane work
+ * space taken from a fixed 64 MiB device array of the library instead of handle memory, so that
+ * no hipMalloc can happen in the call even when the handle never reserved nrhs right-hand sides.
+ * No counterpart in the reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Limit: batch * nrhs * n * element size
+ * <= 64 MiB (CN_EINVAL otherwise); the current device must be the handle's.  Every call on one
+ * device shares that array, so calls that overlap in time on one device race. */
+int cn_bb_solve_noalloc(cn_bb_t h, void* x, int nrhs, int64_t rhs_stride, void* stream);
+
 /* Out-of-place solve with the gather/scatter fused: b_m[i] = rhs[rows[m*n+i]],
  * x[cols[m*n+i]] = x_m[i]; rows/cols int64 (batch, n) device arrays or NULL (contiguous).
  * Replaces BandedBatch.solve_mapped_ptr (core/bindings/module.cpp:335-343). */

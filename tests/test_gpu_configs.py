@@ -1,0 +1,163 @@
+"""GPU tests at the workloads of the benchmark configurations, not only at toy sizes.
+
+* 3D channel 128 x 97 x 128 (Lx = 4 pi, Lz = 2 pi, Re = 2000, SBDF3, dt = 5e-4; the primitive
+  variable form, 8128 complex mode systems n = 395 on the 192 x 145 x 192 grid): the explicit
+  term against the CPU oracle (<= 1e-13 relative), the mode solves of one step for 40 sampled
+  modes (normwise backward error <= 1e-13 against the operator and the assembled right-hand side;
+  the oracle's LAPACK solution of the same system within cond(A) x 1e-15, cond from the dense
+  matrix), the laminar Poiseuille velocity steady to rounding (1e-10, as at the small sizes: the
+  taus take their steady values), graph replays equal to eager steps bit for bit.
+* The uneven ensemble of 256 RB 2D runs (Ny from {32, 48, 64, 96, 128} seeded, Nx = 2 Ny, Ra
+  log-uniform in [1e4, 1e6], Pr = 1, SBDF2): one EnsembleSet, graph-replayed; one member of every
+  size class equal bit for bit to its run alone (same kernel mode) and within 1e-11 of the CPU
+  oracle.
+* A served stream mixing RB 2D requests (Ny 32 .. 128) with 3D channel requests at 64 x 65 x 64,
+  seeded Poisson arrivals in ticks: every result equal bit for bit to its run alone.
+"""
+import numpy as np
+import pytest
+
+from conatus_amd.problems import Channel3D, RayleighBenard2D
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(abs(a - b)) / max(np.max(abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def channel():
+    print("[configs] building the 128x97x128 channel", flush=True)
+    pb = Channel3D(Nx=128, Ny=97, Nz=128, Re=2000.0, Lx=4 * np.pi, Lz=2 * np.pi)
+    return pb, pb.initial_state(seed=0, energy=1e-4)
+
+
+def test_channel_full_size_explicit_term(channel):
+    import torch
+    from conatus_amd.timestep import Channel3DExplicit
+    from oracle.channel3d import Channel3DOracle
+    pb, X0 = channel
+    ex = Channel3DExplicit(pb)
+    out = ex.base().clone()
+    ex.write(torch.from_numpy(X0.ravel()).cuda(), out)
+    got = out.view(pb.K, pb.n).cpu().numpy()
+    ref = Channel3DOracle(pb).explicit(X0)
+    rows = np.concatenate([pb.equation_rows(e) for e in range(3)])
+    r = rel(got[:, rows], ref[:, rows])
+    print(f"[configs] channel explicit term at 128x97x128: relative difference {r:.2e}", flush=True)
+    assert r < 1e-13
+
+
+def test_channel_full_size_step_solves(channel):
+    """One SBDF1 step: the device's assembled right-hand side (M X / dt + F) solved by the device
+    for all 8128 modes; 40 sampled modes checked against the operator and the oracle."""
+    import scipy.sparse as sp
+    from conatus_amd.timestep import IVPSolver
+    from oracle.banded import BorderedBanded as OracleBB
+    from tests.banded_cases import backward_error
+    pb, X0 = channel
+    dt = 5e-4
+    s = IVPSolver(pb, "SBDF1", dt)
+    s.set_state(X0)
+    s.run(1)
+    X1 = s.state()
+    rhs = s.imex._rhs.view(pb.K, pb.n).cpu().numpy()
+    rng = np.random.default_rng(7)
+    sample = np.unique(np.concatenate([[0, 1, pb.K - 1], rng.choice(pb.K, 37, replace=False)]))
+    worst_be, worst_rel = 0.0, 0.0
+    for m in sample:
+        M, L = pb.mats[m]
+        A = (np.complex128(1.0 / dt) * M + L).tocsr()
+        be = backward_error(A, X1[m], rhs[m])
+        xo = OracleBB(sp.csr_matrix(A), pb.border, np.complex128).solve(rhs[m])
+        cond = float(np.linalg.cond(A.toarray()))
+        worst_be = max(worst_be, be)
+        worst_rel = max(worst_rel, rel(X1[m], xo) / cond)
+        assert be <= 1e-13, (m, be)
+        assert rel(X1[m], xo) <= cond * 1e-15, (m, rel(X1[m], xo), cond)
+    print(f"[configs] {sample.size} sampled modes: backward error <= {worst_be:.2e}, "
+          f"oracle difference / cond <= {worst_rel:.2e}", flush=True)
+
+
+def test_channel_full_size_poiseuille_and_graph(channel):
+    from conatus_amd.timestep import IVPSolver
+    pb, X0 = channel
+    P = pb.poiseuille()
+    s = IVPSolver(pb, "SBDF3", 5e-4)
+    s.set_state(P)
+    s.run_graph(4)
+    vel = np.concatenate([pb.field_index(f) for f in (0, 1, 2)])
+    assert rel(s.state()[:, vel], P[:, vel]) < 1e-10
+    a = IVPSolver(pb, "SBDF3", 5e-4)
+    a.set_state(X0)
+    a.run(6)
+    b = IVPSolver(pb, "SBDF3", 5e-4)
+    b.set_state(X0)
+    b.run_graph(6)
+    assert np.array_equal(a.state(), b.state())
+
+
+def _ensemble_problems(runs=256, seed=3):
+    """The 256-run uneven set of the benchmark (same seeded draw)."""
+    rng = np.random.default_rng(seed)
+    nys = rng.choice([32, 48, 64, 96, 128], size=runs)
+    ras = 10 ** rng.uniform(4, 6, size=runs)
+    return [RayleighBenard2D(Nx=2 * int(ny), Nz=int(ny), Lx=4.0, Ra=float(ra), Pr=1.0) for ny, ra in zip(nys, ras)]
+
+
+def test_uneven_256_run_ensemble_members():
+    from conatus_amd.ensembles import EnsembleSet, discretization_key
+    from conatus_amd.timestep import IVPSolver
+    from oracle.rb2d import RB2DOracle
+    dt, steps = 1e-4, 5
+    pbs = _ensemble_problems()
+    X0 = [pb.initial_state(seed=1000 + i) for i, pb in enumerate(pbs)]
+    es = EnsembleSet(pbs, "SBDF2", dt)
+    print("[configs] " + es.explain(), flush=True)
+    es.set_states(X0)
+    es.run_graph(steps)
+    got = es.states()
+    first = {}
+    for i, pb in enumerate(pbs):
+        first.setdefault(discretization_key(pb), i)
+    assert len(first) == 5
+    for key, i in sorted(first.items()):
+        pb = pbs[i]
+        gi, _ = es.where[i]
+        s = IVPSolver(pb, "SBDF2", dt, mode=es.groups[gi].mode)
+        s.set_state(X0[i])
+        s.run_graph(steps)
+        assert np.array_equal(got[i], s.state()), (key, np.max(abs(got[i] - s.state())))
+        r = rel(got[i], RB2DOracle(pb, "SBDF2", dt).run(X0[i], steps))
+        print(f"[configs] run {i} ({pb.Nx}x{pb.Nz}, Ra {pb.Ra:.3g}): oracle difference {r:.2e}", flush=True)
+        assert r < 1e-11
+
+
+def test_served_stream_with_64x65x64_channels():
+    from conatus_amd.ensembles import discretization_key
+    from conatus_amd.serving import Request, Server
+    from conatus_amd.timestep import IVPSolver
+    rng = np.random.default_rng(11)
+    dt = 1e-4
+    spec = [("rb", 32), ("ch", 0), ("rb", 128), ("rb", 64), ("ch", 1), ("rb", 96), ("rb", 48), ("rb", 32)]
+    gaps = rng.exponential(2.0, size=len(spec))                  # Poisson arrivals, in ticks
+    arrivals = np.floor(np.cumsum(gaps) - gaps[0])
+    reqs = []
+    for i, (kind, v) in enumerate(spec):
+        if kind == "ch":
+            pb = Channel3D(Nx=64, Ny=65, Nz=64, Re=2000.0, Lx=4 * np.pi, Lz=2 * np.pi)
+            X0 = pb.initial_state(seed=50 + v, energy=1e-4)
+        else:
+            pb = RayleighBenard2D(Nx=2 * v, Nz=v, Lx=4.0, Ra=1e5, Pr=1.0)
+            X0 = pb.initial_state(seed=60 + i)
+        reqs.append(Request(pb, int(rng.integers(6, 16)), X0, arrival=float(arrivals[i])))
+    srv = Server("SBDF2", dt, page_size=4, channel_page_size=2)
+    st = srv.serve(reqs, clock="ticks")
+    print(f"[configs] served: {st}", flush=True)
+    assert st["requests"] == len(reqs) and "channel 64x65x64" in st["pages"]
+    for q in reqs:
+        s = IVPSolver(q.problem, "SBDF2", dt, mode=srv._modes[discretization_key(q.problem)])
+        s.set_state(q.X0)
+        s.run(q.steps)
+        assert np.array_equal(q.result, s.state()), (q.rid, np.max(abs(q.result - s.state())))

@@ -39,6 +39,7 @@ column 3 = -column 1 there, kept exactly by the explicit term.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -46,7 +47,7 @@ import scipy.sparse as sp
 
 from . import spectral as S
 
-__all__ = ["ChannelPT3D"]
+__all__ = ["ChannelPT3D", "ChannelPTExplicit", "ChannelPTSolver"]
 
 PHI_BORDER = 4
 PSI_BORDER = 2
@@ -337,3 +338,255 @@ def _cheb_diff_last(c: np.ndarray) -> np.ndarray:
         out[..., k] = out[..., k + 2] + 2 * (k + 1) * c[..., k + 1]
     out[..., 0] *= 0.5
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# device stepping
+# ------------------------------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _diagonals(A, offsets, N):
+    """(N, len(offsets)) table of A[m, m + o] (zero outside the matrix)."""
+    A = A.tocsr()
+    out = np.zeros((N, len(offsets)))
+    for t, o in enumerate(offsets):
+        m = np.arange(max(0, N - o))
+        out[m, t] = np.asarray(A[m, m + o]).ravel()
+    return out
+
+
+class ChannelPTExplicit:
+    """F(X) of the poloidal-toroidal channel on the device (cn_chpt_explicit: six launches, one
+    workspace); the tables are built once per discretization."""
+
+    def __init__(self, pb: ChannelPT3D):
+        from . import _lib
+        torch = _torch()
+        self.pb = pb
+        N, My = pb.N, pb.My
+        dev = "cuda"
+        Tb = np.stack([S.cheb_backward_matrix(N, My, 0), S.cheb_backward_matrix(N, My, 1, 2.0)], axis=-1)
+        Fy = S.cheb_forward_matrix(N, My)
+        S14D01 = (pb.S14 @ pb.D01).toarray()
+        S12D01 = (pb.S12 @ pb.D01).toarray()
+        Am = np.stack([S14D01 @ Fy, pb.S2D.toarray() @ Fy, pb.S04.toarray() @ Fy, pb.S02.toarray() @ Fy, S12D01 @ Fy])
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)           # noqa: E731
+        self.Tb, self.Am = t(Tb), t(Am)
+        self.s02c0 = t(pb.S02.toarray()[:, 0].copy())
+        self.Wx = t(np.exp(-2j * np.pi * np.arange(pb.Mx) / pb.Mx))
+        self.Wz = t(np.exp(-2j * np.pi * np.arange(pb.Mz) / pb.Mz))
+        nbytes = int(_lib.lib().cn_chpt_workspace_bytes(pb.Nx, pb.Ny, pb.Nz, pb.Mx, pb.My, pb.Mz))
+        self.ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+
+    def write(self, x, out) -> None:
+        from . import _lib
+        pb = self.pb
+        v = ctypes.c_void_p
+        p = lambda a: v(a.data_ptr())                                             # noqa: E731
+        st = _lib.lib().cn_chpt_explicit(p(x), p(out), p(self.ws), self.ws.numel(), p(self.Tb), p(self.Am),
+                                         p(self.s02c0), p(self.Wx), p(self.Wz), pb.Nx, pb.Ny, pb.Nz, pb.Mx, pb.My,
+                                         pb.Mz, float(pb.kx0), float(pb.kz0), float(pb.G),
+                                         v(_torch().cuda.current_stream().cuda_stream))
+        _lib.check(st, "ChannelPTExplicit")
+
+
+class ChannelPTSolver:
+    """IVP solver of the poloidal-toroidal channel on the device: SBDF1..4 with the lower-order
+    startup (the reference's IVPSolver step, python/conatus/timestep/integrator.py:648-682,
+    361-373).  One step is the explicit term (six launches) written into the newest history
+    level, the fused stage (M X and the right-hand side over X in place) and one in-place solve
+    of each bucket: the 4 P phi and 4 P psi right-hand sides against P real factorizations each
+    (the modes (kx, kz) and (kx, -kz) share theirs).  ``run_graph`` replays full-order steps from
+    captured HIP graphs, one per history ring phase.  The clock is t_origin + (iteration -
+    it_origin) dt, and the state is checked finite every ``check_finite`` steps."""
+
+    def __init__(self, problem: ChannelPT3D, scheme: str, dt: float, pivot: bool = True, mode: str | None = None,
+                 check_finite=True):
+        from .errors import OptionError
+        from .runtime.imex import sbdf_coefficients
+        from .timestep import finite_cadence
+        torch = _torch()
+        if scheme not in ("SBDF1", "SBDF2", "SBDF3", "SBDF4"):
+            raise OptionError(f"ChannelPTSolver: scheme {scheme!r} not supported", fix="one of SBDF1..SBDF4")
+        if not float(dt) > 0:
+            raise OptionError("dt must be > 0")
+        self.problem = pb = problem
+        self.scheme, self.dt, self.order = scheme, float(dt), int(scheme[4])
+        self._coef = sbdf_coefficients
+        self.pivot, self.mode = bool(pivot), mode
+        self._finite_every = finite_cadence(check_finite, f"ChannelPTSolver(problem, {scheme!r}, ...)")
+        self.explicit = ChannelPTExplicit(pb)
+        N = pb.N
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")        # noqa: E731
+        self._s04 = t(_diagonals(pb.S04, (0, 2, 4, 6, 8), N))
+        self._s2d = t(_diagonals(pb.S2D, (0, 2, 4, 6, 8), N))
+        self._s02 = t(_diagonals(pb.S02, (0, 2, 4), N))
+        z = lambda: torch.zeros(pb.size, dtype=torch.float64, device="cuda")       # noqa: E731
+        self.X = z()
+        self._ring = [(z(), z()) for _ in range(self.order)]        # (M X, F) levels
+        self._factors: dict = {}
+        self._graphs: dict = {}
+        self.t, self.t_origin, self.it_origin = 0.0, 0.0, 0
+        self._finite_ok = None
+        self.reset()
+
+    # -- state -------------------------------------------------------------------------------------
+    def reset(self) -> None:
+        self._levels: list[int] = []
+        self._last = None
+        self.iteration = 0
+
+    @property
+    def history_length(self) -> int:
+        return len(self._levels)
+
+    @property
+    def n_factorizations(self) -> int:
+        return len(self._factors)
+
+    def set_state(self, X: np.ndarray, t: float = 0.0) -> None:
+        X = np.asarray(X, dtype=np.float64).ravel()
+        if X.size != self.problem.size:
+            from .errors import OptionError
+            raise OptionError(f"ChannelPTSolver.set_state: {self.problem.size} values expected, got {X.size}")
+        self.X.copy_(_torch().from_numpy(np.ascontiguousarray(X)))
+        self.reset()
+        self.t, self.t_origin, self.it_origin = float(t), float(t), 0
+        self._finite_ok = None
+
+    def state(self) -> np.ndarray:
+        return self.X.cpu().numpy()
+
+    def _clock(self) -> float:
+        return self.t_origin + (self.iteration - self.it_origin) * self.dt
+
+    # -- operators ---------------------------------------------------------------------------------
+    def _factor(self, alpha: float):
+        f = self._factors.get(alpha)
+        if f is None:
+            from .runtime.imex import factor_operator
+            f = []
+            for name, P, n, border, off, mats in self.problem.buckets:
+                h = factor_operator([m[0] for m in mats], [m[1] for m in mats], alpha, 1.0, border, np.float64,
+                                    self.pivot, self.mode)
+                h.reserve(4)
+                f.append((h, off, P * n))
+            self._factors[alpha] = f
+        return f
+
+    # -- steps -------------------------------------------------------------------------------------
+    def _next_levels(self):
+        if len(self._levels) < self.order:
+            used = set(self._levels)
+            slot = next(i for i in range(self.order) if i not in used)
+            return [slot] + self._levels
+        return [self._levels[-1]] + self._levels[:-1]
+
+    def _launch(self, levels) -> None:
+        from . import _lib
+        torch = _torch()
+        pb = self.problem
+        q = len(levels)
+        a, c = self._coef(q)
+        mx0, f0 = self._ring[levels[0]]
+        self.explicit.write(self.X, f0)
+        P = ctypes.c_void_p * 4
+        fl = P(*([self._ring[i][1].data_ptr() for i in levels] + [0] * (4 - q)))
+        mh = P(*([self._ring[i][0].data_ptr() for i in levels[1:]] + [0] * (5 - q)))
+        cc = (ctypes.c_double * 5)(*([0.0] + [float(c[j]) for j in range(1, q + 1)] + [0.0] * (4 - q)))
+        adt = (ctypes.c_double * 5)(*([0.0] + [float(a[j] / self.dt) for j in range(1, q + 1)] + [0.0] * (4 - q)))
+        v = ctypes.c_void_p
+        stream = torch.cuda.current_stream().cuda_stream
+        st = _lib.lib().cn_chpt_sbdf_stage(v(self.X.data_ptr()), fl, v(mx0.data_ptr()), mh, q, cc, adt,
+                                           v(self._s04.data_ptr()), v(self._s2d.data_ptr()), v(self._s02.data_ptr()),
+                                           pb.Nx, pb.Ny, pb.Nz, float(pb.kx0), float(pb.kz0), v(stream))
+        _lib.check(st, "ChannelPTSolver (stage)")
+        for h, off, stride in self._factor(a[0] / self.dt):
+            h.solve_ptr(self.X.data_ptr() + 8 * off, 4, stride, False, stream)
+
+    def _continuation(self) -> None:
+        torch = _torch()
+        if self._levels and not (self._last is not None and self._last[1] == self.t
+                                 and torch.equal(self._last[0].view(torch.uint8), self.X.view(torch.uint8))):
+            from .errors import OptionError
+            raise OptionError("ChannelPTSolver: the history belongs to another state (the state was changed "
+                              "outside the solver)", fix="set_state(X, t) starts a new trajectory (reset)")
+
+    def _remember(self) -> None:
+        self._last = (self.X.clone(), float(self.t)) if self._levels else None
+
+    def _eager(self, steps: int) -> None:
+        for _ in range(steps):
+            levels = self._next_levels()
+            self._launch(levels)
+            self._levels = levels
+            self.iteration += 1
+            self.t = self._clock()
+
+    def step(self) -> None:
+        self.run(1)
+
+    def run(self, steps: int) -> float:
+        self._continuation()
+        fe, left = self._finite_every, int(steps)
+        while left:
+            k = min(left, fe - self.iteration % fe) if fe else left
+            self._eager(k)
+            left -= k
+            if fe and self.iteration % fe == 0 and left:
+                self.check_finite("ChannelPTSolver.run")
+        if fe and steps:
+            self.check_finite("ChannelPTSolver.run (end of the call)")
+        self._remember()
+        return self.t
+
+    def run_graph(self, steps: int) -> float:
+        from .runtime.imex import capture_graph
+        self._continuation()
+        done = 0
+        while self.history_length < self.order - 1 and done < steps:
+            self._eager(1)
+            done += 1
+        if done < steps:
+            a, _ = self._coef(self.order)
+            self._factor(a[0] / self.dt)
+        while done < steps:
+            levels = self._next_levels()
+            key = tuple(levels)
+            g = self._graphs.get(key)
+            if g is None:
+                g = capture_graph(lambda lv=levels: self._launch(lv))
+                self._graphs[key] = g
+            g.replay()
+            self._levels = levels
+            self.iteration += 1
+            self.t = self._clock()
+            done += 1
+        if self._finite_every and steps:
+            self.check_finite("ChannelPTSolver.run_graph (end of the call)")
+        self._remember()
+        return self.t
+
+    def check_finite(self, where: str = "ChannelPTSolver.check_finite") -> None:
+        torch = _torch()
+        if bool(torch.isfinite(self.X).all()):
+            self._finite_ok = (self.iteration, self.t)
+            return
+        from .errors import NumericalError
+        bad = ~torch.isfinite(self.X).cpu().numpy()
+        j = int(np.flatnonzero(bad)[0])
+        last = self._finite_ok
+        since = f"; last checked finite at iteration {last[0]} (t = {last[1]:.10g})" if last else ""
+        raise NumericalError(f"non-finite state (NaN or inf) at iteration {self.iteration}, t = {self.t:.10g}: "
+                             f"{self.problem.unknown_name(j)} ({int(bad.sum())} non-finite values){since}",
+                             where=where, fix=f"reduce dt (now {self.dt:g}) or raise the resolution")
+
+    def explain(self) -> str:
+        pb = self.problem
+        return (f"ChannelPTSolver({pb.describe()}, {self.scheme}, dt={self.dt:g}): {pb.K} complex modes as "
+                f"{pb.P} real phi systems (n={pb.nP}, kl=ku=4, r=4) and {pb.P} real psi systems (n={pb.nS}, "
+                f"kl=ku=2, r=2), four right-hand sides each; F from six products on the {pb.Mx}x{pb.My}x{pb.Mz} grid")

@@ -214,6 +214,37 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
                      const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
                      int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream);
 
+/* ---- 3D plane channel, poloidal-toroidal (velocity-vorticity) form ---------------------------
+ * The reference's channel_3d_pt (tests/channel3d/ch3d_cases.py:90-217).  State X (float64): the
+ * phi systems [4][P][Ny + 4] then the psi systems [4][P][Ny + 2], P = (Nx/2)(Nz/2) slots
+ * s = kx (Nz/2) + |kz|, columns (Re, Im) of mode (kx, +|kz|) then of (kx, -|kz|); phi rows
+ * [phi(-1), phi(1), phi'(-1), phi'(1), C^(4) equation rows], psi rows [psi(-1), psi(1), C^(2)
+ * rows]; psi slot 0 holds the mean flow U (column 0) and W (column 1).
+ *
+ * F = the explicit right-hand sides of every row (the curl curl and curl of N = -div(u u) for the
+ * phi and psi rows, the x-z means of Nx + G and Nz for U and W, zero elsewhere) from the six
+ * products on the Mx x My x Mz grid (Mx, Mz of the form 2^a 3^b).  Replaces the reference's
+ * evaluation of those right-hand sides (python/conatus/timestep/spectral.py:94-122 with the 3D
+ * transforms of python/conatus/timestep/xp_transforms.py:197-548).  Tables: Tb (My x Ny x 2:
+ * T_n(y_j), T_n'(y_j) at the ascending Gauss-Chebyshev nodes), Am (5 x Ny x My: S14 D01 Fy,
+ * S24 D02 Fy, S04 Fy, S02 Fy, S12 D01 Fy with Fy the Gauss-Chebyshev forward transform), s02c0
+ * (column 0 of S02), Wx, Wz (Mx, Mz complex twiddles exp(-2 pi i k / M)).  Workspace: ws of at
+ * least cn_chpt_workspace_bytes(...) bytes.  Six launches on `stream`. */
+size_t cn_chpt_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz);
+int cn_chpt_explicit(const double* X, double* F, void* ws, size_t ws_bytes, const double* Tb, const double* Am,
+                     const double* s02c0, const void* Wx, const void* Wz, int Nx, int Ny, int Nz, int Mx, int My,
+                     int Mz, double kx0, double kz0, double G, void* stream);
+
+/* One SBDF stage of that state (core/include/conatus/imex.hpp:158-176): mx0 = M X (M from the
+ * banded stencils s04, s2d (Ny x 5: S04 and S24 D02 at offsets 0, 2, .., 8) and s02 (Ny x 3)
+ * and k^2 of each slot), then X = sum_{j=1..q} c[j] F_j - a_over_dt[j] M X_j in place (level 1:
+ * f_levels[0] and mx0; levels j >= 2: f_levels[j-1], mx_hist[j-2]), ready for the in-place solve
+ * of each bucket.  Replaces the stage of the reference's IVPSolver step
+ * (python/conatus/timestep/integrator.py:361-373, lincomb.py:146-162) for this problem. */
+int cn_chpt_sbdf_stage(double* X, const double* const* f_levels, double* mx0, const double* const* mx_hist, int q,
+                       const double* c, const double* a_over_dt, const double* s04, const double* s2d,
+                       const double* s02, int Nx, int Ny, int Nz, double kx0, double kz0, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

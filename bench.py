@@ -245,6 +245,85 @@ def channel_leg(steps: int, gate_steps: int = 10) -> dict:
             "perturbation_energy_after": energy, "graph": True}
 
 
+def channel_pt_step_bytes(pb) -> dict:
+    """Algorithmic HBM bytes of one SBDF3 step of the poloidal-toroidal channel, by kernel: every
+    launch reads its inputs and writes its outputs once (state S = 4 P (nP + nS) doubles).
+    Explicit term: A reads the state's coefficients and writes u, v, w at the nodes (Kx My 3 Kz
+    complex); B reads them and writes 3 z-lines of Mz per (kx, y_j); C reads those and writes 6
+    products per (kx, y_j, z_m); D reads them and writes 6 x Kz per (kx, y_j); E reads those and
+    writes F.  Stage: X, three F levels and two M X levels read, M X and the right-hand side (over
+    X) written.  Solves (real factors, four right-hand sides each): per system
+    8 (2 kl + ku + 1) n + 4 n + 16 r n + 8 r^2 + 16 n nrhs."""
+    S = 8 * 4 * pb.P * (pb.nP + pb.nS)
+    z = 16
+    h1 = z * pb.Kx * pb.My * 3 * pb.Kz
+    h2 = z * pb.My * 3 * pb.Kx * pb.Mz
+    h3 = z * pb.My * 6 * pb.Kx * pb.Mz
+    h4 = z * pb.Kx * pb.My * 6 * pb.Kz
+    solve = lambda n, kl, ku, r: pb.P * (8 * (2 * kl + ku + 1) * n + 4 * n + 16 * r * n + 8 * r * r + 16 * n * 4)  # noqa: E731
+    out = {"yback": S + h1, "zback": h1 + h2, "xline": h2 + h3, "zfwd": h3 + h4, "yfwd": h4 + S,
+           "stage": 6 * S + 2 * S, "solve_phi": solve(pb.nP, 4, 4, 4), "solve_psi": solve(pb.nS, 2, 2, 2)}
+    out["total"] = float(sum(out.values()))
+    return out
+
+
+def channel_pt_leg(steps: int) -> dict:
+    """IMEX steps/s of the 3D plane channel in the configuration's velocity-vorticity
+    (poloidal-toroidal) form at 128 x 97 x 128 (Lx = 4 pi, Lz = 2 pi, Re = 2000, Poiseuille + a
+    seeded divergence-free perturbation of energy 1e-4, SBDF3, dt = 5e-4): 8128 complex modes as
+    4096 real phi systems (n = 101, kl = ku = 4, r = 4) and 4096 real psi systems (n = 99,
+    kl = ku = 2, r = 2), four right-hand sides each, and the explicit term from six products on
+    the 192 x 145 x 192 grid.  Accuracy gate first: 20 SBDF3 steps at 16 x 17 x 12 against the CPU
+    oracle (<= 1e-11 relative; the 128 x 97 x 128 parity is a GPU test).  The timed steps are graph
+    replays of full-order steps."""
+    import torch
+    from conatus_amd.channel_pt import ChannelPT3D, ChannelPTSolver
+    from oracle.channel_pt import ChannelPTOracle
+    small = ChannelPT3D(Nx=16, Ny=17, Nz=12, Re=800.0)
+    Xs = small.initial_state(seed=3, energy=1e-3, kmax=3)
+    g = ChannelPTSolver(small, "SBDF3", 2e-3)
+    g.set_state(Xs)
+    g.run(20)
+    ref = ChannelPTOracle(16, 17, 12, 800.0, small.Lx, small.Lz, small.G, scheme="SBDF3", dt=2e-3).run(Xs, 20)
+    gate = float(np.max(abs(g.state() - ref)) / np.max(abs(ref)))
+    log(f"channel (v-omega) accuracy gate (16x17x12, 20 SBDF3 steps): relative difference {gate:.2e}")
+    if not gate <= 1e-11:
+        raise RuntimeError(f"channel (v-omega) accuracy gate failed ({gate:.3e})")
+    pb = ChannelPT3D(Nx=128, Ny=97, Nz=128, Re=2000.0)
+    X0 = pb.initial_state(seed=0, energy=1e-4)
+    s = ChannelPTSolver(pb, "SBDF3", 5e-4)
+    log("channel (v-omega): " + s.explain())
+    s.set_state(X0)
+    s.run_graph(24)
+    torch.cuda.synchronize()
+    quiet_heap()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    s.run_graph(steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    X = s.state()
+    if not np.all(np.isfinite(X)):
+        raise RuntimeError("channel (v-omega) state not finite after the timed steps")
+    energy = pb.kinetic_energy(pb.velocity(X - pb.poiseuille()))
+    if not 0 < energy < 1e-3:
+        raise RuntimeError(f"channel (v-omega) perturbation energy {energy:.3e} out of range")
+    nb = channel_pt_step_bytes(pb)
+    ach = nb["total"] / (ms * 1e-3) / 1e9
+    return {"workload": "channel 3D 128x97x128 velocity-vorticity (poloidal-toroidal) SBDF3, dt = 5e-4: 4096 real "
+                        "phi systems (n=101, kl=ku=4, r=4) + 4096 real psi systems (n=99, kl=ku=2, r=2), 4 rhs "
+                        "each (8128 complex modes); grid 192x145x192",
+            "steps": steps, "ms_per_step": ms, "steps_per_s": 1e3 / ms, "wall_ms_per_step": wall * 1e3 / steps,
+            "mode_solves_per_s": pb.K * 1e3 / ms, "gate_relative_difference": gate,
+            "gate_workload": "16x17x12, 20 SBDF3 steps at dt = 2e-3 against the CPU oracle",
+            "perturbation_energy_after": energy, "graph": True,
+            "algorithmic_bytes_per_step": nb, "achieved_GBs": ach, "roofline_frac": ach / HBM_PEAK_GBS}
+
+
 def rb2d_step_bytes(pb) -> float:
     """Algorithmic HBM bytes of one SBDF2 step of a RB 2D run: the batched solve of its K mode
     systems (band rows of kl + ku + 1 = 31 entries for U and 12 for L, pivots, V^T and W of r = 7
@@ -390,7 +469,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rb2d", action="store_true", help="skip the RB 2D IMEX steps/s leg")
     ap.add_argument("--rb2d-steps", type=int, default=200)
-    ap.add_argument("--no-channel", action="store_true", help="skip the 3D channel IMEX leg")
+    ap.add_argument("--no-channel", action="store_true", help="skip the 3D channel IMEX leg (primitive variables)")
+    ap.add_argument("--no-channel-pt", action="store_true",
+                    help="skip the 3D channel IMEX leg in the velocity-vorticity form")
     ap.add_argument("--channel-steps", type=int, default=50)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-ensemble leg")
     ap.add_argument("--ensemble-runs", type=int, default=256)
@@ -533,6 +614,12 @@ def main() -> int:
         log(f"3D channel IMEX leg ({args.channel_steps} graph-replayed SBDF3 steps)")
         out["imex_channel"] = channel_leg(args.channel_steps)
         log(f"channel: {out['imex_channel']['ms_per_step']:.3f} ms per SBDF3 step")
+    if not args.no_channel_pt:
+        log(f"3D channel (velocity-vorticity) IMEX leg ({args.channel_steps} graph-replayed SBDF3 steps)")
+        out["imex_channel_pt"] = channel_pt_leg(max(args.channel_steps, 100))
+        c = out["imex_channel_pt"]
+        log(f"channel (v-omega): {c['ms_per_step']:.3f} ms per SBDF3 step, {c['achieved_GBs']:.0f} GB/s "
+            f"({100 * c['roofline_frac']:.0f} % of HBM)")
     if not args.no_ensemble:
         log(f"uneven ensemble leg ({args.ensemble_runs} runs)")
         out["ensemble"] = ensemble_leg(args.ensemble_runs, args.ensemble_steps, 1e-4)

@@ -359,24 +359,77 @@ def _diagonals(A, offsets, N):
     return out
 
 
+def _cdiv(a: int, b: int) -> int:
+    return -(-a // b)
+
+
+def y_fold_geometry(N: int, My: int) -> dict:
+    """Sizes of the parity-folded y tables (the same formulas as the kernels' ``Fold``): Mh folded
+    nodes (MHP padded to 16, Mhs to 8), KS coefficient pairs (padded to 8), RE / RO rows of the
+    even / odd row group (padded to 16); the tables carry one zero step more, KT = KS + 4 and
+    MhT = Mhs + 4 rows, for the kernels' load-ahead."""
+    Mh = (My + 1) // 2
+    KS = 8 * _cdiv((N + 1) // 2, 8)
+    Mhs = 8 * _cdiv(Mh, 8)
+    return dict(Mh=Mh, MHP=16 * _cdiv(Mh, 16), Mhs=Mhs, MhT=Mhs + 4, KS=KS, KT=KS + 4,
+                RE=16 * _cdiv((N + 1) // 2, 16), RO=16 * _cdiv(N // 2, 16))
+
+
+def y_fold_tables(pb: "ChannelPT3D"):
+    """The y tables of cn_chpt_explicit.  The Gauss nodes are symmetric (y_{My-1-j} = -y_j), so
+    T_n and T'_n have parity (-1)^n and (-1)^(n+1) over the node pairs, and every row of the five
+    forward matrices has one parity: (-1)^m for S04 Fy, S24 D02 Fy, S02 Fy and (-1)^(m+1) for
+    S14 D01 Fy, S12 D01 Fy.  Both are checked here to 1e-12 of the largest entry before the
+    first Mh node columns are kept.
+
+    Returns (Tb[4][KT][MHP], Am[5][MhT][RE + RO], s02c0): Tb holds T_{2k}, T_{2k+1}, T'_{2k},
+    T'_{2k+1} at the first Mh nodes; Am holds the forward matrices node-major with the rows
+    permuted even first (row m = 2 i at i, m = 2 i + 1 at RE + i); zero padding throughout."""
+    N, My = pb.N, pb.My
+    g = y_fold_geometry(N, My)
+    Mh = g["Mh"]
+    T = S.cheb_backward_matrix(N, My, 0)
+    D = S.cheb_backward_matrix(N, My, 1, 2.0)
+    half = My // 2
+    n = np.arange(N)
+    for B, sg in ((T, (-1.0) ** n), (D, -(-1.0) ** n)):
+        err = np.max(abs(B[::-1][:half] - sg * B[:half]))
+        if not err <= 1e-12 * np.max(abs(B)):
+            raise AssertionError(f"Chebyshev tables not parity-symmetric ({err:.2e})")
+    Tb = np.zeros((4, g["KT"], g["MHP"]))
+    for k in range(g["KS"]):
+        for p, (B, nn) in enumerate(((T, 2 * k), (T, 2 * k + 1), (D, 2 * k), (D, 2 * k + 1))):
+            if nn < N:
+                Tb[p, k, :Mh] = B[:Mh, nn]
+    Fy = S.cheb_forward_matrix(N, My)
+    mats = [(pb.S14 @ pb.D01).toarray() @ Fy, pb.S2D.toarray() @ Fy, pb.S04.toarray() @ Fy,
+            pb.S02.toarray() @ Fy, (pb.S12 @ pb.D01).toarray() @ Fy]
+    odd = (0, 4)                                     # the D01 forms: parity (-1)^(m+1)
+    m = np.arange(N)
+    RP = g["RE"] + g["RO"]
+    perm = np.where(m % 2 == 0, m // 2, g["RE"] + m // 2)
+    Am = np.zeros((5, g["MhT"], RP))
+    for t, A in enumerate(mats):
+        sg = (-1.0) ** m * (-1.0 if t in odd else 1.0)
+        err = np.max(abs(A[:, ::-1][:, :half] - sg[:, None] * A[:, :half]))
+        if not err <= 1e-12 * np.max(abs(A)):
+            raise AssertionError(f"forward matrix {t} not parity-symmetric ({err:.2e})")
+        Am[t, :Mh, perm] = A[:, :Mh]
+    return Tb, Am, pb.S02.toarray()[:, 0].copy()
+
+
 class ChannelPTExplicit:
-    """F(X) of the poloidal-toroidal channel on the device (cn_chpt_explicit: six launches, one
+    """F(X) of the poloidal-toroidal channel on the device (cn_chpt_explicit: five launches, one
     workspace); the tables are built once per discretization."""
 
     def __init__(self, pb: ChannelPT3D):
         from . import _lib
         torch = _torch()
         self.pb = pb
-        N, My = pb.N, pb.My
         dev = "cuda"
-        Tb = np.stack([S.cheb_backward_matrix(N, My, 0), S.cheb_backward_matrix(N, My, 1, 2.0)], axis=-1)
-        Fy = S.cheb_forward_matrix(N, My)
-        S14D01 = (pb.S14 @ pb.D01).toarray()
-        S12D01 = (pb.S12 @ pb.D01).toarray()
-        Am = np.stack([S14D01 @ Fy, pb.S2D.toarray() @ Fy, pb.S04.toarray() @ Fy, pb.S02.toarray() @ Fy, S12D01 @ Fy])
+        Tb, Am, s02c0 = y_fold_tables(pb)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)           # noqa: E731
-        self.Tb, self.Am = t(Tb), t(Am)
-        self.s02c0 = t(pb.S02.toarray()[:, 0].copy())
+        self.Tb, self.Am, self.s02c0 = t(Tb), t(Am), t(s02c0)
         self.Wx = t(np.exp(-2j * np.pi * np.arange(pb.Mx) / pb.Mx))
         self.Wz = t(np.exp(-2j * np.pi * np.arange(pb.Mz) / pb.Mz))
         nbytes = int(_lib.lib().cn_chpt_workspace_bytes(pb.Nx, pb.Ny, pb.Nz, pb.Mx, pb.My, pb.Mz))
@@ -430,6 +483,7 @@ class ChannelPTSolver:
         self._ring = [(z(), z()) for _ in range(self.order)]        # (M X, F) levels
         self._factors: dict = {}
         self._graphs: dict = {}
+        self._side = torch.cuda.Stream()
         self.t, self.t_origin, self.it_origin = 0.0, 0.0, 0
         self._finite_ok = None
         self.reset()
@@ -505,8 +559,14 @@ class ChannelPTSolver:
                                            v(self._s04.data_ptr()), v(self._s2d.data_ptr()), v(self._s02.data_ptr()),
                                            pb.Nx, pb.Ny, pb.Nz, float(pb.kx0), float(pb.kz0), v(stream))
         _lib.check(st, "ChannelPTSolver (stage)")
-        for h, off, stride in self._factor(a[0] / self.dt):
-            h.solve_ptr(self.X.data_ptr() + 8 * off, 4, stride, False, stream)
+        # the two buckets are independent: the psi solve runs on a second stream beside the phi
+        # solve (two branches of a captured graph)
+        (hphi, ophi, sphi), (hpsi, opsi, spsi) = self._factor(a[0] / self.dt)
+        main = torch.cuda.current_stream()
+        self._side.wait_stream(main)
+        hpsi.solve_ptr(self.X.data_ptr() + 8 * opsi, 4, spsi, False, self._side.cuda_stream)
+        hphi.solve_ptr(self.X.data_ptr() + 8 * ophi, 4, sphi, False, stream)
+        main.wait_stream(self._side)
 
     def _continuation(self) -> None:
         torch = _torch()

@@ -7,22 +7,24 @@
 //
 // Explicit term F(X) (the reference's channel_3d_pt right-hand sides,
 // tests/channel3d/ch3d_cases.py:176-191: the six products u_i u_j on the 3/2 grid, N = -div(u u),
-// the wall-normal curl curl and curl of N, the x-z means of Nx and Nz), six launches:
-//   A  y-backward per 16 modes:  H1[kx][j][3][iz]  = u, v, w at the Gauss node y_j, from
+// the wall-normal curl curl and curl of N, the x-z means of Nx and Nz), five launches:
+//   A  y-backward per 8 modes on the FP64 matrix cores (parity-folded Chebyshev tables):
+//      H1[kx][j][3][iz] = u, v, w at the Gauss node y_j, from
 //      u = i kx D phi + i kz psi, v = k^2 phi, w = i kz D phi - i kx psi (mean: U, 0, W)
-//   B  z-backward, one (kx, y_j) per workgroup: H2[j][3][kx][m]  (inverse FFT of length Mz)
-//   C  x-lines, 8 (y_j, z_m) per workgroup:   H3[j][6][kx][m]  = uu, uv, uw, vv, vw, ww
-//      (two real lines per complex inverse FFT of length Mx, products, two real lines per
-//      forward FFT, kx < Kx kept)
-//   D  z-forward, one (kx, y_j) per workgroup: H4[kx][j][6][iz] (forward FFT, kz represented)
-//   E  y-forward + the curl forms per 16 modes, with the dense matrices (N x My) of
-//      A1 = S14 D01 Fy, A2 = S24 D02 Fy, A0 = S04 Fy, B0 = S02 Fy, B1 = S12 D01 Fy (Fy the
+//   B  z-backward, 8 kx of one y_j per workgroup: H2[j][m][3][kx]  (inverse FFTs of length Mz,
+//      written through LDS in runs of 8 kx)
+//   C  x-lines, two (y_j, z_m) per workgroup: H3[j][m][6][kx] = uu, uv, uw, vv, vw, ww (two real
+//      lines per complex inverse FFT of length Mx, the products, two real lines per forward
+//      FFT, kx < Kx kept); input and output lines are contiguous
+//   D  z-forward, 8 kx of one y_j per workgroup: H4[kx][j][6][iz] (forward FFTs, kz represented)
+//   E  y-forward + the curl forms per 8 modes on the FP64 matrix cores, with the parity-folded
+//      matrices A1 = S14 D01 Fy, A2 = S24 D02 Fy, A0 = S04 Fy, B0 = S02 Fy, B1 = S12 D01 Fy (Fy the
 //      Gauss-Chebyshev forward transform), from the node combinations
 //        a = kx^2 P_uu + 2 kx kz P_uw + kz^2 P_ww - k^2 P_vv,  q = i (kx P_uv + kz P_vw),
 //        b = kx kz (P_uu - P_ww) + (kz^2 - kx^2) P_uw,        s = i (kz P_uv - kx P_vw):
 //        F_phi = A1 a - A2 q - A0 (k^2 q)   (C^(4) rows),   F_psi = B0 b - B1 s   (C^(2) rows)
-//      (wavenumbers times kx0, kz0); the kx = 0 plane is written as conjugate pairs.
-//   M  the mean flow: F_U = -B1 P_uv + G S02 e0, F_W = -B1 P_vw (real parts), one workgroup.
+//      (wavenumbers times kx0, kz0); the kx = 0 plane is written as conjugate pairs, and the
+//      mean flow F_U = -B1 Re P_uv + G S02 e0, F_W = -B1 Re P_vw in psi slot 0.
 // Every F entry is written (boundary rows and the unused columns of kz = 0 slots are zero).
 // FFTs are Stockham autosort passes of radix 4, 2 and 3 in LDS (lengths 2^a 3^b), twiddles
 // W_k = exp(-2 pi i k / M) from a table; inverse transforms carry no 1/M, forward ones do.
@@ -43,9 +45,7 @@ using cn::zc;
 
 namespace {
 
-constexpr int MODES = 16;   // modes per workgroup of the y transforms (A, E)
-constexpr int XL = 8;       // (y_j, z_m) lines per workgroup of the x-line kernel (C)
-constexpr int JC = 16;      // nodes per chunk of the y-forward (E)
+constexpr int ZK = 8;       // kx per workgroup of the z transforms (B, D): runs of 8 entries
 constexpr int MAXST = 16;   // FFT passes at most
 
 struct Plan {
@@ -56,8 +56,8 @@ struct Dims {
     int Kx, Kzh, Kz, N, nP, nS, P, Mx, My, Mz;
     double kx0, kz0, G;
     size_t h1() const { return (size_t)Kx * My * 3 * Kz; }
-    size_t h2() const { return (size_t)My * 3 * Kx * Mz; }
-    size_t h3() const { return (size_t)My * 6 * Kx * Mz; }
+    size_t h2() const { return (size_t)My * Mz * 3 * Kx; }
+    size_t h3() const { return (size_t)My * Mz * 6 * Kx; }
     size_t h4() const { return (size_t)Kx * My * 6 * Kz; }
     size_t region1() const { return h1() > h3() ? h1() : h3(); }
     size_t region2() const { return h2() > h4() ? h2() : h4(); }
@@ -146,365 +146,417 @@ __device__ zc* fft_lines(zc* a, zc* b, int M, int nl, const zc* tw, const Plan& 
     return a;
 }
 
+// ---- y transforms: parity folding on the FP64 matrix cores -----------------------------------
+// The Gauss nodes are symmetric, y_{My-1-j} = -y_j, and T_n(-y) = (-1)^n T_n(y).  Both y transforms
+// therefore run over the first Mh = ceil(My / 2) nodes only:
+//  * backward: with E = the sum over even n and O = over odd n of T_n(y_j) c_n, f(y_j) = E + O and
+//    f(y_{My-1-j}) = E - O (for T'_n the roles of even and odd n swap);
+//  * forward: row m of S04 Fy, S24 D02 Fy and S02 Fy has A[m][My-1-j] = (-1)^m A[m][j], of S14 D01 Fy
+//    and S12 D01 Fy (-1)^(m+1) (the operators keep or flip the parity of n), so with
+//    c+(j) = c(j) + c(My-1-j) and c-(j) = c(j) - c(My-1-j) (the middle node of an odd My: c and 0)
+//    a row of parity sign +1 sums A[m][j] c+(j), one of sign -1 sums A[m][j] c-(j).
+// The products are v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l % 16][l / 16], B (4 x 16)
+// lane l holds B[l / 16][l % 16], D (16 x 16) lane l holds D[4 r + l / 16][l % 16] in register r.
+// The 16 columns are 8 modes x (re, im): column 2 l + p is part p of mode iz0 + l.
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int YM = 8;   // modes per workgroup of the y transforms
+
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Table geometry, a function of (N, My) only: Mh folded nodes (MHP: padded to 16, Mhs: to 8), KS
+// coefficient pairs (padded to 8), RE / RO rows of the even / odd row group (padded to 16).  The
+// loops over node steps (E) and coefficient steps (A) run in pairs of 4 with the operands of the
+// next step loaded ahead, so the tables carry one more zero step: KT = KS + 4, MhT = Mhs + 4 rows.
+struct Fold {
+    int Mh, MHP, Mhs, MhT, KS, KT, RE, RO;
+    __host__ __device__ Fold(int N, int My)
+        : Mh((My + 1) / 2), MHP(16 * cdiv((My + 1) / 2, 16)), Mhs(8 * cdiv((My + 1) / 2, 8)),
+          MhT(8 * cdiv((My + 1) / 2, 8) + 4), KS(8 * cdiv((N + 1) / 2, 8)), KT(8 * cdiv((N + 1) / 2, 8) + 4),
+          RE(16 * cdiv((N + 1) / 2, 16)), RO(16 * cdiv(N / 2, 16)) {}
+};
+
 // ---- A: y-backward ------------------------------------------------------------------------
-// LDS: phi and psi coefficients of 16 modes, [2][N][16].  Thread: modes (2 mp, 2 mp + 1),
-// nodes j = g, g + 32, ...
-__global__ void __launch_bounds__(256)
+// Tables Tb[4][KT][MHP]: T_{2k}, T_{2k+1}, T'_{2k}, T'_{2k+1} at the first Mh nodes (zero padded).
+// Workgroup: 8 modes, one wave per tile of 16 folded nodes.  The phi and psi coefficients of the 8
+// modes are staged in LDS as [2 KS][17] (column 2 l + p, zero beyond N); B operands
+// i c = (-Im c, Re c) are read from the partner column.  Per tile: f = T phi, i Df = T' (i phi), i ps = T (i psi), each as E +- O.
+// u = kx (i Df) + kz (i ps), v = k^2 f, w = kz (i Df) - kx (i ps); the mean (kx = kz = 0): u = U,
+// v = 0, w = W (psi column 0 = U, column 1 = W).
+__global__ void __launch_bounds__(512)
 k_pt_yback(const double* __restrict__ X, const double* __restrict__ Tb, zc* __restrict__ H1, Dims d) {
     extern __shared__ __align__(16) char lds[];
-    zc* cf = reinterpret_cast<zc*>(lds);                  // [2][N][MODES]
-    const int kx = blockIdx.y, iz0 = blockIdx.x * MODES, tid = threadIdx.x, N = d.N;
+    const Fold fo(d.N, d.My);
+    const int NK = 2 * fo.KS;
+    double* ca = reinterpret_cast<double*>(lds);             // phi coefficients [NK][17]
+    double* cb = ca + (size_t)NK * 17;                        // psi coefficients [NK][17]
+    const int kx = blockIdx.y, iz0 = blockIdx.x * YM, tid = threadIdx.x, N = d.N;
     const double* Xpsi = X + d.size_phi();
-    for (int q = tid; q < 2 * MODES * N; q += blockDim.x) {
-        const int n = q % N, rest = q / N, l = rest % MODES, fld = rest / MODES;
-        const int iz = iz0 + l;
-        zc v{0.0, 0.0};
-        if (iz < d.Kz) {
+    for (int q = tid; q < 2 * 16 * NK; q += blockDim.x) {
+        const int n = q % NK, rest = q / NK, col = rest & 15, fld = rest >> 4;
+        const int iz = iz0 + (col >> 1), part = col & 1;
+        double v = 0.0;
+        if (n < N && iz < d.Kz) {
             const int kz = kz_of(iz, d.Kzh, d.Kz);
-            const int s = kx * d.Kzh + (kz >= 0 ? kz : -kz), c = kz >= 0 ? 0 : 2;
-            const bool mean = (kx == 0 && iz == 0);
+            const int s = kx * d.Kzh + (kz >= 0 ? kz : -kz), c = (kz >= 0 ? 0 : 2) + part;
             if (fld == 0) {
-                if (!mean) {
-                    const double* p = X + ((size_t)c * d.P + s) * d.nP + n;
-                    v = zc{p[0], p[(size_t)d.P * d.nP]};
-                }
+                if (!(kx == 0 && iz == 0)) v = X[((size_t)c * d.P + s) * d.nP + n];
             } else {
-                const double* p = Xpsi + ((size_t)c * d.P + s) * d.nS + n;
-                v = zc{p[0], p[(size_t)d.P * d.nS]};
+                v = Xpsi[((size_t)c * d.P + s) * d.nS + n];
             }
         }
-        cf[((size_t)fld * N + n) * MODES + l] = v;
+        (fld ? cb : ca)[(size_t)n * 17 + col] = v;
     }
     __syncthreads();
-    const int mp = tid & 7, g = tid >> 3;
-    const int l0 = 2 * mp;
-    double ax[2], az[2], k2[2];
-    bool mean[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int iz = iz0 + l0 + u;
-        const int kz = kz_of(iz < d.Kz ? iz : 0, d.Kzh, d.Kz);
-        ax[u] = kx * d.kx0;
-        az[u] = kz * d.kz0;
-        k2[u] = ax[u] * ax[u] + az[u] * az[u];
-        mean[u] = (kx == 0 && iz == 0);
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 15, kq = lane >> 4;
+    const double sg = (col & 1) ? 1.0 : -1.0;                 // (i c)_p = sg * c_(p ^ 1)
+    const size_t plane = (size_t)fo.KT * fo.MHP;
+    const double* tb = Tb + 16 * wave + (lane & 15);
+    d4 fE = {0.0, 0.0, 0.0, 0.0}, fO = fE, dE = fE, dO = fE, sE = fE, sO = fE;
+    auto load = [&](int k0, double (&T)[4]) {
+        const double* t = tb + (size_t)(k0 + kq) * fo.MHP;
+        T[0] = t[0];
+        T[1] = t[plane];
+        T[2] = t[2 * plane];
+        T[3] = t[3 * plane];
+    };
+    auto mma = [&](int k0, const double (&T)[4]) {
+        const int k = k0 + kq;
+        const double* a0 = ca + (size_t)(2 * k) * 17;
+        const double* b0 = cb + (size_t)(2 * k) * 17;
+        const double ae = a0[col], ao = a0[17 + col];
+        const double iae = sg * a0[col ^ 1], iao = sg * a0[17 + (col ^ 1)];
+        const double ibe = sg * b0[col ^ 1], ibo = sg * b0[17 + (col ^ 1)];
+        fE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[0], ae, fE, 0, 0, 0);
+        fO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[1], ao, fO, 0, 0, 0);
+        dE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[3], iao, dE, 0, 0, 0);
+        dO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[2], iae, dO, 0, 0, 0);
+        sE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[0], ibe, sE, 0, 0, 0);
+        sO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[1], ibo, sO, 0, 0, 0);
+    };
+    double Ta[4], Tn[4];
+    load(0, Ta);
+    for (int k0 = 0; k0 < fo.KS; k0 += 8) {
+        load(k0 + 4, Tn);
+        mma(k0, Ta);
+        load(k0 + 8, Ta);
+        mma(k0 + 4, Tn);
     }
-    for (int j = g; j < d.My; j += 32) {
-        const double* tb = Tb + (size_t)j * N * 2;
-        zc f[2] = {{0.0, 0.0}, {0.0, 0.0}}, df[2] = {{0.0, 0.0}, {0.0, 0.0}}, ps[2] = {{0.0, 0.0}, {0.0, 0.0}};
-        for (int n = 0; n < N; ++n) {
-            const double t = tb[2 * n], tp = tb[2 * n + 1];
-            const zc* c0 = cf + (size_t)n * MODES + l0;
-            const zc* c1 = cf + ((size_t)N + n) * MODES + l0;
+    const int iz = iz0 + (col >> 1), part = col & 1;
+    if (iz >= d.Kz) return;
+    const int kz = kz_of(iz, d.Kzh, d.Kz);
+    const double ax = kx * d.kx0, az = kz * d.kz0, k2 = ax * ax + az * az;
+    const bool mean = (kx == 0 && iz == 0);
+    double* h = reinterpret_cast<double*>(H1);
+    const size_t fs = 2 * (size_t)d.Kz;                       // field stride in doubles
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const zc a = c0[u], b = c1[u];
-                f[u].re = fma(t, a.re, f[u].re);
-                f[u].im = fma(t, a.im, f[u].im);
-                df[u].re = fma(tp, a.re, df[u].re);
-                df[u].im = fma(tp, a.im, df[u].im);
-                ps[u].re = fma(t, b.re, ps[u].re);
-                ps[u].im = fma(t, b.im, ps[u].im);
-            }
-        }
+    for (int r = 0; r < 4; ++r) {
+        const int j = 16 * wave + 4 * r + kq;
+        if (j >= fo.Mh) continue;
+        const int jm = d.My - 1 - j;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int iz = iz0 + l0 + u;
-            if (iz >= d.Kz) continue;
-            zc uu, vv, ww;
-            if (mean[u]) {
-                uu = zc{ps[u].re, 0.0};
-                vv = zc{0.0, 0.0};
-                ww = zc{ps[u].im, 0.0};
-            } else {
-                // u = i (ax Df + az ps), v = k2 f, w = i (az Df - ax ps)
-                const zc su = zc{ax[u] * df[u].re + az[u] * ps[u].re, ax[u] * df[u].im + az[u] * ps[u].im};
-                const zc sw = zc{az[u] * df[u].re - ax[u] * ps[u].re, az[u] * df[u].im - ax[u] * ps[u].im};
-                uu = zmuli(su);
-                vv = zscale(k2[u], f[u]);
-                ww = zmuli(sw);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && jm == j) break;
+            const double sn = side ? -1.0 : 1.0;
+            const double f = fE[r] + sn * fO[r], idf = dE[r] + sn * dO[r], ips = sE[r] + sn * sO[r];
+            double* o = h + 2 * ((((size_t)kx * d.My + (side ? jm : j)) * 3) * d.Kz + iz);
+            if (mean) {                       // i ps = (-W, U): part 1 holds U, part 0 holds -W
+                if (part) { o[0] = ips; o[fs + 1] = 0.0; o[2 * fs + 1] = 0.0; }
+                else { o[1] = 0.0; o[fs] = 0.0; o[2 * fs] = -ips; }
+                continue;
             }
-            zc* o = H1 + (((size_t)kx * d.My + j) * 3) * d.Kz + iz;
-            o[0] = uu;
-            o[d.Kz] = vv;
-            o[2 * (size_t)d.Kz] = ww;
+            o[part] = ax * idf + az * ips;
+            o[fs + part] = k2 * f;
+            o[2 * fs + part] = az * idf - ax * ips;
         }
     }
 }
 
 // ---- B: z-backward ------------------------------------------------------------------------
-__global__ void __launch_bounds__(64)
+// Workgroup: ZK consecutive kx of one y_j; per field the ZK lines are transformed together and
+// written as H2[j][m][f][kx0 .. kx0 + ZK) (runs of ZK entries).  LDS: tw[Mz] | two [ZK][Mz] buffers.
+__global__ void __launch_bounds__(512)
 k_pt_zback(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, Dims d, Plan pz) {
     extern __shared__ __align__(16) char lds[];
-    const int Mz = d.Mz;
+    const int Mz = d.Mz, Kx = d.Kx;
     zc* tw = reinterpret_cast<zc*>(lds);
     zc* a = tw + Mz;
-    zc* b = a + 3 * Mz;
-    const int j = blockIdx.x, kx = blockIdx.y, tid = threadIdx.x;
+    zc* b = a + ZK * Mz;
+    const int kx0 = blockIdx.x * ZK, j = blockIdx.y, tid = threadIdx.x;
+    const int nk = (Kx - kx0) < ZK ? (Kx - kx0) : ZK;
     for (int q = tid; q < Mz; q += blockDim.x) tw[q] = Wz[q];
-    for (int q = tid; q < 3 * Mz; q += blockDim.x) a[q] = zc{0.0, 0.0};
-    __syncthreads();
-    const zc* in = H1 + (((size_t)kx * d.My + j) * 3) * d.Kz;
-    for (int q = tid; q < 3 * d.Kz; q += blockDim.x) {
-        const int f = q / d.Kz, iz = q - f * d.Kz;
-        a[f * Mz + bin_of(kz_of(iz, d.Kzh, d.Kz), Mz)] = in[q];
-    }
-    __syncthreads();
-    const zc* res = fft_lines<true>(a, b, Mz, 3, tw, pz);
-    for (int q = tid; q < 3 * Mz; q += blockDim.x) {
-        const int f = q / Mz, m = q - f * Mz;
-        H2[(((size_t)j * 3 + f) * d.Kx + kx) * Mz + m] = res[q];
+    for (int f = 0; f < 3; ++f) {
+        for (int q = tid; q < ZK * Mz; q += blockDim.x) a[q] = zc{0.0, 0.0};
+        __syncthreads();
+        for (int q = tid; q < nk * d.Kz; q += blockDim.x) {
+            const int kl = q / d.Kz, iz = q - kl * d.Kz;
+            a[kl * Mz + bin_of(kz_of(iz, d.Kzh, d.Kz), Mz)] = H1[(((size_t)(kx0 + kl) * d.My + j) * 3 + f) * d.Kz + iz];
+        }
+        __syncthreads();
+        zc* res = fft_lines<true>(a, b, Mz, ZK, tw, pz);
+        for (int q = tid; q < Mz * nk; q += blockDim.x) {
+            const int m = q / nk, kl = q - m * nk;
+            H2[(((size_t)j * Mz + m) * 3 + f) * Kx + kx0 + kl] = res[(size_t)kl * Mz + m];
+        }
+        __syncthreads();
+        if (res != a) { zc* t = a; a = b; b = t; }      // keep `a` as the buffer filled next
     }
 }
 
 // ---- C: x-lines ---------------------------------------------------------------------------
-// LDS: tw[Mx] | in[3][Kx][XL] | phys[3][XL][Mx] (doubles) | two line buffers [XL][Mx]
-__global__ void __launch_bounds__(256)
+// Workgroup: the two lines (y_j, z_m), (y_j, z_m+1).  Inverse transforms: (u + i v) of each line
+// and (w_m + i w_m+1); then per product pair (uu, uv), (uw, vv), (vw, ww) one forward transform
+// per line.  LDS: tw[Mx] | X[3][Mx] (the time-domain lines) | Y[2][Mx], Z[2][Mx] (transforms).
+__global__ void __launch_bounds__(128)
 k_pt_xline(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, Dims d, Plan px) {
     extern __shared__ __align__(16) char lds[];
     const int Mx = d.Mx, Kx = d.Kx, Mz = d.Mz;
     zc* tw = reinterpret_cast<zc*>(lds);
-    zc* in = tw + Mx;
-    double* ph = reinterpret_cast<double*>(in + 3 * Kx * XL);
-    zc* la = reinterpret_cast<zc*>(ph + 3 * XL * Mx);
-    zc* lb = la + XL * Mx;
-    const int m0 = blockIdx.x * XL, j = blockIdx.y, tid = threadIdx.x;
-    const int nz = (Mz - m0) < XL ? (Mz - m0) : XL;
+    zc* X = tw + Mx;
+    zc* Y = X + 3 * Mx;
+    zc* Z = Y + 2 * Mx;
+    const int m0 = 2 * blockIdx.x, j = blockIdx.y, tid = threadIdx.x;
+    const int two = (Mz - m0) < 2 ? 1 : 2;
     for (int q = tid; q < Mx; q += blockDim.x) tw[q] = Wx[q];
-    for (int q = tid; q < 3 * Kx * XL; q += blockDim.x) {
-        const int mm = q % XL, rest = q / XL, k = rest % Kx, f = rest / Kx;
-        in[q] = mm < nz ? H2[(((size_t)j * 3 + f) * Kx + k) * Mz + m0 + mm] : zc{0.0, 0.0};
+    const zc* in = H2 + ((size_t)j * Mz + m0) * 3 * Kx;    // [line][f][kx]
+    // spectra: line 0: u0 + i v0, line 1: u1 + i v1, line 2: w0 + i w1 (Hermitian completions)
+    for (int q = tid; q < 3 * Mx; q += blockDim.x) {
+        const int line = q / Mx, k = q - line * Mx;
+        int kk = -1;
+        bool conj = false;
+        if (k < Kx) kk = k;
+        else if (k > Mx - Kx) { kk = Mx - k; conj = true; }
+        zc A{0.0, 0.0}, Bv{0.0, 0.0};
+        if (kk >= 0) {
+            if (line < 2) {
+                if (line < two) {
+                    A = in[((size_t)line * 3 + 0) * Kx + kk];
+                    Bv = in[((size_t)line * 3 + 1) * Kx + kk];
+                }
+            } else {
+                A = in[(size_t)2 * Kx + kk];
+                if (two == 2) Bv = in[((size_t)3 + 2) * Kx + kk];
+            }
+            if (kk == 0) { A.im = 0.0; Bv.im = 0.0; }
+            if (conj) { A = zconj(A); Bv = zconj(Bv); }
+        }
+        X[q] = zadd(A, zmuli(Bv));
     }
     __syncthreads();
-    // phase 1: u + i v per line; phase 2: w of line pairs (2p, 2p + 1)
-    for (int phase = 0; phase < 2; ++phase) {
-        const int nl = phase == 0 ? XL : XL / 2;
-        for (int q = tid; q < nl * Mx; q += blockDim.x) {
-            const int line = q / Mx, k = q - line * Mx;
-            zc A{0.0, 0.0}, Bv{0.0, 0.0};
-            bool conj = false;
-            int kk = -1;
-            if (k < Kx) kk = k;
-            else if (k > Mx - Kx) { kk = Mx - k; conj = true; }
-            if (kk >= 0) {
-                const zc* sa = phase == 0 ? in + (size_t)(0 * Kx + kk) * XL + line : in + (size_t)(2 * Kx + kk) * XL + 2 * line;
-                const zc* sb = phase == 0 ? in + (size_t)(1 * Kx + kk) * XL + line : sa + 1;
-                A = sa[0];
-                Bv = sb[0];
-                if (kk == 0) { A.im = 0.0; Bv.im = 0.0; }
-                if (conj) { A = zconj(A); Bv = zconj(Bv); }
-            }
-            la[q] = zadd(A, zmuli(Bv));
+    {
+        const zc* res = fft_lines<true>(X, Y, Mx, 2, tw, px);      // Y, Z as the scratch of 2 lines
+        if (res != X) {
+            for (int q = tid; q < 2 * Mx; q += blockDim.x) X[q] = res[q];
+            __syncthreads();
         }
-        __syncthreads();
-        const zc* res = fft_lines<true>(la, lb, Mx, nl, tw, px);
-        for (int q = tid; q < nl * Mx; q += blockDim.x) {
-            const int line = q / Mx, x = q - line * Mx;
-            if (phase == 0) {
-                ph[(0 * XL + line) * Mx + x] = res[q].re;
-                ph[(1 * XL + line) * Mx + x] = res[q].im;
-            } else {
-                ph[(2 * XL + 2 * line) * Mx + x] = res[q].re;
-                ph[(2 * XL + 2 * line + 1) * Mx + x] = res[q].im;
-            }
+        const zc* rw = fft_lines<true>(X + 2 * Mx, Z, Mx, 1, tw, px);
+        if (rw != X + 2 * Mx) {
+            for (int q = tid; q < Mx; q += blockDim.x) X[2 * Mx + q] = rw[q];
+            __syncthreads();
         }
-        __syncthreads();
     }
-    // products, two per forward transform: (uu, uv), (uw, vv), (vw, ww)
     const double inv = 1.0 / Mx;
+    zc* out = H3 + ((size_t)j * Mz + m0) * 6 * Kx;          // [line][f6][kx]
     for (int pair = 0; pair < 3; ++pair) {
-        const int f1 = 2 * pair, f2 = 2 * pair + 1;          // product indices 0 uu 1 uv 2 uw 3 vv 4 vw 5 ww
-        for (int q = tid; q < XL * Mx; q += blockDim.x) {
+        for (int q = tid; q < 2 * Mx; q += blockDim.x) {
             const int line = q / Mx, x = q - line * Mx;
-            const double u = ph[(0 * XL + line) * Mx + x], v = ph[(1 * XL + line) * Mx + x],
-                         w = ph[(2 * XL + line) * Mx + x];
-            la[q] = pair == 0 ? zc{u * u, u * v} : pair == 1 ? zc{u * w, v * v} : zc{v * w, w * w};
+            const double u = X[(size_t)line * Mx + x].re, v = X[(size_t)line * Mx + x].im;
+            const double w = line == 0 ? X[2 * Mx + x].re : X[2 * Mx + x].im;
+            Y[q] = pair == 0 ? zc{u * u, u * v} : pair == 1 ? zc{u * w, v * v} : zc{v * w, w * w};
         }
         __syncthreads();
-        const zc* res = fft_lines<false>(la, lb, Mx, XL, tw, px);
-        for (int q = tid; q < XL * Kx; q += blockDim.x) {
-            const int mm = q % XL, k = q / XL;
-            const zc Z = res[(size_t)mm * Mx + k];
-            const zc Zc = zconj(res[(size_t)mm * Mx + (k == 0 ? 0 : Mx - k)]);
-            const zc Af = zscale(0.5 * inv, zadd(Z, Zc));
-            const zc Bf = zscale(-0.5 * inv, zmuli(zsub(Z, Zc)));   // (Z - conj Z_-k) / (2 i)
-            if (mm < nz) {
-                H3[(((size_t)j * 6 + f1) * Kx + k) * Mz + m0 + mm] = Af;
-                H3[(((size_t)j * 6 + f2) * Kx + k) * Mz + m0 + mm] = Bf;
-            }
+        const zc* res = fft_lines<false>(Y, Z, Mx, 2, tw, px);
+        for (int q = tid; q < two * Kx; q += blockDim.x) {
+            const int line = q / Kx, k = q - line * Kx;
+            const zc Zk = res[(size_t)line * Mx + k];
+            const zc Zc = zconj(res[(size_t)line * Mx + (k == 0 ? 0 : Mx - k)]);
+            out[((size_t)line * 6 + 2 * pair) * Kx + k] = zscale(0.5 * inv, zadd(Zk, Zc));
+            out[((size_t)line * 6 + 2 * pair + 1) * Kx + k] = zscale(-0.5 * inv, zmuli(zsub(Zk, Zc)));
         }
         __syncthreads();
     }
 }
 
 // ---- D: z-forward -------------------------------------------------------------------------
-__global__ void __launch_bounds__(128)
+// Workgroup: ZK consecutive kx of one y_j; per product the lines are gathered from
+// H3[j][m][f][kx0 ..) (runs of ZK entries), transformed and written as H4[kx][j][f][iz].
+__global__ void __launch_bounds__(512)
 k_pt_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, Dims d, Plan pz) {
     extern __shared__ __align__(16) char lds[];
-    const int Mz = d.Mz;
+    const int Mz = d.Mz, Kx = d.Kx;
     zc* tw = reinterpret_cast<zc*>(lds);
     zc* a = tw + Mz;
-    zc* b = a + 6 * Mz;
-    const int j = blockIdx.x, kx = blockIdx.y, tid = threadIdx.x;
+    zc* b = a + ZK * Mz;
+    const int kx0 = blockIdx.x * ZK, j = blockIdx.y, tid = threadIdx.x;
+    const int nk = (Kx - kx0) < ZK ? (Kx - kx0) : ZK;
     for (int q = tid; q < Mz; q += blockDim.x) tw[q] = Wz[q];
-    for (int q = tid; q < 6 * Mz; q += blockDim.x) {
-        const int f = q / Mz, m = q - f * Mz;
-        a[q] = H3[(((size_t)j * 6 + f) * d.Kx + kx) * Mz + m];
-    }
-    __syncthreads();
-    const zc* res = fft_lines<false>(a, b, Mz, 6, tw, pz);
     const double inv = 1.0 / Mz;
-    zc* out = H4 + (((size_t)kx * d.My + j) * 6) * d.Kz;
-    for (int q = tid; q < 6 * d.Kz; q += blockDim.x) {
-        const int f = q / d.Kz, iz = q - f * d.Kz;
-        out[q] = zscale(inv, res[f * Mz + bin_of(kz_of(iz, d.Kzh, d.Kz), Mz)]);
+    for (int f = 0; f < 6; ++f) {
+        for (int q = tid; q < Mz * ZK; q += blockDim.x) {
+            const int m = q / ZK, kl = q - m * ZK;
+            a[(size_t)kl * Mz + m] = kl < nk ? H3[(((size_t)j * Mz + m) * 6 + f) * Kx + kx0 + kl] : zc{0.0, 0.0};
+        }
+        __syncthreads();
+        zc* res = fft_lines<false>(a, b, Mz, ZK, tw, pz);
+        for (int q = tid; q < nk * d.Kz; q += blockDim.x) {
+            const int kl = q / d.Kz, iz = q - kl * d.Kz;
+            H4[(((size_t)(kx0 + kl) * d.My + j) * 6 + f) * d.Kz + iz] =
+                zscale(inv, res[(size_t)kl * Mz + bin_of(kz_of(iz, d.Kzh, d.Kz), Mz)]);
+        }
+        __syncthreads();
+        if (res != a) { zc* t = a; a = b; b = t; }
     }
 }
 
-// ---- E: y-forward and the curl forms ------------------------------------------------------
-// LDS: combinations [JC][MODES][5] (a, q, k^2 q, b, s) | output rows [MODES][2][nP + nS] doubles
-__global__ void __launch_bounds__(256)
-k_pt_yfwd(const zc* __restrict__ H4, const double* __restrict__ Am, double* __restrict__ F, Dims d) {
+// ---- E: y-forward and the curl forms ----------------------------------------------------------
+// Tables Am[5][MhT][RE + RO]: A1 = S14 D01 Fy, A2 = S24 D02 Fy, A0 = S04 Fy, B0 = S02 Fy,
+// B1 = S12 D01 Fy at the first Mh nodes, node-major, rows permuted even first (m = 2 i at i < RE)
+// then odd (m = 2 i + 1 at RE + i), zero padded.  Workgroup: one wave, 8 modes; PSI selects the
+// bucket, NRT = (RE + RO) / 16 row tiles.  A lane loads the own part (p) and the partner part
+// (p ^ 1) of the products at its node pair, (i z)_p = sg z_(p ^ 1), forms the folded
+// combinations, and all row tiles accumulate in registers; the next node step's operands are
+// loaded while the current step's products run.  The rows are staged in LDS ([col][nrow],
+// boundary rows zero) and written as contiguous runs per column.
+// The mean (kx = kz = 0) is the psi column pair with b = 0 and -s = -(Re P_uv, Re P_vw), plus
+// G S02 e0 in column 0: F_U = -B1 Re P_uv + G S02 e0, F_W = -B1 Re P_vw; its phi rows are zero.
+template <bool PSI, int NRT>
+__global__ void __launch_bounds__(64)
+k_pt_yfwd(const zc* __restrict__ H4, const double* __restrict__ Am, const double* __restrict__ s02c0,
+          double* __restrict__ F, Dims d) {
     extern __shared__ __align__(16) char lds[];
-    zc* cmb = reinterpret_cast<zc*>(lds);                        // [JC][MODES][5]
-    double* orow = reinterpret_cast<double*>(cmb + JC * MODES * 5);   // [MODES][2 (re, im)][nP + nS]
-    const int kx = blockIdx.y, iz0 = blockIdx.x * MODES, tid = threadIdx.x, N = d.N, My = d.My;
-    const int nrow = d.nP + d.nS;
-    const int l = tid & (MODES - 1), g = tid >> 4;               // 16 row groups
-    const int iz = iz0 + l;
+    double* orow = reinterpret_cast<double*>(lds);           // [16][nrow]
+    constexpr bool psi = PSI;
+    constexpr int NM = PSI ? 2 : 3;                          // matrices of the bucket
+    constexpr int F2 = PSI ? 1 : 3;                          // own-part product in slot 2: uv / vv
+    const Fold fo(d.N, d.My);
+    const int kx = blockIdx.y, iz0 = blockIdx.x * YM, lane = threadIdx.x, N = d.N;
+    if (kx == 0 && iz0 >= d.Kzh) return;                     // mirrors, written with their partners
+    const int col = lane & 15, part = col & 1, kq = lane >> 4;
+    const int iz = iz0 + (col >> 1);
     const bool live = iz < d.Kz && !(kx == 0 && iz >= d.Kzh);
     const int kz = kz_of(live ? iz : 0, d.Kzh, d.Kz);
-    const double ax = kx * d.kx0, az = kz * d.kz0, k2 = ax * ax + az * az;
-    constexpr int RPT = 7;                                        // rows per thread (N <= 112)
-    zc fphi[RPT], fpsi[RPT];
+    const double bx = kx * d.kx0, bz = kz * d.kz0, kk = bx * bx + bz * bz;
+    const bool mean = (kx == 0 && iz == 0);
+    const double sg = part ? 1.0 : -1.0;
+    const int RP = 16 * NRT, nre = fo.RE / 16;
+    const size_t ts = (size_t)fo.MhT * RP;                    // matrix stride
+    const size_t fs = 2 * (size_t)d.Kz;                       // product stride in doubles
+    const double* H = reinterpret_cast<const double*>(H4) + 2 * ((size_t)kx * d.My * 6 * d.Kz + (live ? iz : iz0));
+    const double* ab = Am + (PSI ? 3 : 0) * ts + (size_t)kq * RP + (lane & 15);
+    d4 acc[NRT];
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) { fphi[r] = zc{0.0, 0.0}; fpsi[r] = zc{0.0, 0.0}; }
-    const size_t msz = (size_t)N * My;
-    for (int j0 = 0; j0 < My; j0 += JC) {
-        const int nj = (My - j0) < JC ? (My - j0) : JC;
-        __syncthreads();
-        for (int q = tid; q < JC * MODES; q += blockDim.x) {
-            const int ll = q % MODES, jj = q / MODES, izz = iz0 + ll;
-            zc* c = cmb + ((size_t)jj * MODES + ll) * 5;
-            if (jj >= nj || izz >= d.Kz) {
-                for (int t = 0; t < 5; ++t) c[t] = zc{0.0, 0.0};
-                continue;
-            }
-            const int kzz = kz_of(izz, d.Kzh, d.Kz);
-            const double bx = kx * d.kx0, bz = kzz * d.kz0, kk = bx * bx + bz * bz;
-            const zc* p = H4 + (((size_t)kx * My + j0 + jj) * 6) * d.Kz + izz;
-            const zc uu = p[0], uv = p[d.Kz], uw = p[2 * (size_t)d.Kz], vv = p[3 * (size_t)d.Kz],
-                     vw = p[4 * (size_t)d.Kz], ww = p[5 * (size_t)d.Kz];
-            // a = bx^2 uu + 2 bx bz uw + bz^2 ww - k^2 vv ; q = i (bx uv + bz vw)
-            const zc a = zsub(zadd(zadd(zscale(bx * bx, uu), zscale(2.0 * bx * bz, uw)), zscale(bz * bz, ww)),
-                              zscale(kk, vv));
-            const zc qv = zmuli(zadd(zscale(bx, uv), zscale(bz, vw)));
-            const zc bb = zadd(zscale(bx * bz, zsub(uu, ww)), zscale(bz * bz - bx * bx, uw));
-            const zc sv = zmuli(zsub(zscale(bz, uv), zscale(bx, vw)));
-            c[0] = a;
-            c[1] = qv;
-            c[2] = zscale(kk, qv);
-            c[3] = bb;
-            c[4] = sv;
+    for (int t = 0; t < NRT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    auto load = [&](int j0, double (&P)[2][6], double (&A)[NM][NRT]) {
+        const int j = j0 + kq;
+        const bool on = live && j < fo.Mh;
+        const int ja = on ? j : 0, jb = on ? d.My - 1 - j : 0;           // in bounds either way
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double* pp = H + (size_t)(side ? jb : ja) * 6 * fs;
+            const double v0 = pp[part], v1 = pp[2 * fs + part], v2 = pp[F2 * fs + part], v3 = pp[5 * fs + part];
+            const double v4 = pp[fs + (part ^ 1)], v5 = pp[4 * fs + (part ^ 1)];
+            P[side][0] = on ? v0 : 0.0;        // uu
+            P[side][1] = on ? v1 : 0.0;        // uw
+            P[side][2] = on ? v2 : 0.0;        // vv (phi) / uv (psi)
+            P[side][3] = on ? v3 : 0.0;        // ww
+            P[side][4] = on ? v4 : 0.0;        // uv, partner part
+            P[side][5] = on ? v5 : 0.0;        // vw, partner part
         }
-        __syncthreads();
-        for (int jj = 0; jj < nj; ++jj) {
-            const zc* c = cmb + ((size_t)jj * MODES + l) * 5;
-            const zc a = c[0], qv = c[1], kq = c[2], bb = c[3], sv = c[4];
-            const int j = j0 + jj;
+        const double* a = ab + (size_t)j0 * RP;
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int m = g + 16 * r;
-                if (m >= N) break;
-                const double* am = Am + (size_t)m * My + j;
-                const double a1 = am[0], a2 = am[msz], a0 = am[2 * msz], b0 = am[3 * msz], b1 = am[4 * msz];
-                fphi[r].re = fma(a1, a.re, fphi[r].re);
-                fphi[r].im = fma(a1, a.im, fphi[r].im);
-                fphi[r].re = fma(-a2, qv.re, fphi[r].re);
-                fphi[r].im = fma(-a2, qv.im, fphi[r].im);
-                fphi[r].re = fma(-a0, kq.re, fphi[r].re);
-                fphi[r].im = fma(-a0, kq.im, fphi[r].im);
-                fpsi[r].re = fma(b0, bb.re, fpsi[r].re);
-                fpsi[r].im = fma(b0, bb.im, fpsi[r].im);
-                fpsi[r].re = fma(-b1, sv.re, fpsi[r].re);
-                fpsi[r].im = fma(-b1, sv.im, fpsi[r].im);
+        for (int t = 0; t < NM; ++t)
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) A[t][rt] = a[t * ts + 16 * rt];
+    };
+    auto mma = [&](int j0, const double (&P)[2][6], const double (&A)[NM][NRT]) {
+        const bool mid = d.My - 1 - (j0 + kq) == j0 + kq;
+        double c[2][3];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double* q = P[side];
+            if constexpr (!PSI) {
+                const double nq = -sg * (bx * q[4] + bz * q[5]);
+                c[side][0] = bx * bx * q[0] + 2.0 * bx * bz * q[1] + bz * bz * q[3] - kk * q[2];
+                c[side][1] = nq;
+                c[side][2] = kk * nq;
+            } else if (mean) {                                   // -(Re P_uv, Re P_vw)
+                c[side][0] = 0.0;
+                c[side][1] = -(part ? q[5] : q[2]);
+                c[side][2] = 0.0;
+            } else {
+                c[side][0] = bx * bz * (q[0] - q[3]) + (bz * bz - bx * bx) * q[1];
+                c[side][1] = -sg * (bz * q[4] - bx * q[5]);
+                c[side][2] = 0.0;
             }
+        }
+        double cp[3], cm[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            cp[t] = mid ? c[0][t] : c[0][t] + c[1][t];
+            cm[t] = mid ? 0.0 : c[0][t] - c[1][t];
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+            const bool ev = rt < nre;
+            if constexpr (!PSI) {
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[0][rt], ev ? cm[0] : cp[0], acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[1][rt], ev ? cp[1] : cm[1], acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[2][rt], ev ? cp[2] : cm[2], acc[rt], 0, 0, 0);
+            } else {
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[0][rt], ev ? cp[0] : cm[0], acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[1][rt], ev ? cm[1] : cp[1], acc[rt], 0, 0, 0);
+            }
+        }
+    };
+    double Pa[2][6], Pn[2][6], Aa[NM][NRT], An[NM][NRT];
+    load(0, Pa, Aa);
+    for (int j0 = 0; j0 < fo.Mhs; j0 += 8) {
+        load(j0 + 4, Pn, An);
+        mma(j0, Pa, Aa);
+        load(j0 + 8, Pa, Aa);
+        mma(j0 + 4, Pn, An);
+    }
+    const int nrow = psi ? d.nS : d.nP, off = psi ? 2 : 4;
+    for (int q = lane; q < 16 * off; q += 64) orow[(size_t)(q / off) * nrow + q % off] = 0.0;
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * rt + 4 * r + kq;
+            const int m = i < fo.RE ? 2 * i : 2 * (i - fo.RE) + 1;
+            if (m >= N) continue;
+            double v = acc[rt][r];
+            if (psi && mean && part == 0) v += d.G * s02c0[m];
+            orow[(size_t)col * nrow + off + m] = v;
         }
     }
     __syncthreads();
-    // rows into LDS: [mode][re/im][phi rows (nP) | psi rows (nS)]
-    double* ore = orow + (size_t)l * 2 * nrow;
-    double* oim = ore + nrow;
-    if (g == 0) {
-        for (int r = 0; r < 4; ++r) ore[r] = oim[r] = 0.0;
-        ore[d.nP] = oim[d.nP] = ore[d.nP + 1] = oim[d.nP + 1] = 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const int m = g + 16 * r;
-        if (m >= N) break;
-        ore[4 + m] = fphi[r].re;
-        oim[4 + m] = fphi[r].im;
-        ore[d.nP + 2 + m] = fpsi[r].re;
-        oim[d.nP + 2 + m] = fpsi[r].im;
-    }
-    __syncthreads();
-    // write-out: each mode's rows are contiguous per column; 16 threads per mode
-    double* Fphi = F;
-    double* Fpsi = F + d.size_phi();
-    for (int ll = 0; ll < MODES; ++ll) {
+    double* Fb = psi ? F + d.size_phi() : F;
+    const size_t cs = (size_t)d.P * nrow;                     // column stride
+    for (int ll = 0; ll < YM; ++ll) {
         const int izz = iz0 + ll;
         if (izz >= d.Kz || (kx == 0 && izz >= d.Kzh)) continue;
         const int kzz = kz_of(izz, d.Kzh, d.Kz);
         const int s = kx * d.Kzh + (kzz >= 0 ? kzz : -kzz), c = kzz >= 0 ? 0 : 2;
-        const double* sre = orow + (size_t)ll * 2 * nrow;
+        const double* sre = orow + (size_t)(2 * ll) * nrow;
         const double* sim = sre + nrow;
-        const bool mean = (kx == 0 && izz == 0);
-        const bool mirror = (kx == 0 && kzz > 0);     // (0, -kz) = conj (0, kz)
-        const bool dead = (kzz == 0);                 // no (kx, -0) mode: columns 2, 3 zero
-        for (int q = tid; q < nrow; q += blockDim.x) {
-            const bool isphi = q < d.nP;
-            double* base = isphi ? Fphi + (size_t)s * d.nP + q : Fpsi + (size_t)s * d.nS + (q - d.nP);
-            const size_t cs = isphi ? (size_t)d.P * d.nP : (size_t)d.P * d.nS;   // column stride
-            double re = sre[q], im = sim[q];
-            if (mean) {
-                if (isphi) { base[0] = 0.0; base[cs] = 0.0; }   // the (0, 0) pin; psi columns 0, 1: k_pt_mean
-                base[2 * cs] = 0.0;
-                base[3 * cs] = 0.0;
-                continue;
-            }
-            base[c * cs] = re;
-            base[(c + 1) * cs] = im;
+        const bool mirror = (kx == 0 && kzz > 0);   // (0, -kz) = conj (0, kz)
+        const bool dead = (kzz == 0);               // no (kx, -0) mode: columns 2, 3 zero
+        double* base = Fb + (size_t)s * nrow;
+        for (int q = lane; q < nrow; q += 64) {
+            const double re = sre[q], im = sim[q];
+            base[c * cs + q] = re;
+            base[(c + 1) * cs + q] = im;
             if (mirror) {
-                base[2 * cs] = re;
-                base[3 * cs] = -im;
+                base[2 * cs + q] = re;
+                base[3 * cs + q] = -im;
             } else if (dead) {
-                base[2 * cs] = 0.0;
-                base[3 * cs] = 0.0;
+                base[2 * cs + q] = 0.0;
+                base[3 * cs + q] = 0.0;
             }
         }
     }
 }
 
-// ---- M: the mean flow ---------------------------------------------------------------------
-__global__ void __launch_bounds__(128)
-k_pt_mean(const zc* __restrict__ H4, const double* __restrict__ Am, const double* __restrict__ s02c0,
-          double* __restrict__ F, Dims d) {
-    const int m = threadIdx.x, N = d.N, My = d.My;
-    double* Fpsi = F + d.size_phi();
-    const size_t cs = (size_t)d.P * d.nS;
-    if (m < 2) { Fpsi[m] = 0.0; Fpsi[cs + m] = 0.0; }
-    if (m >= N) return;
-    const double* b1 = Am + 4 * (size_t)N * My + (size_t)m * My;
-    double fu = 0.0, fw = 0.0;
-    for (int j = 0; j < My; ++j) {
-        const zc* p = H4 + ((size_t)j * 6) * d.Kz;        // kx = 0, iz = 0
-        fu = fma(-b1[j], p[1 * (size_t)d.Kz].re, fu);     // -B1 P_uv
-        fw = fma(-b1[j], p[4 * (size_t)d.Kz].re, fw);     // -B1 P_vw
-    }
-    Fpsi[2 + m] = fu + d.G * s02c0[m];
-    Fpsi[cs + 2 + m] = fw;
-}
 
 // ---- SBDF stage ---------------------------------------------------------------------------
 struct StageArgs {
@@ -591,15 +643,33 @@ bool make_plan(int M, Plan& p) {
     return m == 1 && M >= 2;
 }
 
-size_t lds_a(const Dims& d) { return (size_t)2 * d.N * MODES * sizeof(zc); }
-size_t lds_b(const Dims& d) { return (size_t)7 * d.Mz * sizeof(zc); }
-size_t lds_c(const Dims& d) {
-    return ((size_t)d.Mx + 3 * d.Kx * XL + 2 * XL * d.Mx) * sizeof(zc) + (size_t)3 * XL * d.Mx * sizeof(double);
+size_t lds_a(const Dims& d) { return (size_t)2 * 2 * Fold(d.N, d.My).KS * 17 * sizeof(double); }
+
+template <int NRT>
+void launch_yfwd_n(dim3 g, size_t lds, hipStream_t s, const zc* h4, const double* Am, const double* s02c0, double* F,
+                   const Dims& d) {
+    hipLaunchKernelGGL((k_pt_yfwd<false, NRT>), g, dim3(64), lds, s, h4, Am, s02c0, F, d);
+    hipLaunchKernelGGL((k_pt_yfwd<true, NRT>), g, dim3(64), lds, s, h4, Am, s02c0, F, d);
 }
-size_t lds_d(const Dims& d) { return (size_t)13 * d.Mz * sizeof(zc); }
-size_t lds_e(const Dims& d) {
-    return (size_t)JC * MODES * 5 * sizeof(zc) + (size_t)MODES * 2 * (d.nP + d.nS) * sizeof(double);
+
+// the two y-forward launches, instantiated per row-tile count (N <= 128: at most 8 tiles)
+void launch_yfwd(dim3 g, size_t lds, hipStream_t s, const zc* h4, const double* Am, const double* s02c0, double* F,
+                 const Dims& d) {
+    const Fold fo(d.N, d.My);
+    switch ((fo.RE + fo.RO) / 16) {
+        case 2: launch_yfwd_n<2>(g, lds, s, h4, Am, s02c0, F, d); break;
+        case 3: launch_yfwd_n<3>(g, lds, s, h4, Am, s02c0, F, d); break;
+        case 4: launch_yfwd_n<4>(g, lds, s, h4, Am, s02c0, F, d); break;
+        case 5: launch_yfwd_n<5>(g, lds, s, h4, Am, s02c0, F, d); break;
+        case 6: launch_yfwd_n<6>(g, lds, s, h4, Am, s02c0, F, d); break;
+        case 7: launch_yfwd_n<7>(g, lds, s, h4, Am, s02c0, F, d); break;
+        default: launch_yfwd_n<8>(g, lds, s, h4, Am, s02c0, F, d); break;
+    }
 }
+size_t lds_b(const Dims& d) { return ((size_t)1 + 2 * ZK) * d.Mz * sizeof(zc); }
+size_t lds_c(const Dims& d) { return (size_t)8 * d.Mx * sizeof(zc); }
+size_t lds_d(const Dims& d) { return lds_b(d); }
+size_t lds_e(const Dims& d) { return (size_t)16 * (d.nP > d.nS ? d.nP : d.nS) * sizeof(double); }
 
 }  // namespace
 
@@ -628,8 +698,8 @@ int cn_chpt_explicit(const double* X, double* F, void* ws, size_t ws_bytes, cons
                       std::to_string(Mx) + ", " + std::to_string(Mz) + ")");
         return CN_EUNSUPPORTED;
     }
-    if (Ny > 16 * 7) {
-        cn::set_error("cn_chpt_explicit: Ny > 112 not supported by the y-forward kernel");
+    if (Ny > 128 || My > 256) {
+        cn::set_error("cn_chpt_explicit: Ny > 128 or My > 256 not supported by the y transforms");
         return CN_EUNSUPPORTED;
     }
     if (d.nP > 128) {
@@ -659,14 +729,14 @@ int cn_chpt_explicit(const double* X, double* F, void* ws, size_t ws_bytes, cons
         h4 = h2;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 gy((d.Kz + MODES - 1) / MODES, d.Kx);
-    hipLaunchKernelGGL(k_pt_yback, gy, dim3(256), lds_a(d), s, X, Tb, h1, d);
-    hipLaunchKernelGGL(k_pt_zback, dim3(My, d.Kx), dim3(64), lds_b(d), s, h1, static_cast<const zc*>(Wz), h2, d, pz);
-    hipLaunchKernelGGL(k_pt_xline, dim3((Mz + XL - 1) / XL, My), dim3(256), lds_c(d), s, h2,
-                       static_cast<const zc*>(Wx), h3, d, px);
-    hipLaunchKernelGGL(k_pt_zfwd, dim3(My, d.Kx), dim3(128), lds_d(d), s, h3, static_cast<const zc*>(Wz), h4, d, pz);
-    hipLaunchKernelGGL(k_pt_yfwd, gy, dim3(256), lds_e(d), s, h4, Am, F, d);
-    hipLaunchKernelGGL(k_pt_mean, dim3(1), dim3(128), 0, s, h4, Am, s02c0, F, d);
+    const dim3 gy((d.Kz + YM - 1) / YM, d.Kx);
+    hipLaunchKernelGGL(k_pt_yback, gy, dim3(64 * (Fold(d.N, d.My).MHP / 16)), lds_a(d), s, X, Tb, h1, d);
+    const dim3 gz((d.Kx + ZK - 1) / ZK, My);
+    hipLaunchKernelGGL(k_pt_zback, gz, dim3(512), lds_b(d), s, h1, static_cast<const zc*>(Wz), h2, d, pz);
+    hipLaunchKernelGGL(k_pt_xline, dim3((Mz + 1) / 2, My), dim3(128), lds_c(d), s, h2, static_cast<const zc*>(Wx),
+                       h3, d, px);
+    hipLaunchKernelGGL(k_pt_zfwd, gz, dim3(512), lds_d(d), s, h3, static_cast<const zc*>(Wz), h4, d, pz);
+    launch_yfwd(gy, lds_e(d), s, h4, Am, s02c0, F, d);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         cn::set_error(std::string("cn_chpt_explicit: ") + hipGetErrorString(e));

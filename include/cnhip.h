@@ -226,8 +226,9 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
  * products on the Mx x My x Mz grid (Mx, Mz of the form 2^a 3^b).  Replaces the reference's
  * evaluation of those right-hand sides (python/conatus/timestep/spectral.py:94-122 with the 3D
  * transforms of python/conatus/timestep/xp_transforms.py:197-548).  Tables: Tb (My x Ny x 2:
- * T_n(y_j), T_n'(y_j) at the ascending Gauss-Chebyshev nodes), Am (5 x Ny x My: S14 D01 Fy,
- * S24 D02 Fy, S04 Fy, S02 Fy, S12 D01 Fy with Fy the Gauss-Chebyshev forward transform), s02c0
+ * T_n(y_j), T_n'(y_j) at the ascending Gauss-Chebyshev nodes), Am (5 x My x Ny, node-major: the
+ * transposes of S14 D01 Fy, S24 D02 Fy, S04 Fy, S02 Fy, S12 D01 Fy with Fy the Gauss-Chebyshev
+ * forward transform), s02c0
  * (column 0 of S02), Wx, Wz (Mx, Mz complex twiddles exp(-2 pi i k / M)).  Workspace: ws of at
  * least cn_chpt_workspace_bytes(...) bytes.  Six launches on `stream`. */
 size_t cn_chpt_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz);

@@ -40,6 +40,7 @@
 
 #include "../../include/cnhip.h"
 #include "cn_common.h"
+#include "fft_reg.h"
 
 using cn::zc;
 
@@ -407,6 +408,204 @@ k_pt_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__
     }
 }
 
+// ---- register-split FFTs for the z and x transforms -------------------------------------------
+// A line of M = FA FB points is transformed by a group of 16 lanes (t = lane & 15): lanes t < FB
+// take the samples n = FB n1 + t (n1 < FA), do an FA-point DFT in registers, multiply by
+// exp(-+2 pi i t k1 / M) (the table W, forward signs) and write row k1 of the line's exchange
+// tile at column t (rows padded to FB + 1); lanes t < FA then read row t and do an FB-point DFT,
+// which leaves output k = t + FA k2 (k2 < FB) in register k2.  One LDS transpose per transform.
+template <int FA, int FB>
+struct Split {
+    static constexpr int M = FA * FB, RS = FB + 1, XS = FA * RS, NV = FA > FB ? FA : FB;
+};
+
+template <int FA, int FB, bool INV>
+__device__ __forceinline__ void split_first(zc* v, int t, const zc* __restrict__ W, zc* xt) {
+    using S = Split<FA, FB>;
+    cnf::dft<FA, INV>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < FA; ++k1) {
+        const int e = (t * k1) % S::M;
+        const zc w = W[e];
+        const zc x = e == 0 ? v[k1] : zmul(v[k1], INV ? zconj(w) : w);
+        xt[k1 * S::RS + t] = x;
+    }
+}
+
+template <int FA, int FB, bool INV>
+__device__ __forceinline__ void split_second(zc* v, int t, const zc* xt) {
+    using S = Split<FA, FB>;
+#pragma unroll
+    for (int n2 = 0; n2 < FB; ++n2) v[n2] = xt[t * S::RS + n2];
+    cnf::dft<FB, INV>(v);
+}
+
+// bin b of an M-point grid -> mode index iz (kz = b or b - M kept when |kz| < Kzh), or -1
+__device__ __forceinline__ int iz_of_bin(int b, int M, int Kzh, int Kz) {
+    return b < Kzh ? b : (b > M - Kzh ? b - M + Kz : -1);
+}
+
+// B (register split): workgroup = 16 consecutive kx of one (y_j, field), one line per 16 lanes;
+// the outputs are staged as [m][17] and written in runs of 16 kx.
+template <int FA, int FB>
+__global__ void __launch_bounds__(256)
+k_pt_zback_s(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, Dims d) {
+    using S = Split<FA, FB>;
+    constexpr int M = S::M;
+    extern __shared__ __align__(16) char lds[];
+    zc* sh = reinterpret_cast<zc*>(lds);                    // 16 exchange tiles, then [M][17]
+    const int tid = threadIdx.x, L = tid >> 4, t = tid & 15;
+    const int kx0 = blockIdx.x * 16, j = blockIdx.y, f = blockIdx.z, kx = kx0 + L;
+    const bool on = kx < d.Kx;
+    zc v[S::NV];
+    if (t < FB) {
+        const zc* src = H1 + (((size_t)(on ? kx : 0) * d.My + j) * 3 + f) * d.Kz;
+#pragma unroll
+        for (int n1 = 0; n1 < FA; ++n1) {
+            const int iz = iz_of_bin(FB * n1 + t, M, d.Kzh, d.Kz);
+            v[n1] = (on && iz >= 0) ? src[iz] : zc{0.0, 0.0};
+        }
+        split_first<FA, FB, true>(v, t, Wz, sh + L * S::XS);
+    }
+    __syncthreads();
+    if (t < FA) split_second<FA, FB, true>(v, t, sh + L * S::XS);
+    __syncthreads();
+    if (t < FA) {
+#pragma unroll
+        for (int k2 = 0; k2 < FB; ++k2) sh[(t + FA * k2) * 17 + L] = v[k2];
+    }
+    __syncthreads();
+    const int nk = (d.Kx - kx0) < 16 ? (d.Kx - kx0) : 16;
+    for (int q = tid; q < M * 16; q += blockDim.x) {
+        const int m = q >> 4, l = q & 15;
+        if (l < nk) H2[(((size_t)j * M + m) * 3 + f) * d.Kx + kx0 + l] = sh[m * 17 + l];
+    }
+}
+
+// D (register split): workgroup = 16 consecutive kx of one (y_j, product); the input lines are
+// gathered from H3 in runs of 16 kx into [m][17], transformed, and each lane writes its kept bins.
+template <int FA, int FB>
+__global__ void __launch_bounds__(256)
+k_pt_zfwd_s(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, Dims d) {
+    using S = Split<FA, FB>;
+    constexpr int M = S::M;
+    extern __shared__ __align__(16) char lds[];
+    zc* sh = reinterpret_cast<zc*>(lds);                    // [M][17], then 16 exchange tiles
+    const int tid = threadIdx.x, L = tid >> 4, t = tid & 15;
+    const int kx0 = blockIdx.x * 16, j = blockIdx.y, f = blockIdx.z, kx = kx0 + L;
+    const int nk = (d.Kx - kx0) < 16 ? (d.Kx - kx0) : 16;
+    for (int q = tid; q < M * 16; q += blockDim.x) {
+        const int m = q >> 4, l = q & 15;
+        sh[m * 17 + l] = l < nk ? H3[(((size_t)j * M + m) * 6 + f) * d.Kx + kx0 + l] : zc{0.0, 0.0};
+    }
+    __syncthreads();
+    zc v[S::NV];
+    if (t < FB) {
+#pragma unroll
+        for (int n1 = 0; n1 < FA; ++n1) v[n1] = sh[(FB * n1 + t) * 17 + L];
+    }
+    __syncthreads();
+    if (t < FB) split_first<FA, FB, false>(v, t, Wz, sh + L * S::XS);
+    __syncthreads();
+    if (t < FA) {
+        split_second<FA, FB, false>(v, t, sh + L * S::XS);
+        if (kx < d.Kx) {
+            const double inv = 1.0 / M;
+            zc* dst = H4 + (((size_t)kx * d.My + j) * 6 + f) * d.Kz;
+#pragma unroll
+            for (int k2 = 0; k2 < FB; ++k2) {
+                const int iz = iz_of_bin(t + FA * k2, M, d.Kzh, d.Kz);
+                if (iz >= 0) dst[iz] = zscale(inv, v[k2]);
+            }
+        }
+    }
+}
+
+// C (register split): one wave per (y_j; z_m, z_m+1).  Inverse: groups 0, 1, 2 transform
+// u_m + i v_m, u_m+1 + i v_m+1 and w_m + i w_m+1 (Hermitian completions) into the physical lines
+// ph[3][M]; then per z line the three product pairs (uu, uv), (uw, vv), (vw, ww), one per group,
+// are transformed forward, staged in natural order and split into the two real spectra.
+template <int FA, int FB>
+__global__ void __launch_bounds__(64)
+k_pt_xline_s(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, Dims d) {
+    using S = Split<FA, FB>;
+    constexpr int M = S::M, XE = S::XS > M ? S::XS : M;
+    extern __shared__ __align__(16) char lds[];
+    zc* ex = reinterpret_cast<zc*>(lds);                    // [4][XE] exchange tiles / spectra
+    zc* ph = ex + 4 * XE;                                   // [3][M] physical lines
+    const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+    const int Kx = d.Kx, Mz = d.Mz;
+    const int m0 = 2 * blockIdx.x, j = blockIdx.y;
+    const int two = (Mz - m0) < 2 ? 1 : 2;
+    const zc* in = H2 + ((size_t)j * Mz + m0) * 3 * Kx;     // [line][f][kx]
+    zc* xt = ex + g * XE;
+    zc v[S::NV];
+    if (g < 3 && t < FB) {
+#pragma unroll
+        for (int n1 = 0; n1 < FA; ++n1) {
+            const int k = FB * n1 + t;
+            int kk = -1;
+            bool cj = false;
+            if (k < Kx) kk = k;
+            else if (k > M - Kx) { kk = M - k; cj = true; }
+            zc A{0.0, 0.0}, B{0.0, 0.0};
+            if (kk >= 0) {
+                if (g < 2) {
+                    if (g < two) {
+                        A = in[((size_t)g * 3 + 0) * Kx + kk];
+                        B = in[((size_t)g * 3 + 1) * Kx + kk];
+                    }
+                } else {
+                    A = in[(size_t)2 * Kx + kk];
+                    if (two == 2) B = in[((size_t)3 + 2) * Kx + kk];
+                }
+                if (kk == 0) { A.im = 0.0; B.im = 0.0; }
+                if (cj) { A = zconj(A); B = zconj(B); }
+            }
+            v[n1] = zadd(A, zmuli(B));
+        }
+        split_first<FA, FB, true>(v, t, Wx, xt);
+    }
+    __syncthreads();
+    if (g < 3 && t < FA) {
+        split_second<FA, FB, true>(v, t, xt);
+#pragma unroll
+        for (int k2 = 0; k2 < FB; ++k2) ph[g * M + t + FA * k2] = v[k2];
+    }
+    __syncthreads();
+    const double h = 0.5 / M;
+    for (int zl = 0; zl < two; ++zl) {
+        if (g < 3 && t < FB) {
+#pragma unroll
+            for (int n1 = 0; n1 < FA; ++n1) {
+                const int x = FB * n1 + t;
+                const zc uv = ph[zl * M + x], wp = ph[2 * M + x];
+                const double u = uv.re, vv = uv.im, w = zl ? wp.im : wp.re;
+                v[n1] = g == 0 ? zc{u * u, u * vv} : g == 1 ? zc{u * w, vv * vv} : zc{vv * w, w * w};
+            }
+            split_first<FA, FB, false>(v, t, Wx, xt);
+        }
+        __syncthreads();
+        if (g < 3 && t < FA) split_second<FA, FB, false>(v, t, xt);
+        __syncthreads();
+        if (g < 3 && t < FA) {
+#pragma unroll
+            for (int k2 = 0; k2 < FB; ++k2) xt[t + FA * k2] = v[k2];
+        }
+        __syncthreads();
+        if (g < 3) {
+            zc* out = H3 + ((size_t)j * Mz + m0 + zl) * 6 * Kx;
+            for (int k = t; k < Kx; k += 16) {
+                const zc Zk = xt[k];
+                const zc Zc = zconj(xt[k == 0 ? 0 : M - k]);
+                out[(size_t)(2 * g) * Kx + k] = zscale(h, zadd(Zk, Zc));
+                out[(size_t)(2 * g + 1) * Kx + k] = zscale(-h, zmuli(zsub(Zk, Zc)));
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- E: y-forward and the curl forms ----------------------------------------------------------
 // Tables Am[5][MhT][RE + RO]: A1 = S14 D01 Fy, A2 = S24 D02 Fy, A0 = S04 Fy, B0 = S02 Fy,
 // B1 = S12 D01 Fy at the first Mh nodes, node-major, rows permuted even first (m = 2 i at i < RE)
@@ -645,6 +844,53 @@ bool make_plan(int M, Plan& p) {
 
 size_t lds_a(const Dims& d) { return (size_t)2 * 2 * Fold(d.N, d.My).KS * 17 * sizeof(double); }
 
+// Register splits M = FA x FB of the z and x transforms; other 2^a 3^b sizes use the Stockham
+// kernels.
+#define CN_PT_SPLITS(X) X(12, 4, 3) X(18, 6, 3) X(24, 4, 6) X(36, 6, 6) X(48, 8, 6) X(72, 6, 12) \
+    X(96, 8, 12) X(144, 12, 12) X(192, 16, 12)
+
+template <int FA, int FB>
+size_t lds_zs() {
+    using S = Split<FA, FB>;
+    return (size_t)(16 * S::XS > S::M * 17 ? 16 * S::XS : S::M * 17) * sizeof(zc);
+}
+template <int FA, int FB>
+size_t lds_xs() {
+    using S = Split<FA, FB>;
+    return (size_t)(4 * (S::XS > S::M ? S::XS : S::M) + 3 * S::M) * sizeof(zc);
+}
+
+bool zback_split(const Dims& d, hipStream_t s, const zc* h1, const zc* Wz, zc* h2) {
+    const dim3 g((d.Kx + 15) / 16, d.My, 3);
+    switch (d.Mz) {
+#define CN_X(M, FA, FB) \
+    case M: hipLaunchKernelGGL((k_pt_zback_s<FA, FB>), g, dim3(256), (lds_zs<FA, FB>()), s, h1, Wz, h2, d); return true;
+        CN_PT_SPLITS(CN_X)
+#undef CN_X
+        default: return false;
+    }
+}
+bool zfwd_split(const Dims& d, hipStream_t s, const zc* h3, const zc* Wz, zc* h4) {
+    const dim3 g((d.Kx + 15) / 16, d.My, 6);
+    switch (d.Mz) {
+#define CN_X(M, FA, FB) \
+    case M: hipLaunchKernelGGL((k_pt_zfwd_s<FA, FB>), g, dim3(256), (lds_zs<FA, FB>()), s, h3, Wz, h4, d); return true;
+        CN_PT_SPLITS(CN_X)
+#undef CN_X
+        default: return false;
+    }
+}
+bool xline_split(const Dims& d, hipStream_t s, const zc* h2, const zc* Wx, zc* h3) {
+    const dim3 g((d.Mz + 1) / 2, d.My);
+    switch (d.Mx) {
+#define CN_X(M, FA, FB) \
+    case M: hipLaunchKernelGGL((k_pt_xline_s<FA, FB>), g, dim3(64), (lds_xs<FA, FB>()), s, h2, Wx, h3, d); return true;
+        CN_PT_SPLITS(CN_X)
+#undef CN_X
+        default: return false;
+    }
+}
+
 template <int NRT>
 void launch_yfwd_n(dim3 g, size_t lds, hipStream_t s, const zc* h4, const double* Am, const double* s02c0, double* F,
                    const Dims& d) {
@@ -732,10 +978,14 @@ int cn_chpt_explicit(const double* X, double* F, void* ws, size_t ws_bytes, cons
     const dim3 gy((d.Kz + YM - 1) / YM, d.Kx);
     hipLaunchKernelGGL(k_pt_yback, gy, dim3(64 * (Fold(d.N, d.My).MHP / 16)), lds_a(d), s, X, Tb, h1, d);
     const dim3 gz((d.Kx + ZK - 1) / ZK, My);
-    hipLaunchKernelGGL(k_pt_zback, gz, dim3(512), lds_b(d), s, h1, static_cast<const zc*>(Wz), h2, d, pz);
-    hipLaunchKernelGGL(k_pt_xline, dim3((Mz + 1) / 2, My), dim3(128), lds_c(d), s, h2, static_cast<const zc*>(Wx),
-                       h3, d, px);
-    hipLaunchKernelGGL(k_pt_zfwd, gz, dim3(512), lds_d(d), s, h3, static_cast<const zc*>(Wz), h4, d, pz);
+    const zc* wz = static_cast<const zc*>(Wz);
+    const zc* wx = static_cast<const zc*>(Wx);
+    if (!zback_split(d, s, h1, wz, h2))
+        hipLaunchKernelGGL(k_pt_zback, gz, dim3(512), lds_b(d), s, h1, wz, h2, d, pz);
+    if (!xline_split(d, s, h2, wx, h3))
+        hipLaunchKernelGGL(k_pt_xline, dim3((Mz + 1) / 2, My), dim3(128), lds_c(d), s, h2, wx, h3, d, px);
+    if (!zfwd_split(d, s, h3, wz, h4))
+        hipLaunchKernelGGL(k_pt_zfwd, gz, dim3(512), lds_d(d), s, h3, wz, h4, d, pz);
     launch_yfwd(gy, lds_e(d), s, h4, Am, s02c0, F, d);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

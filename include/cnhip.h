@@ -225,12 +225,16 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
  * phi and psi rows, the x-z means of Nx + G and Nz for U and W, zero elsewhere) from the six
  * products on the Mx x My x Mz grid (Mx, Mz of the form 2^a 3^b).  Replaces the reference's
  * evaluation of those right-hand sides (python/conatus/timestep/spectral.py:94-122 with the 3D
- * transforms of python/conatus/timestep/xp_transforms.py:197-548).  Tables: Tb (My x Ny x 2:
- * T_n(y_j), T_n'(y_j) at the ascending Gauss-Chebyshev nodes), Am (5 x My x Ny, node-major: the
- * transposes of S14 D01 Fy, S24 D02 Fy, S04 Fy, S02 Fy, S12 D01 Fy with Fy the Gauss-Chebyshev
- * forward transform), s02c0
- * (column 0 of S02), Wx, Wz (Mx, Mz complex twiddles exp(-2 pi i k / M)).  Workspace: ws of at
- * least cn_chpt_workspace_bytes(...) bytes.  Six launches on `stream`. */
+ * transforms of python/conatus/timestep/xp_transforms.py:197-548).  The y tables are folded by
+ * the parity of the Chebyshev polynomials over the symmetric Gauss nodes (y_{My-1-j} = -y_j),
+ * with Mh = ceil(My/2), KS = 8 ceil(ceil(Ny/2)/8), Mhs = 8 ceil(Mh/8), MHP = 16 ceil(Mh/16),
+ * RE = 16 ceil(ceil(Ny/2)/16), RO = 16 ceil(floor(Ny/2)/16), zero padded:
+ *   Tb [4][KS + 4][MHP]: T_{2k}(y_j), T_{2k+1}(y_j), T'_{2k}(y_j), T'_{2k+1}(y_j), j < Mh;
+ *   Am [5][Mhs + 4][RE + RO]: S14 D01 Fy, S24 D02 Fy, S04 Fy, S02 Fy, S12 D01 Fy (Fy the
+ *      Gauss-Chebyshev forward transform) at node j < Mh, row m at index m/2 (even m) or
+ *      RE + (m-1)/2 (odd m);
+ * s02c0 (column 0 of S02), Wx, Wz (Mx, Mz complex twiddles exp(-2 pi i k / M)).  Workspace: ws
+ * of at least cn_chpt_workspace_bytes(...) bytes.  Six launches on `stream`. */
 size_t cn_chpt_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz);
 int cn_chpt_explicit(const double* X, double* F, void* ws, size_t ws_bytes, const double* Tb, const double* Am,
                      const double* s02c0, const void* Wx, const void* Wz, int Nx, int Ny, int Nz, int Mx, int My,

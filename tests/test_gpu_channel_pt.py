@@ -44,8 +44,11 @@ def _device_F(pb, X):
     return out.cpu().numpy()
 
 
-@pytest.mark.parametrize("shape", [(8, 9, 8), (16, 17, 12), (32, 33, 32), (64, 65, 64)])
+@pytest.mark.parametrize("shape", [(8, 9, 8), (16, 17, 12), (32, 33, 32), (64, 65, 64), (256, 9, 4)])
 def test_explicit_term_matches_oracle(shape):
+    """Grids 12 .. 96 points take the register-split transforms; (256, 9, 4) has Mx = 384 and
+    Mz = 6, which take the Stockham kernels.  Its bar is 1e-12: with kx up to 127 the combination
+    a = kx^2 P_uu + .. - k^2 P_vv cancels terms ~1.6e4 times larger than the result."""
     nx, ny, nz = shape
     print(f"[chpt explicit] {shape}", flush=True)
     pb = ChannelPT3D(Nx=nx, Ny=ny, Nz=nz, Re=400.0, Lx=2 * np.pi, Lz=np.pi)
@@ -55,7 +58,7 @@ def test_explicit_term_matches_oracle(shape):
     ref = _oracle(pb).explicit(X)
     r = rel(F, ref)
     print(f"relative difference {r:.2e}", flush=True)
-    assert r < 1e-13
+    assert r < (1e-12 if nx > 128 else 1e-13)
     A, B = pb.split(F)
     assert not A[:, :, :4].any() and not B[:, :, :2].any()        # boundary rows
     for s in range(0, pb.P, pb.Kzh):                             # kz = 0 slots: no (kx, -0) mode

@@ -213,34 +213,21 @@ k_pt_yback(const double* __restrict__ X, const double* __restrict__ Tb, zc* __re
     const size_t plane = (size_t)fo.KT * fo.MHP;
     const double* tb = Tb + 16 * wave + (lane & 15);
     d4 fE = {0.0, 0.0, 0.0, 0.0}, fO = fE, dE = fE, dO = fE, sE = fE, sO = fE;
-    auto load = [&](int k0, double (&T)[4]) {
-        const double* t = tb + (size_t)(k0 + kq) * fo.MHP;
-        T[0] = t[0];
-        T[1] = t[plane];
-        T[2] = t[2 * plane];
-        T[3] = t[3 * plane];
-    };
-    auto mma = [&](int k0, const double (&T)[4]) {
+    for (int k0 = 0; k0 < fo.KS; k0 += 4) {
         const int k = k0 + kq;
+        const double* t = tb + (size_t)k * fo.MHP;
+        const double Te = t[0], To = t[plane], De = t[2 * plane], Do = t[3 * plane];
         const double* a0 = ca + (size_t)(2 * k) * 17;
         const double* b0 = cb + (size_t)(2 * k) * 17;
         const double ae = a0[col], ao = a0[17 + col];
         const double iae = sg * a0[col ^ 1], iao = sg * a0[17 + (col ^ 1)];
         const double ibe = sg * b0[col ^ 1], ibo = sg * b0[17 + (col ^ 1)];
-        fE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[0], ae, fE, 0, 0, 0);
-        fO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[1], ao, fO, 0, 0, 0);
-        dE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[3], iao, dE, 0, 0, 0);
-        dO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[2], iae, dO, 0, 0, 0);
-        sE = __builtin_amdgcn_mfma_f64_16x16x4f64(T[0], ibe, sE, 0, 0, 0);
-        sO = __builtin_amdgcn_mfma_f64_16x16x4f64(T[1], ibo, sO, 0, 0, 0);
-    };
-    double Ta[4], Tn[4];
-    load(0, Ta);
-    for (int k0 = 0; k0 < fo.KS; k0 += 8) {
-        load(k0 + 4, Tn);
-        mma(k0, Ta);
-        load(k0 + 8, Ta);
-        mma(k0 + 4, Tn);
+        fE = __builtin_amdgcn_mfma_f64_16x16x4f64(Te, ae, fE, 0, 0, 0);
+        fO = __builtin_amdgcn_mfma_f64_16x16x4f64(To, ao, fO, 0, 0, 0);
+        dE = __builtin_amdgcn_mfma_f64_16x16x4f64(Do, iao, dE, 0, 0, 0);
+        dO = __builtin_amdgcn_mfma_f64_16x16x4f64(De, iae, dO, 0, 0, 0);
+        sE = __builtin_amdgcn_mfma_f64_16x16x4f64(Te, ibe, sE, 0, 0, 0);
+        sO = __builtin_amdgcn_mfma_f64_16x16x4f64(To, ibo, sO, 0, 0, 0);
     }
     const int iz = iz0 + (col >> 1), part = col & 1;
     if (iz >= d.Kz) return;
@@ -618,9 +605,8 @@ k_pt_xline_s(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restric
 // The mean (kx = kz = 0) is the psi column pair with b = 0 and -s = -(Re P_uv, Re P_vw), plus
 // G S02 e0 in column 0: F_U = -B1 Re P_uv + G S02 e0, F_W = -B1 Re P_vw; its phi rows are zero.
 template <bool PSI, int NRT>
-__global__ void __launch_bounds__(64)
-k_pt_yfwd(const zc* __restrict__ H4, const double* __restrict__ Am, const double* __restrict__ s02c0,
-          double* __restrict__ F, Dims d) {
+__device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const double* __restrict__ Am,
+                                            const double* __restrict__ s02c0, double* __restrict__ F, const Dims& d) {
     extern __shared__ __align__(16) char lds[];
     double* orow = reinterpret_cast<double*>(lds);           // [16][nrow]
     constexpr bool psi = PSI;
@@ -775,40 +761,49 @@ k_pt_sbdf(double* __restrict__ X, const double* __restrict__ s04, const double* 
     const int n = bucket == 0 ? d.nP : d.nS;
     const size_t off = bucket == 0 ? ((size_t)c * d.P + s) * d.nP : d.size_phi() + ((size_t)c * d.P + s) * d.nS;
     double* x = X + off;
-    if (row < n) xl[row] = x[row];
-    __syncthreads();
-    if (row >= n) return;
+    const bool on = row < n;
     const int kx = s / d.Kzh, kzv = s - kx * d.Kzh;
     const double ax = kx * d.kx0, az = kzv * d.kz0, k2 = ax * ax + az * az;
-    double mx = 0.0;
-    const int nb = bucket == 0 ? 4 : 2;
-    if (row >= nb) {
-        const int m = row - nb, N = d.N;
+    const int nb = bucket == 0 ? 4 : 2, m = row - nb, N = d.N;
+    // every global operand is requested before the barrier (one memory round trip per wave)
+    const double xv = on ? x[row] : 0.0;
+    double fv[4], mv[3], cf[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) fv[j] = (on && j < a.q) ? a.f[j][off + row] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) mv[j] = (on && j + 1 < a.q) ? a.mx[j][off + row] : 0.0;
+    if (on && row >= nb) {
         if (bucket == 0) {
             if (s != 0) {
                 const double k4 = k2 * k2;
-                for (int t = 0; t < 5; ++t) {
-                    const int col = m + 2 * t;
-                    if (col >= N) break;
-                    const double coef = (-k2) * s2d[m * 5 + t] + k4 * s04[m * 5 + t];
-                    mx = mx + coef * xl[col];
-                }
+#pragma unroll
+                for (int t = 0; t < 5; ++t) cf[t] = (-k2) * s2d[m * 5 + t] + k4 * s04[m * 5 + t];
             }
         } else {
-            for (int t = 0; t < 3; ++t) {
-                const int col = m + 2 * t;
-                if (col >= N) break;
-                const double coef = s == 0 ? s02[m * 3 + t] : (-k2) * s02[m * 3 + t];
-                mx = mx + coef * xl[col];
-            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) cf[t] = s == 0 ? s02[m * 3 + t] : (-k2) * s02[m * 3 + t];
+        }
+    }
+    xl[row] = xv;
+    __syncthreads();
+    if (!on) return;
+    double mx = 0.0;
+    if (row >= nb) {
+        const int nt = bucket == 0 ? (s != 0 ? 5 : 0) : 3;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            const int col = m + 2 * t;
+            if (t >= nt || col >= N) break;
+            mx = mx + cf[t] * xl[col];
         }
     }
     a.mx0[off + row] = mx;
     double rhs = 0.0;
-    for (int j = 1; j <= a.q; ++j) {
-        const double fj = a.f[j - 1][off + row];
-        const double mj = j == 1 ? mx : a.mx[j - 2][off + row];
-        const double t1 = a.c[j] * fj, t2 = a.adt[j] * mj;
+#pragma unroll
+    for (int j = 1; j <= 4; ++j) {
+        if (j > a.q) break;
+        const double mj = j == 1 ? mx : mv[j - 2];
+        const double t1 = a.c[j] * fv[j - 1], t2 = a.adt[j] * mj;
         rhs = rhs + (t1 - t2);
     }
     x[row] = rhs;
@@ -841,6 +836,16 @@ bool make_plan(int M, Plan& p) {
     while (m % 3 == 0 && p.n < MAXST) { p.r[p.n++] = 3; m /= 3; }
     return m == 1 && M >= 2;
 }
+
+// E launch: blockIdx.z = 0 the phi bucket, 1 the psi bucket (their waves share the SIMDs)
+template <int NRT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_pt_yfwd(const zc* __restrict__ H4, const double* __restrict__ Am, const double* __restrict__ s02c0,
+          double* __restrict__ F, Dims d) {
+    if (blockIdx.z == 0) yfwd_bucket<false, NRT>(H4, Am, s02c0, F, d);
+    else yfwd_bucket<true, NRT>(H4, Am, s02c0, F, d);
+}
+
 
 size_t lds_a(const Dims& d) { return (size_t)2 * 2 * Fold(d.N, d.My).KS * 17 * sizeof(double); }
 
@@ -894,11 +899,10 @@ bool xline_split(const Dims& d, hipStream_t s, const zc* h2, const zc* Wx, zc* h
 template <int NRT>
 void launch_yfwd_n(dim3 g, size_t lds, hipStream_t s, const zc* h4, const double* Am, const double* s02c0, double* F,
                    const Dims& d) {
-    hipLaunchKernelGGL((k_pt_yfwd<false, NRT>), g, dim3(64), lds, s, h4, Am, s02c0, F, d);
-    hipLaunchKernelGGL((k_pt_yfwd<true, NRT>), g, dim3(64), lds, s, h4, Am, s02c0, F, d);
+    hipLaunchKernelGGL((k_pt_yfwd<NRT>), dim3(g.x, g.y, 2), dim3(64), lds, s, h4, Am, s02c0, F, d);
 }
 
-// the two y-forward launches, instantiated per row-tile count (N <= 128: at most 8 tiles)
+// the y-forward launch, instantiated per row-tile count (N <= 128: at most 8 tiles)
 void launch_yfwd(dim3 g, size_t lds, hipStream_t s, const zc* h4, const double* Am, const double* s02c0, double* F,
                  const Dims& d) {
     const Fold fo(d.N, d.My);

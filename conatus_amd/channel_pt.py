@@ -454,8 +454,10 @@ class ChannelPTSolver:
     level, the fused stage (M X and the right-hand side over X in place) and one in-place solve
     of each bucket: the 4 P phi and 4 P psi right-hand sides against P real factorizations each
     (the modes (kx, kz) and (kx, -kz) share theirs).  ``run_graph`` replays full-order steps from
-    captured HIP graphs, one per history ring phase.  The clock is t_origin + (iteration -
-    it_origin) dt, and the state is checked finite every ``check_finite`` steps."""
+    captured HIP graphs: one per whole turn of the history ring (order steps) while at least a
+    turn is left, one per ring phase for the rest.  The clock is t_origin + (iteration -
+    it_origin) dt, and the state is checked finite every ``check_finite`` steps (``run``) or at
+    the end of the call (``run_graph``).""" 
 
     def __init__(self, problem: ChannelPT3D, scheme: str, dt: float, pivot: bool = True, mode: str | None = None,
                  check_finite=True):
@@ -483,7 +485,6 @@ class ChannelPTSolver:
         self._ring = [(z(), z()) for _ in range(self.order)]        # (M X, F) levels
         self._factors: dict = {}
         self._graphs: dict = {}
-        self._side = torch.cuda.Stream()
         self.t, self.t_origin, self.it_origin = 0.0, 0.0, 0
         self._finite_ok = None
         self.reset()
@@ -533,12 +534,13 @@ class ChannelPTSolver:
         return f
 
     # -- steps -------------------------------------------------------------------------------------
-    def _next_levels(self):
-        if len(self._levels) < self.order:
-            used = set(self._levels)
+    def _next_levels(self, levels=None):
+        levels = self._levels if levels is None else levels
+        if len(levels) < self.order:
+            used = set(levels)
             slot = next(i for i in range(self.order) if i not in used)
-            return [slot] + self._levels
-        return [self._levels[-1]] + self._levels[:-1]
+            return [slot] + levels
+        return [levels[-1]] + levels[:-1]
 
     def _launch(self, levels) -> None:
         from . import _lib
@@ -559,14 +561,10 @@ class ChannelPTSolver:
                                            v(self._s04.data_ptr()), v(self._s2d.data_ptr()), v(self._s02.data_ptr()),
                                            pb.Nx, pb.Ny, pb.Nz, float(pb.kx0), float(pb.kz0), v(stream))
         _lib.check(st, "ChannelPTSolver (stage)")
-        # the two buckets are independent: the psi solve runs on a second stream beside the phi
-        # solve (two branches of a captured graph)
-        (hphi, ophi, sphi), (hpsi, opsi, spsi) = self._factor(a[0] / self.dt)
-        main = torch.cuda.current_stream()
-        self._side.wait_stream(main)
-        hpsi.solve_ptr(self.X.data_ptr() + 8 * opsi, 4, spsi, False, self._side.cuda_stream)
-        hphi.solve_ptr(self.X.data_ptr() + 8 * ophi, 4, sphi, False, stream)
-        main.wait_stream(self._side)
+        # the buckets one after the other on this stream: each solve fills the CUs with its own
+        # LDS-sized workgroups, so two streams would only add fork and join gaps
+        for h, off, st in self._factor(a[0] / self.dt):
+            h.solve_ptr(self.X.data_ptr() + 8 * off, 4, st, False, stream)
 
     def _continuation(self) -> None:
         torch = _torch()
@@ -615,6 +613,24 @@ class ChannelPTSolver:
             a, _ = self._coef(self.order)
             self._factor(a[0] / self.dt)
         while done < steps:
+            if len(self._levels) == self.order and steps - done >= self.order:
+                # a whole turn of the history ring (order steps) in one graph: one replay
+                # instead of order, so the launch gaps between replays fall away
+                seq, lv = [], self._levels
+                for _ in range(self.order):
+                    lv = self._next_levels(lv)
+                    seq.append(lv)
+                key = ("turn",) + tuple(seq[0])
+                g = self._graphs.get(key)
+                if g is None:
+                    g = capture_graph(lambda sq=seq: [self._launch(x) for x in sq])
+                    self._graphs[key] = g
+                g.replay()
+                self._levels = seq[-1]
+                self.iteration += self.order
+                self.t = self._clock()
+                done += self.order
+                continue
             levels = self._next_levels()
             key = tuple(levels)
             g = self._graphs.get(key)

@@ -400,7 +400,7 @@ void run_factor(cn_bb& h, hipStream_t s) {
         // done above
     } else if (fs) {
         const int nr = nr_for(h.r);
-        fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(192), streamed_lds(bb, spx, (int)h.esz), s, h.Uf.p,
+        fs(dim3((h.B + IL - 1) / IL, (h.r + nr - 1) / nr), dim3(SWEEP_THREADS), streamed_lds(bb, spx, (int)h.esz), s, h.Uf.p,
            h.Vt.p, h.W.p, h.S.p,
            h.B, h.n, h.r, h.US, bb);
         hip_check(hipGetLastError(), "k_border_s launch");
@@ -724,7 +724,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s, void* ws = nullpt
             SolveSFn fs = (h.dtype == CN_C128) ? solve_s_fn<zc>(h.KLP, h.WU, nr, spx, h.r)
                                                : solve_s_fn<double>(h.KLP, h.WU, nr, spx, h.r);
             const size_t off = (size_t)k0 * (size_t)rs * h.esz;
-            fs(dim3((h.B + IL - 1) / IL), dim3(192), streamed_lds(bufbytes, spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
+            fs(dim3((h.B + IL - 1) / IL), dim3(SWEEP_THREADS), streamed_lds(bufbytes, spx, (int)h.esz), s, h.Uf.p, h.Lf.p,
                static_cast<const int*>(h.piv.p), h.Vt.p, h.W.p, h.S.p, static_cast<const int*>(h.spiv.p),
                static_cast<const int*>(h.perm.p), ws, bb + off, xb + off, rs, h.B, h.n, h.r, h.US, bufbytes);
             hip_check(hipGetLastError(), "k_solve_s launch");

@@ -534,6 +534,7 @@ struct LaneSlot;
 template <>
 struct LaneSlot<zc> {
     static constexpr int BYTES = 64 * 16;
+    static constexpr int LOADS = 1;            // vector memory instructions per load()
     __device__ static void load(const zc* src, char* slot) {
         __builtin_amdgcn_global_load_lds(static_cast<const void*>(src), (lds_ptr)slot, 16, 0, 0);
     }
@@ -544,6 +545,7 @@ struct LaneSlot<zc> {
 template <>
 struct LaneSlot<double> {
     static constexpr int BYTES = 64 * 8;
+    static constexpr int LOADS = 2;
     __device__ static void load(const double* src, char* slot) {
         const char* s = reinterpret_cast<const char*>(src);
         __builtin_amdgcn_global_load_lds(static_cast<const void*>(s), (lds_ptr)slot, 4, 0, 0);
@@ -622,8 +624,9 @@ __device__ __forceinline__ int bwd_dma_f(const T* __restrict__ Uf, const T* __re
     dma16(Uf + (ch * n * US + (size_t)jlo * US) * IL, buf, cnt * US * CE, lane);
     return r * dma16_count(cnt * CE) + dma16_count(cnt * US * CE);
 }
+// (returns the vector memory instructions issued: none when no lane of the wave is live)
 template <class T, int SPX, int RB>
-__device__ __forceinline__ void bwd_dma_l(char* buf, int n, int US, int blk, const T* in, size_t in_st, bool live) {
+__device__ __forceinline__ int bwd_dma_l(char* buf, int n, int US, int blk, const T* in, size_t in_st, bool live) {
     constexpr int CE = IL * (int)sizeof(T);
     const int jhi = n - 1 - blk * SPX;
     const int jlo = max(0, jhi - SPX + 1);
@@ -633,6 +636,7 @@ __device__ __forceinline__ void bwd_dma_l(char* buf, int n, int US, int blk, con
         for (int s = 0; s < SPX; ++s)
             if (jhi - s >= jlo) LaneSlot<T>::load(in + (size_t)(jhi - s) * in_st, buf + bU + bV + s * LaneSlot<T>::BYTES);
     }
+    return __ballot(live) != 0 ? (jhi - jlo + 1) * LaneSlot<T>::LOADS : 0;
 }
 
 // One backward step of a lane: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), t += V^T(:, j) out_j.
@@ -690,16 +694,19 @@ __device__ __forceinline__ T bwd_ops_step(const BwdOps<T, WU, RB>& op, T (&xw)[W
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 // ------------------------------------------------------------------------------------------
-// Three-wave workgroups of the streamed sweeps.  Wave 0 (compute) runs the recurrences from LDS
-// and leaves its results in an LDS stage; waves 1 and 2 issue every HBM access while block b is
-// computed: wave 1 the factor rows of the next blocks (whole-wave global_load_lds copies), wave 2
-// the lane operands of block b+1 and the stores of block b-1 from the other stage.  A wave that
-// issues memory instructions stalls while its own copies drain (at most 63 in flight per wave);
-// with the roles split, that stall holds up neither the arithmetic nor the other memory wave, and
-// one workgroup barrier per block hands buffers and stages over.
+// Four-wave workgroups of the streamed sweeps.  Wave 0 (compute) runs the recurrences from LDS
+// and leaves its results in an LDS stage; waves 1, 2 and 3 issue every HBM access while block b
+// is computed: wave 1 the factor rows of the next blocks (whole-wave global_load_lds copies),
+// wave 2 the lane operands of the next blocks (loads only), wave 3 the stores of block b-1 from
+// the other stage.  A wave that issues memory instructions stalls while its own copies drain (at
+// most 63 in flight per wave); with the roles split, that stall holds up neither the arithmetic
+// nor the other memory waves, and one workgroup barrier per block hands buffers and stages over.
 // LDS: [buffers][stage 0][stage 1], stage = stage_bytes (the larger of SPX lane slots and the
-// segment layout).  Where three buffers of a sweep fit, wave 1 keeps two blocks in flight and,
-// before each barrier, waits only until the block needed next has landed (counted waits).
+// segment layout).  Where three buffers of a sweep fit, waves 1 and 2 keep two blocks in flight
+// and, before each barrier, wait only until the block needed next has landed (counted waits:
+// loads complete in issue order, and each wave's count covers its own loads only).  Wave 3 waits
+// for its stores once, at the end of the sweep, before the next sweep may read them.
+constexpr int SWEEP_THREADS = 256;
 // ------------------------------------------------------------------------------------------
 __host__ __device__ constexpr int stage_bytes(int SPX, int esz) { return seg_bytes(SPX, esz); }
 // LDS buffers of a sweep: nb = 3 when three of the sweep's buffers fit the launch's buffer region
@@ -728,9 +735,9 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
     if (wave == 1) {   // factor rows of blocks 0 .. ahead-1 in flight, block 0 complete
         bwd_dma_f<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, lane);
         vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, lane) : 0);
-    } else if (wave == 2) {
+    } else if (wave == 2) {   // lane operands likewise
         bwd_dma_l<T, SPX, RB>(lds, n, US, 0, in, in_st, live);
-        dma_wait();
+        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_l<T, SPX, RB>(lds + bufbytes, n, US, 1, in, in_st, live) : 0);
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -744,13 +751,17 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
                                            : 0;
             vm_wait_le(ahead > 1 ? c : 0);
         } else if (wave == 2) {
+            // lane operands of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
+            const int c = b + ahead < nblk ? bwd_dma_l<T, SPX, RB>(lds + ((b + ahead) % nb) * bufbytes, n, US, b + ahead,
+                                                                    in, in_st, live)
+                                           : 0;
+            vm_wait_le(ahead > 1 ? c : 0);
+        } else if (wave == 3) {
             if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
                 const char* sb = stage + ((b - 1) & 1) * SB;
 #pragma unroll
                 for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
-            if (b + 1 < nblk) bwd_dma_l<T, SPX, RB>(lds + ((b + 1) % nb) * bufbytes, n, US, b + 1, in, in_st, live);
-            dma_wait();
         } else {
             const char* bb = lds + (b % nb) * bufbytes;
             char* sb = stage + (b & 1) * SB;
@@ -772,7 +783,7 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         }
         wg_sync();
     }
-    if (wave == 2) {
+    if (wave == 3) {
         if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
             const int jhi = n - 1 - (nblk - 1) * SPX;
             const char* sb = stage + ((nblk - 1) & 1) * SB;
@@ -850,10 +861,11 @@ __device__ __forceinline__ void fwd_block(T (&y)[KL + 1], char* sb, const char* 
 // contiguous out of place.  The work space holds entry j of every (system, rhs) lane of the
 // chunk in one contiguous row (coalesced); the right-hand side enters and x leaves by lane
 // segments (Seg): per block, each lane's SPX entries travel as one run.  Forward blocks without
-// any row interchange in the chunk skip the (select-based) interchanges.  Three waves: lanes
-// 0..63 compute, 64..127 copy factor rows, 128..191 move the lane data (same lane mapping).
+// any row interchange in the chunk skip the (select-based) interchanges.  Four waves: lanes
+// 0..63 compute, 64..127 copy factor rows, 128..191 load the lane data, 192..255 store it (same
+// lane mapping).
 template <class T, int KL, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(SWEEP_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restrict__ piv,
           const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
           const int* __restrict__ spiv, const int* __restrict__ perm, T* __restrict__ ws,
@@ -891,12 +903,26 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int fbuf = fwd_bytes(SPX, KL, (int)sizeof(T));
     const int fnb = sweep_nbuf(fbuf, region), fahead = fnb - 1;
     T y[KL + 1];
-    // wave 2: right-hand-side segments of forward block bn into buffer nb; pc = row of the
-    // right-hand side behind entry se of block bn, on return the one of block bn+1
-    int pc = (KL + 1 + se < n) ? perm[KL + 1 + se] : -1;
-    auto fwd_copy_l = [&](int bn, char* nb) {
-        const int j0 = bn * SPX;
+    // wave 2: right-hand-side segments of forward block bn into buffer nb.  Entry e of the block's
+    // segments is right-hand-side row perm[bn * SPX + KL + 1 + e] (or zero beyond n); the SPX
+    // entries of perm are read at wave-uniform addresses (scalar loads), so that no vector load
+    // result is waited for between the counted waits.  Returns the segment loads issued (a lower
+    // bound on the wave's vector memory instructions of the call).
+    auto fwd_copy_l = [&](int bn, char* nb) -> int {
+        const int base = bn * SPX + KL + 1;
+        int pc = -1;
+#pragma unroll
+        for (int e = 0; e < SPX; ++e) {
+            const int q = base + e < n ? perm[base + e] : -1;
+            pc = se == e ? q : pc;
+        }
         char* cs = nb + fL + fP;
+        int issued = 0;
+        if (__ballot(pc >= 0) != 0) {
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                if ((SG::SPI / NR) * i < seg_live) issued += LaneSlot<T>::LOADS;
+        }
         if (pc >= 0) {
             const T* src = bin + seg_off + pc;
 #pragma unroll
@@ -906,8 +932,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
 #pragma unroll
             for (int i = 0; i < NI; ++i) LaneSlot<T>::zero(cs + i * SG::PITCH, lane);
         }
-        const int jn = j0 + SPX + KL + 1 + se;    // entry se of block bn+1
-        pc = jn < n ? perm[jn] : -1;
+        return issued;
     };
     // wave 1: L rows and pivots of block bn (returns the number of instructions issued)
     auto fwd_copy_f = [&](int bn, char* nb) -> int {
@@ -931,9 +956,9 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     } else if (wave == 1) {   // blocks 0 .. ahead-1 in flight, block 0 complete
         fwd_copy_f(0, lds);
         vm_wait_le(fahead > 1 && nblk > 1 ? fwd_copy_f(1, lds + fbuf) : 0);
-    } else {
+    } else if (wave == 2) {
         fwd_copy_l(0, lds);
-        dma_wait();
+        vm_wait_le(fahead > 1 && nblk > 1 ? fwd_copy_l(1, lds + fbuf) : 0);
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -941,9 +966,10 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             const int c = b + fahead < nblk ? fwd_copy_f(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
             vm_wait_le(fahead > 1 ? c : 0);
         } else if (wave == 2) {
+            const int c = b + fahead < nblk ? fwd_copy_l(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
+            vm_wait_le(fahead > 1 ? c : 0);
+        } else if (wave == 3) {
             if (b >= 1 && live) fwd_store(b - 1);
-            if (b + 1 < nblk) fwd_copy_l(b + 1, lds + ((b + 1) % fnb) * fbuf);
-            dma_wait();
         } else {
             const char* fb = lds + (b % fnb) * fbuf;
             const char* cb = fb + fL + fP + my_slot;
@@ -969,7 +995,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         wg_sync();
     }
-    if (wave == 2) {
+    if (wave == 3) {
         if (live) fwd_store(nblk - 1);
         dma_wait();
     }
@@ -1000,13 +1026,14 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
     }
     T* xseg = xout + seg_off + se;
-    auto cor_copy_l = [&](int bn, char* nb) {        // wave 2: the work-space rows y of block bn
+    auto cor_copy_l = [&](int bn, char* nb) -> int { // wave 2: the work-space rows y of block bn
         const int j0 = bn * SPX;
         if (live) {
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
                 if (j0 + s < n) LaneSlot<T>::load(wl + (size_t)(j0 + s) * NL, nb + cW + s * GB);
         }
+        return __ballot(live) != 0 ? min(SPX, n - j0) * LaneSlot<T>::LOADS : 0;
     };
     auto cor_copy_f = [&](int bn, char* nb) -> int { // wave 1: the W rows of block bn
         const int j0 = bn * SPX;
@@ -1028,7 +1055,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_f(1, lds + cbuf) : 0);
     } else if (wave == 2) {
         cor_copy_l(0, lds);
-        dma_wait();
+        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_l(1, lds + cbuf) : 0);
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -1036,9 +1063,10 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             const int c = b + cahead < nblk ? cor_copy_f(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
             vm_wait_le(cahead > 1 ? c : 0);
         } else if (wave == 2) {
+            const int c = b + cahead < nblk ? cor_copy_l(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
+            vm_wait_le(cahead > 1 ? c : 0);
+        } else if (wave == 3) {
             if (b >= 1) cor_store(b - 1);
-            if (b + 1 < nblk) cor_copy_l(b + 1, lds + ((b + 1) % cnb) * cbuf);
-            dma_wait();
         } else {
             const char* cb = lds + (b % cnb) * cbuf;
             char* sb = stage + (b & 1) * SB + my_slot;
@@ -1052,16 +1080,16 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         wg_sync();
     }
-    if (wave == 2) {
+    if (wave == 3) {
         cor_store(nblk - 1);
         dma_wait();
     }
 }
 
 // Streamed completion of W = B^-1 U (in place over its forward half) and S = I + V^T W: lane =
-// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s (three waves).
+// (system, border column a = blockIdx.y * NR + k), the backward sweep of k_solve_s (four waves).
 template <class T, int WU, int NR, int SPX, int RB>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(SWEEP_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restrict__ S, int B, int n, int r, int US,
            int bufbytes) {
     extern __shared__ __align__(16) char lds[];

@@ -513,7 +513,7 @@ k_pt_zfwd_s(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict
 // ph[3][M]; then per z line the three product pairs (uu, uv), (uw, vv), (vw, ww), one per group,
 // are transformed forward, staged in natural order and split into the two real spectra.
 template <int FA, int FB>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 k_pt_xline_s(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, Dims d) {
     using S = Split<FA, FB>;
     constexpr int M = S::M, XE = S::XS > M ? S::XS : M;
@@ -852,7 +852,7 @@ size_t lds_a(const Dims& d) { return (size_t)2 * 2 * Fold(d.N, d.My).KS * 17 * s
 // Register splits M = FA x FB of the z and x transforms; other 2^a 3^b sizes use the Stockham
 // kernels.
 #define CN_PT_SPLITS(X) X(12, 4, 3) X(18, 6, 3) X(24, 4, 6) X(36, 6, 6) X(48, 8, 6) X(72, 6, 12) \
-    X(96, 8, 12) X(144, 12, 12) X(192, 16, 12)
+    X(96, 6, 16) X(144, 12, 12) X(192, 12, 16)
 
 template <int FA, int FB>
 size_t lds_zs() {

@@ -94,8 +94,8 @@ def test_graph_split_runs_and_poiseuille():
     a.run(20)
     b = ChannelPTSolver(pb, "SBDF3", 1e-3)
     b.set_state(X0)
-    b.run_graph(8)
-    b.run_graph(12)
+    b.run_graph(9)                  # 2 startup steps, 2 ring turns, 1 single phase
+    b.run_graph(11)                 # 3 ring turns, 2 single phases
     c = ChannelPTSolver(pb, "SBDF3", 1e-3)
     c.set_state(X0)
     c.run(7)

@@ -518,14 +518,14 @@ k_pt_xline_s(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restric
     using S = Split<FA, FB>;
     constexpr int M = S::M, XE = S::XS > M ? S::XS : M;
     extern __shared__ __align__(16) char lds[];
-    zc* ex = reinterpret_cast<zc*>(lds);                    // [4][XE] exchange tiles / spectra
-    zc* ph = ex + 4 * XE;                                   // [3][M] physical lines
+    zc* ex = reinterpret_cast<zc*>(lds);                    // [3][XE] exchange tiles / spectra (groups 0..2)
+    zc* ph = ex + 3 * XE;                                   // [3][M] physical lines
     const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
     const int Kx = d.Kx, Mz = d.Mz;
     const int m0 = 2 * blockIdx.x, j = blockIdx.y;
     const int two = (Mz - m0) < 2 ? 1 : 2;
     const zc* in = H2 + ((size_t)j * Mz + m0) * 3 * Kx;     // [line][f][kx]
-    zc* xt = ex + g * XE;
+    zc* xt = ex + (g < 3 ? g : 0) * XE;                     // group 3 idles (no tile)
     zc v[S::NV];
     if (g < 3 && t < FB) {
 #pragma unroll
@@ -862,7 +862,7 @@ size_t lds_zs() {
 template <int FA, int FB>
 size_t lds_xs() {
     using S = Split<FA, FB>;
-    return (size_t)(4 * (S::XS > S::M ? S::XS : S::M) + 3 * S::M) * sizeof(zc);
+    return (size_t)(3 * (S::XS > S::M ? S::XS : S::M) + 3 * S::M) * sizeof(zc);
 }
 
 bool zback_split(const Dims& d, hipStream_t s, const zc* h1, const zc* Wz, zc* h2) {

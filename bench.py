@@ -314,6 +314,7 @@ def channel_pt_leg(steps: int) -> dict:
         raise RuntimeError(f"channel (v-omega) perturbation energy {energy:.3e} out of range")
     nb = channel_pt_step_bytes(pb)
     ach = nb["total"] / (ms * 1e-3) / 1e9
+    survey = 1.70e9            # the configuration's v-omega step model (solve, history, M X, F traffic)
     return {"workload": "channel 3D 128x97x128 velocity-vorticity (poloidal-toroidal) SBDF3, dt = 5e-4: 4096 real "
                         "phi systems (n=101, kl=ku=4, r=4) + 4096 real psi systems (n=99, kl=ku=2, r=2), 4 rhs "
                         "each (8128 complex modes); grid 192x145x192",
@@ -321,7 +322,9 @@ def channel_pt_leg(steps: int) -> dict:
             "mode_solves_per_s": pb.K * 1e3 / ms, "gate_relative_difference": gate,
             "gate_workload": "16x17x12, 20 SBDF3 steps at dt = 2e-3 against the CPU oracle",
             "perturbation_energy_after": energy, "graph": True,
-            "algorithmic_bytes_per_step": nb, "achieved_GBs": ach, "roofline_frac": ach / HBM_PEAK_GBS}
+            "algorithmic_bytes_per_step": nb, "achieved_GBs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
+            "survey_model_bytes_per_step": survey,
+            "survey_model_roofline_frac": survey / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def rb2d_step_bytes(pb) -> float:
@@ -460,6 +463,33 @@ def served_leg(nreq: int, dt: float, channel_share: float = 0.1) -> dict:
     return st
 
 
+def profile_summary(path: str) -> str:
+    """Markdown table of a rocprofv3 kernel trace (``--kernel-trace`` database, ``*_results.db``
+    under ``path``): per kernel name the dispatch count, the average and total duration and the
+    share of the summed kernel time, longest total first."""
+    import glob
+    import sqlite3
+    dbs = sorted(glob.glob(os.path.join(path, "**", "*_results.db"), recursive=True))
+    if not dbs:
+        raise SystemExit(f"no rocprofv3 results database under {path}")
+    rows = []
+    for db in dbs:
+        con = sqlite3.connect(db)
+        rows += con.execute("select name, count(*), sum(end - start) from kernels group by name").fetchall()
+        con.close()
+    agg: dict = {}
+    for name, n, tot in rows:
+        a = agg.setdefault(name, [0, 0])
+        a[0] += n
+        a[1] += tot
+    total = sum(v[1] for v in agg.values()) or 1
+    out = ["| kernel | calls | avg µs | total µs | share |", "|---|---|---|---|---|"]
+    for name, (n, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        short = name if len(name) <= 90 else name[:87] + "..."
+        out.append(f"| `{short}` | {n} | {tot / n / 1e3:.1f} | {tot / 1e3:.1f} | {100 * tot / total:.1f} % |")
+    return "\n".join(out)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,10 +508,19 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--served-requests", type=int, default=120)
+    ap.add_argument("--only-channel-pt", action="store_true",
+                    help="run only the velocity-vorticity channel leg (--channel-steps timed steps) and print its "
+                         "JSON object: the command the channel's rocprofv3 summaries are taken from")
+    ap.add_argument("--profile-summary", metavar="DIR",
+                    help="print the per-kernel table (calls, average and total duration, share) of the rocprofv3 "
+                         "kernel-trace database under DIR and exit (no GPU needed)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain streaming kernel y = 2 x over 2^26 doubles (512 MiB read and 512 MiB "
                          "written per launch, 5 launches): the reference reading for the HBM byte counters")
     args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary))
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the "
@@ -500,6 +539,13 @@ def main() -> int:
             torch.mul(xs, 2.0, out=ys)
         torch.cuda.synchronize()
         log("calibration kernel done")
+        return 0
+    if args.only_channel_pt:
+        log(f"3D channel (velocity-vorticity) IMEX leg ({args.channel_steps} graph-replayed SBDF3 steps)")
+        c = channel_pt_leg(args.channel_steps)
+        log(f"channel (v-omega): {c['ms_per_step']:.4f} ms per SBDF3 step, {c['achieved_GBs']:.0f} GB/s "
+            f"({100 * c['roofline_frac']:.0f} % of HBM)")
+        print(json.dumps(c), flush=True)
         return 0
     from conatus_amd.runtime.banded import BatchHandle
     from tests.banded_cases import microbench_csr

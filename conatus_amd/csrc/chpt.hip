@@ -630,6 +630,8 @@ __device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const dou
     d4 acc[NRT];
 #pragma unroll
     for (int t = 0; t < NRT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    // load only issues the loads (nodes past Mh and dead lanes read node 0, which mma masks), so
+    // that no loaded value is consumed before the next step
     auto load = [&](int j0, double (&P)[2][6], double (&A)[NM][NRT]) {
         const int j = j0 + kq;
         const bool on = live && j < fo.Mh;
@@ -637,14 +639,12 @@ __device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const dou
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const double* pp = H + (size_t)(side ? jb : ja) * 6 * fs;
-            const double v0 = pp[part], v1 = pp[2 * fs + part], v2 = pp[F2 * fs + part], v3 = pp[5 * fs + part];
-            const double v4 = pp[fs + (part ^ 1)], v5 = pp[4 * fs + (part ^ 1)];
-            P[side][0] = on ? v0 : 0.0;        // uu
-            P[side][1] = on ? v1 : 0.0;        // uw
-            P[side][2] = on ? v2 : 0.0;        // vv (phi) / uv (psi)
-            P[side][3] = on ? v3 : 0.0;        // ww
-            P[side][4] = on ? v4 : 0.0;        // uv, partner part
-            P[side][5] = on ? v5 : 0.0;        // vw, partner part
+            P[side][0] = pp[part];                  // uu
+            P[side][1] = pp[2 * fs + part];         // uw
+            P[side][2] = pp[F2 * fs + part];        // vv (phi) / uv (psi)
+            P[side][3] = pp[5 * fs + part];         // ww
+            P[side][4] = pp[fs + (part ^ 1)];       // uv, partner part
+            P[side][5] = pp[4 * fs + (part ^ 1)];   // vw, partner part
         }
         const double* a = ab + (size_t)j0 * RP;
 #pragma unroll
@@ -652,8 +652,14 @@ __device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const dou
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) A[t][rt] = a[t * ts + 16 * rt];
     };
-    auto mma = [&](int j0, const double (&P)[2][6], const double (&A)[NM][NRT]) {
+    auto mma = [&](int j0, const double (&Pr)[2][6], const double (&A)[NM][NRT]) {
         const bool mid = d.My - 1 - (j0 + kq) == j0 + kq;
+        const bool on = live && j0 + kq < fo.Mh;
+        double P[2][6];
+#pragma unroll
+        for (int side = 0; side < 2; ++side)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) P[side][k] = on ? Pr[side][k] : 0.0;
         double c[2][3];
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
@@ -693,12 +699,17 @@ __device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const dou
         }
     };
     double Pa[2][6], Pn[2][6], Aa[NM][NRT], An[NM][NRT];
+    // the scheduling barriers keep each step's loads ahead of the previous step's products
     load(0, Pa, Aa);
     for (int j0 = 0; j0 < fo.Mhs; j0 += 8) {
         load(j0 + 4, Pn, An);
+        __builtin_amdgcn_sched_barrier(0);
         mma(j0, Pa, Aa);
+        __builtin_amdgcn_sched_barrier(0);
         load(j0 + 8, Pa, Aa);
+        __builtin_amdgcn_sched_barrier(0);
         mma(j0 + 4, Pn, An);
+        __builtin_amdgcn_sched_barrier(0);
     }
     const int nrow = psi ? d.nS : d.nP, off = psi ? 2 : 4;
     for (int q = lane; q < 16 * off; q += 64) orow[(size_t)(q / off) * nrow + q % off] = 0.0;

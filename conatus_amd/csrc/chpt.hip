@@ -685,18 +685,22 @@ __device__ __forceinline__ void yfwd_bucket(const zc* __restrict__ H4, const dou
             cp[t] = mid ? c[0][t] : c[0][t] + c[1][t];
             cm[t] = mid ? 0.0 : c[0][t] - c[1][t];
         }
-#pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) {
-            const bool ev = rt < nre;
-            if constexpr (!PSI) {
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[0][rt], ev ? cm[0] : cp[0], acc[rt], 0, 0, 0);
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[1][rt], ev ? cp[1] : cm[1], acc[rt], 0, 0, 0);
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[2][rt], ev ? cp[2] : cm[2], acc[rt], 0, 0, 0);
-            } else {
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[0][rt], ev ? cp[0] : cm[0], acc[rt], 0, 0, 0);
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[1][rt], ev ? cm[1] : cp[1], acc[rt], 0, 0, 0);
-            }
+        // matrix-outer, tile-inner: consecutive products go to different accumulators (each
+        // accumulator still sums its matrices in the same order)
+        double be[NM], bo[NM];                  // B operand of the even / odd row tiles
+        if constexpr (!PSI) {
+            be[0] = cm[0]; bo[0] = cp[0];
+            be[1] = cp[1]; bo[1] = cm[1];
+            be[2] = cp[2]; bo[2] = cm[2];
+        } else {
+            be[0] = cp[0]; bo[0] = cm[0];
+            be[1] = cm[1]; bo[1] = cp[1];
         }
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt)
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[t][rt], rt < nre ? be[t] : bo[t], acc[rt], 0, 0, 0);
     };
     double Pa[2][6], Pn[2][6], Aa[NM][NRT], An[NM][NRT];
     // the scheduling barriers keep each step's loads ahead of the previous step's products

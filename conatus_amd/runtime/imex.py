@@ -529,6 +529,16 @@ class Imex:
         self._levels, self.iteration = saved
         return capture_graph(lambda: [f() for f in fns])
 
+    def startup_step(self, d, t: float) -> None:
+        """One eager SBDF step of the state ``d`` at time ``t``, at the order the history allows,
+        for an owner that keeps the trajectory itself (no continuation check, no host
+        synchronisation; every factorization the step uses must exist already)."""
+        if not self.order:
+            raise OptionError("Imex.startup_step: SBDF schemes only")
+        self._t_step = float(t)
+        self._sbdf_step(d)
+        self.iteration += 1
+
     def replay_step(self, d) -> None:
         """One full-order step of ``d`` from the captured graphs, for an owner that keeps the
         trajectory itself (no continuation check, no host synchronisation): the history must be

@@ -3,7 +3,7 @@ requests.
 
 Requests (a problem, an initial state, a number of steps) arrive at any time.  The server keeps
 every admitted run's state, SBDF history and factored mode systems resident in HBM and advances
-all of them in ticks; one tick is one step of every active run.
+them page by page, each page (a batch of runs of one size class) at its own pace.
 
 * Pages: every size class (grid) holds pages, resident ``Ensemble``s of ``page_size`` slots
   (``channel_page_size`` for channels, whose steps are larger) built once (explicit-term tables,
@@ -24,12 +24,19 @@ all of them in ticks; one tick is one step of every active run.
   ring slots the page's next step reads).  From then on it shares the page's batched step.
 * Retirement: a run that reached its step count leaves with its state copied to the host; its
   slot is free for the next run (a free slot keeps being stepped, its values are never read).
+* Clocks: every page steps on its own stream at its own pace.  The serving loop issues one step
+  of every idle page that holds runs (recording an event behind it), then polls the events: a
+  page whose step has completed counts that step for the runs it held when the step was issued,
+  retires the runs that are done, and is issued again.  Nothing waits for the whole device, so a
+  small run's steps are not held up by a large run's.  A run's step latency is the wall time from
+  the issue of its page's step to the moment the loop sees it complete.  With clock='ticks' the
+  loop instead waits for every issued step before the next round (one round = one tick), which
+  makes admission times reproducible in tests; the arithmetic is the same either way.
 
 Moving a run only copies values: every run's arithmetic stays the one of its single run (tested:
-each result is the same alone and in the mix).  The step latency of a run is the wall time of
-the tick it advanced in (joins, admission, every page's step, until the device finished).  There
-is no counterpart in the reference (it has no server); the per-run step is the reference's
-IVPSolver step (python/conatus/timestep/integrator.py:648-682).
+each result is the same alone and in the mix).  There is no counterpart in the reference (it has
+no server); the per-run step is the reference's IVPSolver step
+(python/conatus/timestep/integrator.py:648-682).
 """
 from __future__ import annotations
 
@@ -101,7 +108,10 @@ class _Page:
         torch = _torch()
         self.P, self.startup, self.dt = P, startup, dt
         self.stream = torch.cuda.Stream()
+        self.event = torch.cuda.Event()
         self.members: dict = {}
+        self.inflight: dict | None = None     # the members of the step in flight (None: idle)
+        self.t_issue = 0.0
         self.ens = Ensemble.from_problems([template] * P, scheme, dt, mode=mode)
         im = self.ens.imex
         with torch.cuda.stream(self.stream):
@@ -125,13 +135,27 @@ class _Page:
                 self.slot_params[q][s] = _params(req.problem)
 
     def step(self) -> None:
+        """Issue one step of every slot on the page's stream (no host synchronisation)."""
         torch = _torch()
         with torch.cuda.stream(self.stream):
             if self.startup:
-                self.ens.run(1)
+                self.ens.imex.startup_step(self.ens._flat(), self.ens.t)
             else:
                 self.ens.imex.replay_step(self.ens._flat())
-                self.ens.t += self.ens.dt
+            self.ens.t += self.ens.dt
+
+    def issue(self) -> None:
+        self.inflight = dict(self.members)
+        self.t_issue = time.perf_counter()
+        self.step()
+        self.event.record(self.stream)
+
+    def slot_state(self, s: int) -> np.ndarray:
+        """Host copy of slot s (waits for the page's stream only)."""
+        torch = _torch()
+        K = self.ens.K
+        with torch.cuda.stream(self.stream):
+            return self.ens._flat()[s * K:(s + 1) * K].cpu().numpy()
 
 
 class Server:
@@ -160,7 +184,7 @@ class Server:
         self.waiting: list[Request] = []
         self.finished: list[Request] = []
         self.resident = 0
-        self.ticks: list[float] = []
+        self.page_steps: list[float] = []   # wall time of every completed page step
         self.joins = 0
         self._next_id = 0
 
@@ -252,12 +276,20 @@ class Server:
             if not pg.members:
                 pg.ens.imex.reset()
                 pg.ens.t = 0.0
-            K = pg.ens.K
-            with torch.cuda.stream(pg.stream):
-                pg.ens._flat()[s * K:(s + 1) * K].copy_(torch.from_numpy(np.ascontiguousarray(q.X0)).to("cuda", non_blocking=False))
-                pg.set_factors(s, q, self._runfac)
+            self._upload(pg, s, q)
             pg.members[s] = q
         self.waiting = rest
+
+    def _upload(self, pg: _Page, s: int, q: Request) -> None:
+        """Initial state and factors of run q into slot s of page pg, on the page's stream: the
+        state travels from a pinned host copy (kept with the request until the run retires), so
+        that the host does not wait for the copy."""
+        torch = _torch()
+        K = pg.ens.K
+        q._pinned = torch.from_numpy(np.ascontiguousarray(q.X0, dtype=np.complex128)).pin_memory()
+        with torch.cuda.stream(pg.stream):
+            pg.ens._flat()[s * K:(s + 1) * K].copy_(q._pinned, non_blocking=True)
+            pg.set_factors(s, q, self._runfac)
 
     def _join(self, q: Request, src, sslot: int) -> None:
         """Move run q (state and history in slot sslot of startup page src, or its initial state
@@ -279,9 +311,9 @@ class Server:
                     for j in range(2):
                         im._ring[im._levels[i]][j].view(im.batch, n)[s * K:(s + 1) * K].copy_(
                             cim._ring[cs][j].view(cim.batch, n)[sslot * K:(sslot + 1) * K])
-            else:
-                pg.ens._flat()[s * K:(s + 1) * K].copy_(torch.from_numpy(np.ascontiguousarray(q.X0)).to("cuda"))
-            pg.set_factors(s, q, self._runfac)
+                pg.set_factors(s, q, self._runfac)
+        if src is None:
+            self._upload(pg, s, q)
         if src is not None:
             # the startup page may be reset and refilled in this same tick (admission writes its
             # slots on its own stream): that work waits until the copies above have read them
@@ -290,54 +322,77 @@ class Server:
         self.joins += 1
 
     def _join_ready(self) -> None:
-        """Startup pages whose history is complete move their unfinished runs into stepping pages."""
+        """Idle startup pages whose history is complete move their runs into stepping pages."""
         for (key, startup), pages in list(self.pages.items()):
             if not startup:
                 continue
             for pg in pages:
                 im = pg.ens.imex
-                if pg.members and im.history_length >= im.order - 1:
+                if pg.members and pg.inflight is None and im.history_length >= im.order - 1:
                     for s, q in sorted(pg.members.items()):
                         self._join(q, pg, s)
                     pg.members = {}
 
+    def _all_pages(self) -> list:
+        return [pg for pages in self.pages.values() for pg in pages]
+
+    def _issue_idle(self) -> int:
+        """Issue one step of every idle page that holds runs; returns how many were issued."""
+        k = 0
+        for pg in self._all_pages():
+            if pg.members and pg.inflight is None:
+                pg.issue()
+                k += 1
+        return k
+
+    def _complete(self, pg: _Page, now: float) -> None:
+        """Count the completed step of page pg for the runs it held at issue; retire finished runs."""
+        lat = time.perf_counter() - pg.t_issue
+        stepped, pg.inflight = pg.inflight, None
+        self.page_steps.append(lat)
+        for s, q in stepped.items():
+            q.step_latencies.append(lat)
+            if len(q.step_latencies) >= q.steps and pg.members.get(s) is q:
+                self._retire(pg.members.pop(s), pg.slot_state(s), now)
+
+    def _poll(self, now: float) -> int:
+        """Complete every page whose step has finished on the device; returns how many did."""
+        k = 0
+        for pg in self._all_pages():
+            if pg.inflight is not None and pg.event.query():
+                self._complete(pg, now)
+                k += 1
+        return k
+
     def tick(self, now: float) -> float:
-        """Admit, join, step every page that holds runs, retire; returns the tick's wall time (s)."""
-        torch = _torch()
+        """One synchronous round: join, admit, step every page that holds runs and wait for all of
+        them, retire; returns the round's wall time (s)."""
         t0 = time.perf_counter()
         self._join_ready()
         self._admit(now)
-        active = [pg for pages in self.pages.values() for pg in pages if pg.members]
-        for pg in active:
-            pg.step()
-        torch.cuda.synchronize()
-        lat = time.perf_counter() - t0
-        for pg in active:
-            done = []
-            for s, q in pg.members.items():
-                q.step_latencies.append(lat)
-                if len(q.step_latencies) >= q.steps:
-                    done.append(s)
-            if done:
-                X = pg.ens.states()
-                for s in done:
-                    self._retire(pg.members.pop(s), X[s], now + lat)
-        self.ticks.append(lat)
-        return lat
+        self._issue_idle()
+        for pg in self._all_pages():
+            if pg.inflight is not None:
+                pg.event.synchronize()
+                self._complete(pg, now)
+        return time.perf_counter() - t0
 
     def _retire(self, q: Request, X: np.ndarray, t: float) -> None:
         q.result = np.array(X)
         q.finished = t
+        q._pinned = None
         self.finished.append(q)
         self.resident -= footprint_bytes(q.problem, self._order)
 
     def _busy(self) -> bool:
-        return any(pg.members for pages in self.pages.values() for pg in pages)
+        return any(pg.members or pg.inflight is not None for pg in self._all_pages())
 
     def serve(self, requests: list[Request], clock: str = "wall") -> dict:
         """Submit ``requests`` (their ``arrival`` times) and serve until all finished.
         clock='wall': a request becomes admissible when the wall time since the start passes its
         arrival in seconds (idle gaps are slept); clock='ticks': arrivals count in ticks."""
+        if clock not in ("wall", "ticks"):
+            raise OptionError(f"Server.serve: clock={clock!r}", fix="clock='wall' or clock='ticks'")
         for q in sorted(requests, key=lambda q: q.arrival):
             self.submit(q)
         start = time.perf_counter()
@@ -351,8 +406,16 @@ class Server:
                 else:
                     ntick = int(np.ceil(nxt))
                 continue
-            self.tick(now)
-            ntick += 1
+            if clock == "ticks":
+                self.tick(now)
+                ntick += 1
+                continue
+            # wall clock: pages step independently; the loop reacts to whichever finishes first
+            self._poll(now)
+            self._join_ready()
+            self._admit(now)
+            if self._issue_idle() == 0 and not self._poll(time.perf_counter() - start):
+                time.sleep(2e-6)
         wall = time.perf_counter() - start
         return self.stats(wall)
 
@@ -363,6 +426,6 @@ class Server:
                 "steps_per_s": steps / wall if wall > 0 else float("nan"),
                 "p50_step_ms": float(np.percentile(lat, 50)) if steps else float("nan"),
                 "p99_step_ms": float(np.percentile(lat, 99)) if steps else float("nan"),
-                "ticks": len(self.ticks), "joins": self.joins, "pages_built": self.pages_built,
+                "page_steps": len(self.page_steps), "joins": self.joins, "pages_built": self.pages_built,
                 "pages": {f"{_grid(k[0])}{' startup' if k[1] else ''}": len(v) for k, v in self.pages.items()},
                 "modes": {_grid(k): v for k, v in self._modes.items()}}

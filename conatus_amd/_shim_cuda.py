@@ -4,6 +4,9 @@ tensors as the device memory.
 
     available() device_name() synchronize() pool_stats() pool_trim() plan_cache_size()
     DeviceArray{D,Z}.from_host(a) / .zeros(rows, cols) / .to_host() / .copy() / .shape / .ptr
+    cheb_forward(grid, N) cheb_backward(coef, M) (DeviceArrayD or Z),
+    realfourier_forward(grid, N) realfourier_backward(coef, M) (DeviceArrayD),
+    complexfourier_forward(grid, N) complexfourier_backward(coef, M) (DeviceArrayZ)
     BandedBatch{D,Z}(mats, border_rows, pivot=True, level_schedule=True, level_merge=True,
                      col_border=False)
         .solve(DeviceArray) .solve_transpose(DeviceArray) .prepare_transpose()
@@ -30,7 +33,9 @@ import scipy.sparse as sp
 from .runtime.banded import BatchHandle
 
 __all__ = ["available", "device_name", "synchronize", "pool_stats", "pool_trim", "plan_cache_size",
-           "DeviceArrayD", "DeviceArrayZ", "BandedBatchD", "BandedBatchZ", "ImexD", "ImexZ"]
+           "DeviceArrayD", "DeviceArrayZ", "BandedBatchD", "BandedBatchZ", "ImexD", "ImexZ",
+           "cheb_forward", "cheb_backward", "realfourier_forward", "realfourier_backward",
+           "complexfourier_forward", "complexfourier_backward"]
 
 
 def _torch():
@@ -114,6 +119,47 @@ class DeviceArrayD(_DeviceArray):
 
 class DeviceArrayZ(_DeviceArray):
     _dtype = np.complex128
+
+
+# ---- transforms (core/bindings/module.cpp:297-311, 476-496): rows of a DeviceArray along the last
+# axis, the grid size taken from the input (forward) or given (backward); a new DeviceArray returned
+def _transform(a, kind: str, direction: str, size: int, kinds: tuple):
+    from . import transforms as TR
+    if not isinstance(a, kinds):
+        raise TypeError(f"{kind} transforms take {' or '.join(k.__name__ for k in kinds)}, got {type(a).__name__}")
+    size = int(size)
+    rows, n = a.shape
+    M, N = (n, size) if direction == "forward" else (size, n)
+    if N < 1 or N > M:
+        raise ValueError(f"{kind} {direction}: {N} coefficients and {M} grid points (need 1 <= N <= M)")
+    if kind in ("rf", "cf") and N % 2:
+        raise ValueError(f"{kind} {direction}: Fourier sizes must be even (Nyquist convention)")
+    Ar, Ai = TR._maps(kind, direction, M, N, a._t.device)
+    return type(a)(TR._apply(a._t, Ar, Ai, size))
+
+
+def cheb_forward(grid, N):
+    return _transform(grid, "cheb", "forward", N, (DeviceArrayD, DeviceArrayZ))
+
+
+def cheb_backward(coef, M):
+    return _transform(coef, "cheb", "backward", M, (DeviceArrayD, DeviceArrayZ))
+
+
+def realfourier_forward(grid, N):
+    return _transform(grid, "rf", "forward", N, (DeviceArrayD,))
+
+
+def realfourier_backward(coef, M):
+    return _transform(coef, "rf", "backward", M, (DeviceArrayD,))
+
+
+def complexfourier_forward(grid, N):
+    return _transform(grid, "cf", "forward", N, (DeviceArrayZ,))
+
+
+def complexfourier_backward(coef, M):
+    return _transform(coef, "cf", "backward", M, (DeviceArrayZ,))
 
 
 def _triples(mats, dtype):

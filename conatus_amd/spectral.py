@@ -104,6 +104,68 @@ def fourier_forward_matrix(K: int, M: int) -> np.ndarray:
     return np.exp(-1j * np.outer(np.arange(K), 2 * np.pi * np.arange(M) / M)) / M
 
 
+def _phase(kk: np.ndarray, M: int) -> np.ndarray:
+    """2 pi (k m mod M) / M for every k of ``kk`` (rows) and grid index m < M (columns): the
+    product is reduced exactly in integers before scaling, so the angles carry no large-argument
+    rounding."""
+    return 2 * np.pi * (np.outer(np.asarray(kk, dtype=np.int64), np.arange(M, dtype=np.int64)) % M) / M
+
+
+def realfourier_forward_matrix(N: int, M: int) -> np.ndarray:
+    """(N, M) real: the RealFourier coefficients [cos_0, msin_0, cos_1, msin_1, ...] of M grid values
+    (reference/transforms.py:124-130: c_0 = F_0, c_2k = 2 Re F_k, c_2k+1 = 2 Im F_k with
+    F = rfft / M; msin_0 = 0, no Nyquist mode)."""
+    K = N // 2
+    A = np.zeros((N, M))
+    A[0] = 1.0 / M
+    if K > 1:
+        th = _phase(np.arange(1, K), M)
+        A[2:N:2] = 2.0 * np.cos(th) / M
+        A[3:N:2] = -2.0 * np.sin(th) / M
+    return A
+
+
+def realfourier_backward_matrix(N: int, M: int) -> np.ndarray:
+    """(M, N) real: f_m = c_0 + sum_{k=1}^{N/2-1} (c_2k cos - c_2k+1 sin)(2 pi k m / M)
+    (reference/transforms.py:170-177; c_1 = msin_0 does not enter)."""
+    K = N // 2
+    A = np.zeros((M, N))
+    A[:, 0] = 1.0
+    if K > 1:
+        th = _phase(np.arange(1, K), M).T
+        A[:, 2:N:2] = np.cos(th)
+        A[:, 3:N:2] = -np.sin(th)
+    return A
+
+
+def complexfourier_freqs(N: int) -> np.ndarray:
+    """Wavenumber of each ComplexFourier coefficient (fftfreq order; the Nyquist slot N/2 gets 0 and
+    carries no mode)."""
+    K = N // 2
+    q = np.arange(N)
+    f = np.where(q < K, q, q - N)
+    f[K] = 0
+    return f
+
+
+def complexfourier_forward_matrix(N: int, M: int) -> np.ndarray:
+    """(N, M) complex: c_q = (1/M) sum_m f_m e^{-2 pi i k_q m / M} for the kept wavenumbers k_q, the
+    Nyquist row zero (reference/transforms.py:140-145)."""
+    th = _phase(complexfourier_freqs(N) % M, M)
+    A = np.exp(-1j * th) / M
+    A[N // 2] = 0.0
+    return A
+
+
+def complexfourier_backward_matrix(N: int, M: int) -> np.ndarray:
+    """(M, N) complex: f_m = sum_q c_q e^{2 pi i k_q m / M}, the Nyquist column zero
+    (reference/transforms.py:181-187)."""
+    th = _phase(complexfourier_freqs(N) % M, M).T
+    A = np.exp(1j * th)
+    A[:, N // 2] = 0.0
+    return A
+
+
 def integration_weights(N: int, length: float) -> np.ndarray:
     """int_a^b T_n(t(z)) dz = (L/2) * 2/(1 - n^2) for even n, 0 for odd n."""
     n = np.arange(N)

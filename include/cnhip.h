@@ -250,6 +250,21 @@ int cn_chpt_sbdf_stage(double* X, const double* const* f_levels, double* mx0, co
                        const double* c, const double* a_over_dt, const double* s04, const double* s2d,
                        const double* s02, int Nx, int Ny, int Nz, double kx0, double kz0, void* stream);
 
+/* ---- Batched grid <-> coefficient transforms along the last axis ------------------------------
+ * out[r][k] = sum_m A[k][m] in[r][m] for every row r < rows of a C-contiguous (rows, nin) array,
+ * out (rows, nout): parts = 1 for real rows, 2 for complex rows ((re, im) interleaved, the same
+ * real map A = Ar applied to both parts), or a complex map A = Ar + i Ai (Ai non-null, parts = 2).
+ * Ar and Ai are (nout, nin) row-major device matrices: the transform of one basis and direction,
+ * built by the caller with the reference's conventions (Gauss-Chebyshev nodes and the (-1)^n
+ * DCT-II / M scaling with c_0 halved; RealFourier [cos_k, msin_k] pairs; ComplexFourier in fftfreq
+ * order with the Nyquist entry zero).  Replaces the device transforms behind
+ * python/conatus/runtime/cuda.py:93-125 (forward, backward) and their bindings cheb_forward,
+ * cheb_backward, realfourier_forward, realfourier_backward, complexfourier_forward,
+ * complexfourier_backward (core/bindings/module.cpp:300-495).  Out of place (in != out); one
+ * launch on `stream`, no allocation, no host synchronisation. */
+int cn_tr_rows(const double* in, double* out, int64_t rows, int nin, int nout, int parts, const double* Ar,
+               const double* Ai, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

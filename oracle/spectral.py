@@ -9,12 +9,14 @@ Restates, in numpy / scipy.fft:
   Chebyshev on ascending Gauss-Chebyshev nodes t_j = -cos(pi (j + 1/2) / M), forward
   c = (-1)^n DCT-II(f)/M with c_0 /= 2 (first N kept), backward by the inverse;
   RealFourier pairs [cos_0, msin_0, cos_1, msin_1, ...] (Nyquist not represented),
-  f = sum c_k cos(k (x-a) 2pi/L) + s_k (-sin(k (x-a) 2pi/L)).
+  f = sum c_k cos(k (x-a) 2pi/L) + s_k (-sin(k (x-a) 2pi/L));
+  ComplexFourier (transforms.py:140-145, 181-187): fft / M in fftfreq order, Nyquist slot zero.
 * the Chebyshev derivative backward recurrence (python/conatus/timestep/spectral.py:53-82).
 
 Pinned by tests/test_oracle_spectral.py with closed forms: T_n(t_j) = cos(n acos t_j), the
 derivative of known polynomials (numpy.polynomial.chebyshev.chebder), C^(lam) conversion checked
-by evaluating both sides with scipy.special.eval_gegenbauer, Fourier modes of cos/sin.
+by evaluating both sides with scipy.special.eval_gegenbauer, Fourier modes of cos/sin and of
+complex exponentials of positive and negative wavenumber.
 """
 from __future__ import annotations
 
@@ -101,6 +103,31 @@ def rf_backward(c: np.ndarray, M: int, axis: int = 0) -> np.ndarray:
     F[1:K] = (c[2:N:2] + 1j * c[3:N:2]) / 2
     F *= M
     return np.moveaxis(np.fft.irfft(F, n=M, axis=0), 0, axis)
+
+
+def cf_forward(f: np.ndarray, N: int, axis: int = 0) -> np.ndarray:
+    """ComplexFourier forward (reference/transforms.py:140-145): fft / M, the first N/2 wavenumbers
+    then the last N/2 - 1 (fftfreq order), the Nyquist slot N/2 zero."""
+    f = np.moveaxis(np.asarray(f), axis, 0)
+    M = f.shape[0]
+    F = np.fft.fft(f, axis=0) / M
+    K = N // 2
+    c = np.zeros((N,) + f.shape[1:], dtype=complex)
+    c[:K] = F[:K]
+    c[K + 1:] = F[M - K + 1:]
+    return np.moveaxis(c, 0, axis)
+
+
+def cf_backward(c: np.ndarray, M: int, axis: int = 0) -> np.ndarray:
+    """ComplexFourier backward (reference/transforms.py:181-187): the spectrum zero padded to M
+    (Nyquist slot dropped), times M, inverse fft."""
+    c = np.moveaxis(np.asarray(c), axis, 0)
+    N = c.shape[0]
+    K = N // 2
+    F = np.zeros((M,) + c.shape[1:], dtype=complex)
+    F[:K] = c[:K]
+    F[M - K + 1:] = c[K + 1:]
+    return np.moveaxis(np.fft.ifft(F * M, axis=0), 0, axis)
 
 
 def cheb_diff(c: np.ndarray, L: float = 2.0, axis: int = 0) -> np.ndarray:

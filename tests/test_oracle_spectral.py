@@ -77,3 +77,43 @@ def test_cheb_diff_recurrence_matches_chebder():
     assert np.max(abs(d - ref)) < 1e-12
     d4 = S.cheb_diff(c, L=4.0)
     assert np.max(abs(d4 - ref / 2)) < 1e-12
+
+
+def test_complexfourier_modes_and_round_trip():
+    """Closed form: f = 2 + (1 - 3i) e^{i 3 k x} + 4i e^{-i 5 k x} has coefficients 2 at wavenumber
+    0, 1 - 3i at +3 and 4i at -5 (fftfreq order: index N - 5), and nothing in the Nyquist slot."""
+    M, N, L = 48, 32, 3.0
+    x = S.fourier_grid(M, -0.7, L)
+    k = 2 * np.pi / L
+    f = 2.0 + (1 - 3j) * np.exp(1j * 3 * k * (x + 0.7)) + 4j * np.exp(-1j * 5 * k * (x + 0.7))
+    c = S.cf_forward(f, N)
+    e = np.zeros(N, complex)
+    e[0], e[3], e[N - 5] = 2.0, 1 - 3j, 4j
+    assert np.max(abs(c - e)) < 1e-13 and c[N // 2] == 0
+    assert np.max(abs(S.cf_backward(c, M) - f)) < 1e-12
+    rng = np.random.default_rng(1)
+    c = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    c[N // 2] = 0
+    assert np.max(abs(S.cf_forward(S.cf_backward(c, M), N) - c)) < 1e-13
+
+
+@pytest.mark.parametrize("M,N", [(12, 8), (48, 32), (145, 97), (192, 128)])
+def test_product_transform_matrices_are_the_oracle_transforms(M, N):
+    """conatus_amd.spectral's dense transform matrices (what the device applies) against the
+    oracle's FFT / DCT transforms, on random data."""
+    from conatus_amd import spectral as P
+    rng = np.random.default_rng(M)
+    f = rng.standard_normal((M, 3))
+    c = rng.standard_normal((N, 3))
+    assert np.max(abs(P.cheb_forward_matrix(N, M) @ f - S.cheb_forward(f, N))) < 1e-13
+    assert np.max(abs(P.cheb_backward_matrix(N, M) @ c - S.cheb_backward(c, M))) < 1e-12
+    if N % 2 == 0:
+        assert np.max(abs(P.realfourier_forward_matrix(N, M) @ f - S.rf_forward(f, N))) < 1e-13
+        cr = c.copy()
+        cr[1] = 0.0
+        assert np.max(abs(P.realfourier_backward_matrix(N, M) @ cr - S.rf_backward(cr, M))) < 1e-12
+        fz = f + 1j * rng.standard_normal((M, 3))
+        cz = c + 1j * rng.standard_normal((N, 3))
+        cz[N // 2] = 0
+        assert np.max(abs(P.complexfourier_forward_matrix(N, M) @ fz - S.cf_forward(fz, N))) < 1e-13
+        assert np.max(abs(P.complexfourier_backward_matrix(N, M) @ cz - S.cf_backward(cz, M))) < 1e-12

@@ -39,6 +39,34 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+#: banded_lane.hip is compiled once per (element type, block length, border rows, kl): one object
+#: each, so that the lane-mode kernels build in parallel
+LANE_TYPES = (("zc", 1), ("d", 0))
+LANE_SPX = (16, 8)
+LANE_RB = (2, 8)
+LANE_KL = (1, 2, 4, 6)
+
+
+def compile_units() -> list[tuple[Path, str, list[str]]]:
+    """(source, object stem, extra flags) of every object of the library."""
+    units = []
+    for src in sources():
+        if src.name != "banded_lane.hip":
+            units.append((src, src.stem, []))
+            continue
+        for tag, zc in LANE_TYPES:
+            for spx in LANE_SPX:
+                for rb in LANE_RB:
+                    for kl in LANE_KL:
+                        t = f"{tag}{spx}_r{rb}"
+                        flags = [f"-DCN_LANE_ZC={zc}", f"-DCN_LANE_SPX={spx}", f"-DCN_LANE_RB={rb}",
+                                 f"-DCN_LANE_KL={kl}", f"-DCN_LANE_SOLVE=lane_solve_{t}_k{kl}"]
+                        if kl == LANE_KL[0]:
+                            flags.append(f"-DCN_LANE_BORDER=lane_border_{t}")
+                        units.append((src, f"banded_lane_{t}_k{kl}", flags))
+    return units
+
+
 def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -55,11 +83,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    srcs = sources()
-    objs = [objdir / (p.stem + ".o") for p in srcs]
+    units = compile_units()
+    objs = [objdir / (stem + ".o") for _, stem, _ in units]
     flags = [f for f in HIPCC_FLAGS if f != "-shared"]
-    cmds = [[hipcc(), *flags, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
-            for src, obj in zip(srcs, objs)]
+    cmds = [[hipcc(), *flags, *extra, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+            for (src, _, extra), obj in zip(units, objs)]
     njobs = jobs or max(1, min(len(cmds), len(os.sched_getaffinity(0))))
     with ThreadPoolExecutor(max_workers=njobs) as ex:
         for f in [ex.submit(_run, c, verbose) for c in cmds]:

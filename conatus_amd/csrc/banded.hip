@@ -206,19 +206,28 @@ BorderFn border_fn(int WU) {
 }
 
 // streamed kernels for r border rows (compiled for rb_for(r) rows) and blocks of spx steps
+#define CN_LANE_BY_KL(tag)                                        \
+    switch (KL) {                                                 \
+        case 1: return lane_solve_##tag##_k1(WU, nr);             \
+        case 2: return lane_solve_##tag##_k2(WU, nr);             \
+        case 4: return lane_solve_##tag##_k4(WU, nr);             \
+        case 6: return lane_solve_##tag##_k6(WU, nr);             \
+    }                                                             \
+    return nullptr;
 template <class T>
 SolveSFn solve_s_fn(int KL, int WU, int nr, int spx, int r) {
     const bool r2 = rb_for(r) == 2;
     if (r > MAXR) return nullptr;
     if constexpr (sizeof(T) == 16) {
-        if (spx == 16) return r2 ? lane_solve_zc16_r2(KL, WU, nr) : lane_solve_zc16_r8(KL, WU, nr);
-        if (spx == 8) return r2 ? lane_solve_zc8_r2(KL, WU, nr) : lane_solve_zc8_r8(KL, WU, nr);
+        if (spx == 16) { if (r2) { CN_LANE_BY_KL(zc16_r2) } else { CN_LANE_BY_KL(zc16_r8) } }
+        if (spx == 8) { if (r2) { CN_LANE_BY_KL(zc8_r2) } else { CN_LANE_BY_KL(zc8_r8) } }
     } else {
-        if (spx == 16) return r2 ? lane_solve_d16_r2(KL, WU, nr) : lane_solve_d16_r8(KL, WU, nr);
-        if (spx == 8) return r2 ? lane_solve_d8_r2(KL, WU, nr) : lane_solve_d8_r8(KL, WU, nr);
+        if (spx == 16) { if (r2) { CN_LANE_BY_KL(d16_r2) } else { CN_LANE_BY_KL(d16_r8) } }
+        if (spx == 8) { if (r2) { CN_LANE_BY_KL(d8_r2) } else { CN_LANE_BY_KL(d8_r8) } }
     }
     return nullptr;
 }
+#undef CN_LANE_BY_KL
 template <class T>
 BorderSFn border_s_fn(int WU, int nr, int spx, int r) {
     const bool r2 = rb_for(r) == 2;

@@ -21,10 +21,13 @@ using SolveSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const vo
 using BorderSFn = void (*)(dim3, dim3, size_t, hipStream_t, const void*, const void*, void*, void*, int, int, int, int,
                            int);
 
-// launch tables of the streamed (lane mode) kernels, one translation unit per (type, block
-// length, border rows compiled for)
+// launch tables of the streamed (lane mode) kernels, one object per (type, block length, border
+// rows compiled for, kl) built from banded_lane.hip; the border tables live in the kl = 1 objects
 #define CN_LANE_TABLES(tag)                                  \
-    SolveSFn lane_solve_##tag(int KL, int WU, int nr);       \
+    SolveSFn lane_solve_##tag##_k1(int WU, int nr);          \
+    SolveSFn lane_solve_##tag##_k2(int WU, int nr);          \
+    SolveSFn lane_solve_##tag##_k4(int WU, int nr);          \
+    SolveSFn lane_solve_##tag##_k6(int WU, int nr);          \
     BorderSFn lane_border_##tag(int WU, int nr);
 CN_LANE_TABLES(zc16_r2)
 CN_LANE_TABLES(zc16_r8)

@@ -1,14 +1,25 @@
 // Launch tables of the lane-mode streamed kernels (k_solve_s, k_border_s) for one element type,
-// one block length and one bound on the border rows; included by the banded_lane_*.hip
-// translation units, which define CN_LANE_T (element type), CN_LANE_SPX (steps per block),
-// CN_LANE_RB (border rows compiled for) and the two table names.
-#pragma once
+// one block length, one bound on the border rows and one band width kl.  The build
+// (conatus_amd/build.py) compiles this file once per combination, defining CN_LANE_ZC (1:
+// complex128, 0: float64), CN_LANE_SPX (steps per block), CN_LANE_RB (border rows compiled for),
+// CN_LANE_KL, the solve table's name CN_LANE_SOLVE and, in the kl = 1 objects, the border
+// completion table's name CN_LANE_BORDER; one small object per combination keeps the build
+// parallel.
 #include "banded_dev.h"
+
+#if !defined(CN_LANE_ZC) || !defined(CN_LANE_SPX) || !defined(CN_LANE_RB) || !defined(CN_LANE_KL) || \
+    !defined(CN_LANE_SOLVE)
+#error "banded_lane.hip is compiled by conatus_amd/build.py with the CN_LANE_* definitions of one variant"
+#endif
 
 namespace cnk {
 namespace {   // internal to each translation unit: the helpers below differ per (type, block length)
 
-using T = CN_LANE_T;
+#if CN_LANE_ZC
+using T = cn::zc;
+#else
+using T = double;
+#endif
 constexpr int SPX = CN_LANE_SPX;
 constexpr int RB = CN_LANE_RB;
 
@@ -40,6 +51,7 @@ SolveSFn solve_wu(int WU, int nr) {
     return nullptr;
 }
 
+#ifdef CN_LANE_BORDER
 template <int WU, int NR>
 void launch_border_s(dim3 g, dim3 b, size_t lds, hipStream_t s, const void* Uf, const void* Vt, void* W, void* S, int B,
                      int n, int r, int US, int bufbytes) {
@@ -55,19 +67,13 @@ BorderSFn border_nr(int nr) {
     }
     return nullptr;
 }
+#endif
 
 }  // namespace
 
-SolveSFn CN_LANE_SOLVE(int KL, int WU, int nr) {
-    switch (KL) {
-        case 1: return solve_wu<1>(WU, nr);
-        case 2: return solve_wu<2>(WU, nr);
-        case 4: return solve_wu<4>(WU, nr);
-        case 6: return solve_wu<6>(WU, nr);
-    }
-    return nullptr;
-}
+SolveSFn CN_LANE_SOLVE(int WU, int nr) { return solve_wu<CN_LANE_KL>(WU, nr); }
 
+#ifdef CN_LANE_BORDER
 BorderSFn CN_LANE_BORDER(int WU, int nr) {
     switch (WU) {
         case 4: return border_nr<4>(nr);
@@ -76,5 +82,6 @@ BorderSFn CN_LANE_BORDER(int WU, int nr) {
     }
     return nullptr;
 }
+#endif
 
 }  // namespace cnk

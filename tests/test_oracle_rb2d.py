@@ -1,5 +1,6 @@
 """Pins of the RB 2D discretization and of the oracle's explicit term (CPU)."""
 import numpy as np
+import pytest
 import numpy.polynomial.chebyshev as C
 import scipy.linalg as la
 import scipy.sparse as sp
@@ -77,3 +78,66 @@ def test_explicit_term_closed_form():
     assert np.max(abs(F[0, rows] - pb.S02 @ e0)) < 1e-14
     assert np.max(abs(F[2, rows] - pb.S02 @ e2)) < 1e-14
     assert np.max(abs(F[1, rows])) < 1e-14 and np.max(abs(F[3, rows])) < 1e-14
+
+
+def _brute_explicit_rb2d(pb, X):
+    """-(u d_x + w d_z) of (u, w, T) by explicit Fourier-Chebyshev convolution: every pair of
+    modes (k1, k2) of both signs multiplied with chebmul, summed into mode k1 + k2, truncated to
+    |k| < K and n < N, converted by S_{0->2} (no grid, no transform, no dealiasing rule)."""
+    K, N = pb.K, pb.N
+    cols = [pb.field_index(f) for f in range(3)]
+    full = []                                     # {k: coefficients} over k = -(K-1) .. K-1
+    for f in range(3):
+        c = X[:, cols[f]]
+        d = {k: c[k].copy() for k in range(K)}
+        d.update({-k: np.conj(c[k]) for k in range(1, K)})
+        full.append(d)
+    dz = 2.0 / pb.Lz
+
+    def ddx(d):
+        return {k: 1j * k * pb.kappa * v for k, v in d.items()}
+
+    def ddz(d):
+        return {k: (C.chebder(v) * dz if len(v) > 1 else 0 * v) for k, v in d.items()}
+
+    F = RB2DOracle(pb).b0.copy()
+    u, w = full[0], full[1]
+    for e, f in enumerate((0, 1, 2)):             # x momentum (u), z momentum (w), temperature (T)
+        acc = {}
+        for vel, der in ((u, ddx(full[f])), (w, ddz(full[f]))):
+            for k1, c1 in vel.items():
+                for k2, c2 in der.items():
+                    k = k1 + k2
+                    if 0 <= k < K:
+                        m = C.chebmul(c1, c2)[:N]
+                        v = np.zeros(N, complex)
+                        v[:m.size] = m
+                        acc[k] = acc.get(k, 0) - v
+        c = np.zeros((K, N), complex)
+        for k, v in acc.items():
+            c[k] = v
+        F[:, pb.equation_rows(e)] = (pb.S02 @ c.T).T
+    return F
+
+
+@pytest.mark.parametrize("shape", [(8, 10), (12, 9)])
+def test_explicit_term_with_vertical_velocity_equals_brute_force_projection(shape):
+    """u, w and T all non-zero and of low degree (every product representable on the 3/2 grid):
+    the oracle's transform-based explicit term equals the brute-force convolution to rounding, so
+    the -w d_z terms are pinned (a swapped u_z / w_z or a wrong d/dz scale fails)."""
+    nx, nz = shape
+    pb = RayleighBenard2D(Nx=nx, Nz=nz, Lx=3.0, Ra=1.0, Pr=1.0)
+    rng = np.random.default_rng(nx * 31 + nz)
+    X = np.zeros((pb.K, pb.n), complex)
+    deg = nz // 2
+    for f in range(3):
+        c = (rng.standard_normal((pb.K, deg)) + 1j * rng.standard_normal((pb.K, deg))) * 0.3
+        c[0] = c[0].real
+        c[pb.K // 2 + 1:] = 0.0                   # modes up to K/2: products stay below K
+        X[:, pb.field_index(f)[:deg]] = c
+    got = RB2DOracle(pb).explicit(X)
+    ref = _brute_explicit_rb2d(pb, X)
+    rows = np.concatenate([pb.equation_rows(e) for e in range(3)])
+    scale = np.abs(ref[:, rows]).max()
+    assert scale > 1e-2
+    assert np.abs(got[:, rows] - ref[:, rows]).max() < 1e-13 * scale

@@ -24,7 +24,8 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode",
            "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit",
            "cn_bb_solve_noalloc", "cn_chpt_workspace_bytes", "cn_chpt_explicit", "cn_chpt_sbdf_stage",
-           "cn_tr_rows")
+           "cn_tr_rows", "cn_imex_create", "cn_imex_set_pointwise", "cn_imex_run", "cn_imex_reset",
+           "cn_imex_info", "cn_imex_destroy")
 
 
 class Props(ctypes.Structure):
@@ -92,6 +93,14 @@ def _declare(lib):
                              + [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_double, _vp]),
         "cn_tr_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
                                       _vp]),
+        "cn_imex_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                           + [ctypes.POINTER(_vp)] * 3 + [_i64p] + [ctypes.POINTER(_vp)] * 3 + [_i64p]
+                           + [_i32p, ctypes.c_int, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]),
+        "cn_imex_set_pointwise": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+        "cn_imex_run": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_int, _dp, _vp]),
+        "cn_imex_reset": (ctypes.c_int, [_vp]),
+        "cn_imex_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i64p]),
+        "cn_imex_destroy": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

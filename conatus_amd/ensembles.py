@@ -239,11 +239,8 @@ def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, 
 
 class EnsembleSet:
     """Runs of uneven sizes advanced together: one Ensemble per discretization (size class), one
-    captured HIP graph per full-order step of the whole set (module doc).  In the graph every
-    class is its own branch (captured on a stream of its own, forked from and joined back into the
-    capturing stream), so that the classes' kernels overlap on the device instead of running one
-    class after another: a small class's launches no longer leave most of the chip idle while the
-    next class waits."""
+    captured HIP graph per full-order step of the whole set (module doc), the classes' launches one
+    after another on the capturing stream."""
 
     def __init__(self, problems: Sequence[RayleighBenard2D], scheme: str, dt: float, *,
                  autotune: bool = True, mode: str | None = None, pivot: bool = True):
@@ -271,7 +268,6 @@ class EnsembleSet:
         self.t = 0.0
         self.iteration = 0
         self._graphs: dict = {}
-        self._streams = [_torch().cuda.Stream() for _ in self.groups]   # one graph branch per class
 
     def set_states(self, states: Sequence) -> None:
         """Initial (K, n) state of every run, in the order of the problems; resets every history."""
@@ -307,7 +303,7 @@ class EnsembleSet:
             graph = self._graphs.get(key)
             if graph is None:
                 fns = [p[1] for p in parts]
-                graph = capture_graph(lambda: self._branches(fns))
+                graph = capture_graph(lambda: [f() for f in fns])
                 self._graphs[key] = graph
             graph.replay()
             for g, p in zip(self.groups, parts):
@@ -319,21 +315,9 @@ class EnsembleSet:
         self.t = self.groups[0].t
         self.iteration += int(steps)
 
-    def _branches(self, fns) -> None:
-        """Enqueue fns[i] (class i's step) on class i's stream, each forked from the current stream
-        and joined back into it (inside a capture: parallel branches of the graph)."""
-        torch = _torch()
-        cur = torch.cuda.current_stream()
-        for st, f in zip(self._streams, fns):
-            st.wait_stream(cur)
-            with torch.cuda.stream(st):
-                f()
-        for st in self._streams:
-            cur.wait_stream(st)
-
     def explain(self) -> str:
         lines = [f"EnsembleSet({self.runs} runs in {len(self.groups)} size classes, {self.scheme}, dt={self.dt:g}): "
-                 "one captured graph per full-order step of all classes, one branch per class"]
+                 "one captured graph per full-order step of all classes"]
         for g, tu in zip(self.groups, self.tuning):
             lines.append(f"  class {tu['class']}: {tu['runs']} runs, mode {tu['mode'] or 'automatic'} "
                          + (f"(ms per solve: {', '.join(f'{k} {v:.3f}' for k, v in tu['ms_per_solve'].items())})"

@@ -250,6 +250,29 @@ int cn_chpt_sbdf_stage(double* X, const double* const* f_levels, double* mx0, co
                        const double* c, const double* a_over_dt, const double* s04, const double* s2d,
                        const double* s02, int Nx, int Ny, int Nz, double kx0, double kz0, void* stream);
 
+/* ---- IMEX integrator handle ---------------------------------------------------------------------
+ * The reference's device integrator cuda.Imex{D,Z} (core/src/cuda/imex.cu:56-244, bindings
+ * core/bindings/module.cpp:411-442; host twin ImexIntegrator, core/include/conatus/imex.hpp:73-219)
+ * for callers that bind the library directly: batch systems M_k dX/dt + L_k X = F of size n (CSR
+ * per system, as cn_bb_create takes them), border rows, scheme "SBDF1".."SBDF4", "RK111", "RK222"
+ * or "RK443", constant dt.  F is zero or pointwise (cn_imex_set_pointwise: F = g .* X + h, device
+ * arrays of batch * n entries, copied).  cn_imex_run advances the (batch, n) device state x in
+ * place by nsteps on `stream` (the factorization of a new implicit operator synchronises once);
+ * *t_out = t0 plus one addition of dt per step.  A run that holds an SBDF history refuses an
+ * (x, t0) other than the one the previous run returned (CN_EINVAL; cn_imex_reset starts a new
+ * trajectory, factorizations kept).  The arithmetic is that of conatus_amd.runtime.imex.Imex,
+ * bit for bit. */
+typedef struct cn_imex* cn_imex_t;
+int cn_imex_create(cn_imex_t* out, int dtype, int batch, int n, const int32_t* const* M_indptr,
+                   const int32_t* const* M_indices, const void* const* M_data, const int64_t* M_nnz,
+                   const int32_t* const* L_indptr, const int32_t* const* L_indices, const void* const* L_data,
+                   const int64_t* L_nnz, const int32_t* border, int r, const char* scheme, double dt, int pivot);
+int cn_imex_set_pointwise(cn_imex_t h, const void* g, const void* hp, void* stream);
+int cn_imex_run(cn_imex_t h, void* x, double t0, int nsteps, double* t_out, void* stream);
+int cn_imex_reset(cn_imex_t h);
+int cn_imex_info(cn_imex_t h, int* history_length, int* n_factorizations, int64_t* iteration);
+void cn_imex_destroy(cn_imex_t h);
+
 /* ---- Batched grid <-> coefficient transforms along the last axis ------------------------------
  * out[r][k] = sum_m A[k][m] in[r][m] for every row r < rows of a C-contiguous (rows, nin) array,
  * out (rows, nout): parts = 1 for real rows, 2 for complex rows ((re, im) interleaved, the same

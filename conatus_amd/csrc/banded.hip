@@ -374,6 +374,9 @@ void run_factor(cn_bb& h, hipStream_t s) {
     const dim3 fb(256), fg((h.Bp + spb - 1) / spb);
     FactorFn f = factor_fn<T>(h.KLP);
     if (!f) fail(CN_ELOGIC, "no factor kernel for kl " + std::to_string(h.KLP));
+    // the factor kernel indexes within an interleave chunk with 32-bit offsets
+    if ((int64_t)h.n * std::max({h.M, h.US, h.r, 1}) * h.il >= ((int64_t)1 << 31))
+        fail(CN_EOVERFLOW, "BandedBatch: n = " + std::to_string(h.n) + " is too large for the factor kernel");
     // band LU, with the forward half L^-1 P_B U of W in the r extra lanes
     f(fg, fb, s, h.Ab.p, h.Uf.p, h.Lf.p, static_cast<int*>(h.piv.p), h.W.p, static_cast<int*>(h.info.p), h.B, h.n,
       h.M, h.r, h.G, h.US, h.pivot, h.il, h.mode);

@@ -68,75 +68,105 @@ k_band_lu(const T* __restrict__ Ab, T* __restrict__ Uf, T* __restrict__ Lf, int*
     const int k = c - M;
     const int gbase = (t & 63) - c;   // wavefront lane of the group's lane 0 (G divides 64)
 
+    // Offsets within a chunk are int (the launch checks n * max(M, US, r) * il < 2^31), and the
+    // step's owner and column offsets advance by one step at a time: no 64-bit multiply and no
+    // division by M in the loop.
     T win[KL + 1];
 #pragma unroll
     for (int i = 0; i <= KL; ++i) {
         T v = cn::zero<T>();
         if (band && i < n) {
             const int d = c - i + KL;
-            if (d >= 0 && d < M) v = A0[((size_t)i * M + d) * il];
+            if (d >= 0 && d < M) v = A0[(i * M + d) * il];
         }
         if (wl) v = (i == k) ? cn::one<T>() : cn::zero<T>();
         win[i] = v;
     }
-    int first_singular = 0;
-    for (int j = 0; j < n; ++j) {
-        // win[i] holds row j+i of this lane's column
-        const int owner = j % M;
-        const int src = gbase + owner;
-        const bool is_owner = (c == owner);
-        // pivot search on the owner's column (first maximum of |re|+|im|, as i?amax)
-        double best = cn::cabs1(win[0]);
-        int ip = 0;
-        if (dopivot) {
-#pragma unroll
-            for (int i = 1; i <= KL; ++i) {
-                const double v = cn::cabs1(win[i]);
-                if (v > best) { best = v; ip = i; }
-            }
-        }
-        ip = __shfl(ip, src, 64);
-        best = __shfl(best, src, 64);
-        if (ip != 0) cn::swap_first<T, KL>(win, ip);   // row interchange in every column of the window and in W
-        const bool singular = !(best != 0.0);
-        T rc = cn::recip(win[0]);
-        if (singular) rc = cn::zero<T>();
-        T l[KL];
-#pragma unroll
-        for (int i = 1; i <= KL; ++i) l[i - 1] = cn::shfl(cn::mul(win[i], rc), src);
-        const int d = (c - owner + M) % M;   // column offset of this lane's column from j
-        if (band) {
-            T ov = win[0];
-            if (is_owner) ov = rc;
-            if (!ucol) U0[((size_t)j * US + d) * il] = ov;          // row j of U: U(j, j+d)
-            else if (j + d < n) U0[((size_t)(j + d) * US + d) * il] = ov;   // column j+d: U(j, j+d)
-        }
-        if (wl) W0[((size_t)k * n + j) * il] = win[0];
-        if (band && is_owner) {
-#pragma unroll
-            for (int i = 0; i < KL; ++i) L0[((size_t)j * KL + i) * il] = l[i];
-            P0[(size_t)j * il] = ip;
-        }
-        if ((band && !is_owner) || wl) {
-#pragma unroll
-            for (int i = 1; i <= KL; ++i) win[i] = cn::fnms(l[i - 1], win[0], win[i]);
-        }
-        // row j leaves the window; row j+KL+1 enters at the bottom
-        const int jn = j + KL + 1;
+    // The band element that enters the window at the end of step s (row s+KL+1 of this lane's
+    // column at step s, or of column s+M for the lane own_s = s mod M that owned column s), read
+    // PF steps before it is needed: a read issued in the step before its use would put one memory
+    // latency on the chain of every step.
+    auto entering = [&](int s, int own_s) -> T {
         T v = cn::zero<T>();
-        if (band) {
-            if (is_owner) {   // the owner takes column j+M: rows j+1..j+KL are zero there
+        const int jn = s + KL + 1;
+        if (band && jn < n) {
+            int dA = c - own_s - 1;   // d - 1 for the other lanes, M - 1 for the owner
+            if (dA < 0) dA += M;
+            v = A0[(jn * M + dA) * il];
+        }
+        return v;
+    };
+    constexpr int PF = 8;
+    T ahead[PF];
+    int own_pf = 0;               // (j + PF) mod M, advanced with j
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        ahead[u] = entering(u, own_pf);
+        own_pf = own_pf + 1 == M ? 0 : own_pf + 1;
+    }
+    int owner = 0;                // j mod M
+    int first_singular = 0;
+    for (int j0 = 0; j0 < n; j0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = j0 + u;
+            if (j >= n) break;
+            // win[i] holds row j+i of this lane's column
+            const int src = gbase + owner;
+            const bool is_owner = (c == owner);
+            // pivot search on the owner's column (first maximum of |re|+|im|, as i?amax)
+            double best = cn::cabs1(win[0]);
+            int ip = 0;
+            if (dopivot) {
+#pragma unroll
+                for (int i = 1; i <= KL; ++i) {
+                    const double v = cn::cabs1(win[i]);
+                    if (v > best) { best = v; ip = i; }
+                }
+            }
+            ip = __shfl(ip, src, 64);
+            best = __shfl(best, src, 64);
+            if (ip != 0) cn::swap_first<T, KL>(win, ip);   // row interchange in every column of the window and in W
+            const bool singular = !(best != 0.0);
+            T rc = cn::recip(win[0]);
+            if (singular) rc = cn::zero<T>();
+            T l[KL];
+#pragma unroll
+            for (int i = 1; i <= KL; ++i) l[i - 1] = cn::shfl(cn::mul(win[i], rc), src);
+            int d = c - owner;   // column offset of this lane's column from j
+            if (d < 0) d += M;
+            if (band) {
+                T ov = win[0];
+                if (is_owner) ov = rc;
+                if (!ucol) U0[(j * US + d) * il] = ov;                   // row j of U: U(j, j+d)
+                else if (j + d < n) U0[((j + d) * US + d) * il] = ov;   // column j+d: U(j, j+d)
+            }
+            if (wl) W0[(k * n + j) * il] = win[0];
+            if (band && is_owner) {
+#pragma unroll
+                for (int i = 0; i < KL; ++i) L0[(j * KL + i) * il] = l[i];
+                P0[j * il] = ip;
+            }
+            if ((band && !is_owner) || wl) {
+#pragma unroll
+                for (int i = 1; i <= KL; ++i) win[i] = cn::fnms(l[i - 1], win[0], win[i]);
+            }
+            // row j leaves the window; row j+KL+1 enters at the bottom
+            const int jn = j + KL + 1;
+            T v = ahead[u];
+            ahead[u] = entering(j + PF, own_pf);
+            own_pf = own_pf + 1 == M ? 0 : own_pf + 1;
+            if (band && is_owner) {   // the owner takes column j+M: rows j+1..j+KL are zero there
 #pragma unroll
                 for (int i = 1; i <= KL; ++i) win[i] = cn::zero<T>();
             }
-            const int dA = is_owner ? (M - 1) : (d - 1);
-            if (jn < n) v = A0[((size_t)jn * M + dA) * il];
-        }
-        if (wl && jn == k) v = cn::one<T>();
+            if (wl && jn == k) v = cn::one<T>();
 #pragma unroll
-        for (int i = 0; i < KL; ++i) win[i] = win[i + 1];
-        win[KL] = v;
-        if (singular && first_singular == 0) first_singular = j + 1;
+            for (int i = 0; i < KL; ++i) win[i] = win[i + 1];
+            win[KL] = v;
+            if (singular && first_singular == 0) first_singular = j + 1;
+            owner = owner + 1 == M ? 0 : owner + 1;
+        }
     }
     if (live && c == 0) info[m] = first_singular;
 }

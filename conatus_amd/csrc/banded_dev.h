@@ -392,6 +392,38 @@ __device__ __forceinline__ void cap_solve(T (&t)[MAXR], const T* __restrict__ Sm
     }
 }
 
+// cap_solve with the factors already in registers (r <= RB): Sr[q][i] = S(i, q), P the pivots
+template <class T, int RB>
+__device__ __forceinline__ void cap_solve_reg(T (&t)[MAXR], const T (&Sr)[RB][RB], const int (&P)[RB], int r) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i >= r) break;
+        const int pi = P[i];
+#pragma unroll
+        for (int q = i + 1; q < RB; ++q) {
+            const bool c = (q == pi);
+            const T ti = t[i], tq = t[q];
+            t[i] = cn::sel(c, tq, ti);
+            t[q] = cn::sel(c, ti, tq);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i >= r) break;
+#pragma unroll
+        for (int q = 0; q < i; ++q) t[i] = cn::fnms(Sr[q][i], t[q], t[i]);
+    }
+#pragma unroll
+    for (int i = RB - 1; i >= 0; --i) {
+        if (i < r) {
+#pragma unroll
+            for (int q = i + 1; q < RB; ++q)
+                if (q < r) t[i] = cn::fnms(Sr[q][i], t[q], t[i]);
+            t[i] = cn::mul(t[i], cn::recip(Sr[i][i]));
+        }
+    }
+}
+
 // Direct solve (any shape and IO mode), the Woodbury order of banded.hpp:312-341:
 //   c = P b;  y = U^-1 L^-1 P_B c (forward sweep, then backward sweep);  t = V^T y accumulated
 //   as the backward sweep produces y;  z = S^-1 t;  x = y - W z (a third pass over x).
@@ -641,16 +673,24 @@ __host__ __device__ constexpr int cor_bytes(int SPX, int rb, int esz) { return r
 // border rows a kernel is compiled for: 2 (r <= 2) or MAXR
 __host__ __device__ constexpr int rb_for(int r) { return r <= 2 ? 2 : MAXR; }
 
-// backward block blk (rows jhi = n-1-blk*SPX down to jlo) into buf:
-// [U: SPX rows x US][V^T: RB x SPX (rows >= r: pads)][in: SPX lane slots, slot s = row jhi - s]
-// Backward block blk, in two parts: the factor rows (V^T rows, then U; returns the number of
-// instructions issued, all whole-wave) and the lane operands (in slots; lane-dependent).
+// Backward block blk covers the rows of forward block nblk-1-blk: rows jhi down to jlo with
+// jlo = (nblk-1-blk) * SPX and jhi = min(n, jlo + SPX) - 1 (the first backward block is the short
+// one when SPX does not divide n), so that a block's lane operands are one forward block's results.
+__device__ __forceinline__ void bwd_rows(int n, int SPX, int blk, int& jlo, int& jhi) {
+    const int nblk = (n + SPX - 1) / SPX;
+    jlo = (nblk - 1 - blk) * SPX;
+    jhi = min(n, jlo + SPX) - 1;
+}
+// Backward block blk into buf:
+// [U: SPX rows x US][V^T: RB x SPX (rows >= r: pads)][in: SPX lane slots, slot s = row jhi - s],
+// in two parts: the factor rows (V^T rows, then U; returns the number of instructions issued, all
+// whole-wave) and the lane operands (in slots; lane-dependent).
 template <class T, int SPX, int RB>
 __device__ __forceinline__ int bwd_dma_f(const T* __restrict__ Uf, const T* __restrict__ Vt, char* buf, size_t ch, int n,
                                          int r, int US, int blk, int lane) {
     constexpr int CE = IL * (int)sizeof(T);
-    const int jhi = n - 1 - blk * SPX;
-    const int jlo = max(0, jhi - SPX + 1);
+    int jlo, jhi;
+    bwd_rows(n, SPX, blk, jlo, jhi);
     const int cnt = jhi - jlo + 1;
     const int bU = SPX * US * CE;
     for (int a = 0; a < r; ++a) dma16(Vt + (ch * r * n + (size_t)a * n + jlo) * IL, buf + bU + a * SPX * CE, cnt * CE, lane);
@@ -661,8 +701,8 @@ __device__ __forceinline__ int bwd_dma_f(const T* __restrict__ Uf, const T* __re
 template <class T, int SPX, int RB>
 __device__ __forceinline__ int bwd_dma_l(char* buf, int n, int US, int blk, const T* in, size_t in_st, bool live) {
     constexpr int CE = IL * (int)sizeof(T);
-    const int jhi = n - 1 - blk * SPX;
-    const int jlo = max(0, jhi - SPX + 1);
+    int jlo, jhi;
+    bwd_rows(n, SPX, blk, jlo, jhi);
     const int bU = SPX * US * CE, bV = RB * SPX * CE;
     if (live) {
 #pragma unroll
@@ -751,31 +791,51 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 // Streamed backward sweep: out_j = (in_j - sum_d U(j, j+d) out_{j+d}) / U(j, j), j = n-1 .. 0, and
 // t[a] += V^T(a, j) out_j (a < RB; the entries a >= r accumulate the LDS pads and are not used).
 // `in` and `out` may be the same array (in place).  Every wave calls it (t is the compute wave's).
-// Full blocks run without per-step tests; a partial last block step by step.
-template <class T, int WU, int SPX, int RB>
+// Full blocks run without per-step tests; a short first block step by step.
+// Hand-over with the sweeps on either side (the solve; the border completion passes 0, false and
+// a no-op): the first `stage_in` blocks (0..2) take their operands from the LDS stages, where the
+// forward sweep left its last two blocks (forward block f in stage f & 1, slot = row - f * SPX),
+// instead of from `in`; with `keep_out` the last two blocks' results stay in their stages (block
+// b in stage b & 1, slot s = row jhi - s) and are not stored; `tail()` is called by wave 1 in the
+// last block, when no buffer but the last block's is in use, to start the next sweep's copies.
+template <class T, int WU, int SPX, int RB, class Tail>
 __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int region,
                                           size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
-                                          size_t out_st, bool live, int wave, int lane, int mlx, T (&t)[RB]) {
+                                          size_t out_st, bool live, int wave, int lane, int mlx, T (&t)[RB],
+                                          int stage_in, bool keep_out, Tail tail) {
     constexpr int GB = LaneSlot<T>::BYTES;
     const int nblk = (n + SPX - 1) / SPX;
     char* stage = lds + region;
     const int SB = stage_bytes(SPX, (int)sizeof(T));
     const int bufbytes = bwd_bytes(SPX, US, RB, (int)sizeof(T));
     const int nb = sweep_nbuf(bufbytes, region), ahead = nb - 1;
+    const int bLS = (SPX * US + RB * SPX) * IL * (int)sizeof(T);   // lane slots within a buffer
+    const int nin = min(stage_in, nblk);
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
     if (wave == 1) {   // factor rows of blocks 0 .. ahead-1 in flight, block 0 complete
         bwd_dma_f<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, lane);
         vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, lane) : 0);
-    } else if (wave == 2) {   // lane operands likewise
-        bwd_dma_l<T, SPX, RB>(lds, n, US, 0, in, in_st, live);
-        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_l<T, SPX, RB>(lds + bufbytes, n, US, 1, in, in_st, live) : 0);
+    } else if (wave == 2) {   // lane operands likewise: blocks 0 .. nin-1 out of the stages, the rest from `in`
+        for (int bk = 0; bk < nin; ++bk) {
+            int jlo, jhi;
+            bwd_rows(n, SPX, bk, jlo, jhi);
+            const int cnt = jhi - jlo + 1;
+            const char* sf = stage + ((jlo / SPX) & 1) * SB;
+            char* dst = lds + (bk % nb) * bufbytes + bLS;
+#pragma unroll
+            for (int s = 0; s < SPX; ++s)
+                if (s < cnt) LaneSlot<T>::write(dst + s * GB, lane, LaneSlot<T>::read(sf + (cnt - 1 - s) * GB, lane));
+        }
+        const int c0 = nin > 0 ? 0 : bwd_dma_l<T, SPX, RB>(lds, n, US, 0, in, in_st, live);
+        (void)c0;
+        vm_wait_le(ahead > 1 && nblk > 1 && nin < 2 ? bwd_dma_l<T, SPX, RB>(lds + bufbytes, n, US, 1, in, in_st, live) : 0);
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
-        const int jhi = n - 1 - b * SPX;
-        const int jlo = max(0, jhi - SPX + 1);
+        int jlo, jhi;
+        bwd_rows(n, SPX, b, jlo, jhi);
         const int cnt = jhi - jlo + 1;
         if (wave == 1) {
             // factor rows of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
@@ -783,17 +843,21 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
                                                                     US, b + ahead, lane)
                                            : 0;
             vm_wait_le(ahead > 1 ? c : 0);
+            if (b == nblk - 1) tail();
         } else if (wave == 2) {
             // lane operands of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
-            const int c = b + ahead < nblk ? bwd_dma_l<T, SPX, RB>(lds + ((b + ahead) % nb) * bufbytes, n, US, b + ahead,
-                                                                    in, in_st, live)
-                                           : 0;
+            const int c = b + ahead < nblk && b + ahead >= nin
+                              ? bwd_dma_l<T, SPX, RB>(lds + ((b + ahead) % nb) * bufbytes, n, US, b + ahead, in, in_st, live)
+                              : 0;
             vm_wait_le(ahead > 1 ? c : 0);
         } else if (wave == 3) {
-            if (b > 0 && live) {   // results of block b-1 (rows jhi+1 .. jhi+SPX)
+            if (b > 0 && live && !(keep_out && b - 1 >= nblk - 2)) {   // results of block b-1
+                int jlo1, jhi1;
+                bwd_rows(n, SPX, b - 1, jlo1, jhi1);
                 const char* sb = stage + ((b - 1) & 1) * SB;
 #pragma unroll
-                for (int s = 0; s < SPX; ++s) out[(size_t)(jhi + SPX - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
+                for (int s = 0; s < SPX; ++s)
+                    if (jhi1 - s >= jlo1) out[(size_t)(jhi1 - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
         } else {
             const char* bb = lds + (b % nb) * bufbytes;
@@ -817,8 +881,9 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         wg_sync();
     }
     if (wave == 3) {
-        if (live) {   // the last block: rows jhi .. 0 with jhi = n-1-(nblk-1)*SPX
-            const int jhi = n - 1 - (nblk - 1) * SPX;
+        if (live && !keep_out) {   // the last block: rows jhi .. 0
+            int jlo, jhi;
+            bwd_rows(n, SPX, nblk - 1, jlo, jhi);
             const char* sb = stage + ((nblk - 1) & 1) * SB;
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
@@ -932,6 +997,21 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     const int seg_live = B - (int)ch * IL - sg / NR;   // instruction i is live if (SPI/NR)*i < seg_live
     const int my_slot = SG::slot(lane);                // where this lane's own segment is read/written
 
+    // the capacitance LU of this lane's system, read at the start (RB <= 2) so that the z solve
+    // between the backward sweep and the correction waits for no memory
+    T Sr[RB <= 2 ? RB : 1][RB <= 2 ? RB : 1];
+    int Pr[RB <= 2 ? RB : 1];
+    if constexpr (RB <= 2) {
+        if (wave == 0 && live) {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                Pr[q] = q < r ? spiv[(size_t)m * r + q] : q;
+#pragma unroll
+                for (int i = 0; i < RB; ++i) Sr[q][i] = (q < r && i < r) ? S[(size_t)m * r * r + q * r + i] : cn::zero<T>();
+            }
+        }
+    }
+
     // ---- forward sweep: y_c = L^-1 P_B P b into the work space
     const int fbuf = fwd_bytes(SPX, KL, (int)sizeof(T));
     const int fnb = sweep_nbuf(fbuf, region), fahead = fnb - 1;
@@ -976,6 +1056,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         return dma16_count(cnt * KL * CE) + dma16_count(cnt * IL * 4);
     };
     auto fwd_store = [&](int bs) {                // forward results of block bs from its stage
+        if (bs >= nblk - 2) return;               // the last two blocks reach the backward sweep in their stages
         const char* sb = stage + (bs & 1) * SB;
         const int j0 = bs * SPX;
 #pragma unroll
@@ -1034,24 +1115,45 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     }
     wg_sync();
 
-    // ---- backward sweep: y = U^-1 y_c in place, t = V^T y
+    // ---- backward sweep: y = U^-1 y_c in place, t = V^T y; its first two blocks come from the
+    // forward sweep's stages, its last two stay in its own for the correction
     T t[RB];
 #pragma unroll
     for (int a = 0; a < RB; ++a) t[a] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, region, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t);
-
-    // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
-    // slots]; x leaves by segments through the stages
     const int cW = RB * SPX * CE;
     const int cbuf = cor_bytes(SPX, RB, (int)sizeof(T));
     const int cnb = sweep_nbuf(cbuf, region), cahead = cnb - 1;
+    auto cor_copy_f = [&](int bn, char* nb) -> int { // wave 1: the W rows of block bn
+        const int j0 = bn * SPX;
+        const int cnt = min(SPX, n - j0);
+        for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
+        return r * dma16_count(cnt * CE);
+    };
+    // The W rows of the correction's first blocks are copied during the backward sweep's last
+    // block when their buffers lie below that block's buffer (the only one still in use).
+    const int bbuf = bwd_bytes(SPX, US, RB, (int)sizeof(T));
+    const int npre = min(cahead, nblk);
+    const bool early_w = npre * cbuf <= ((nblk - 1) % sweep_nbuf(bbuf, region)) * bbuf;
+    int early_c = 0;   // (wave 1) instructions of the second early block
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, region, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t, 2, true,
+                              [&] {
+                                  if (!early_w) return;
+                                  cor_copy_f(0, lds);
+                                  if (npre > 1) early_c = cor_copy_f(1, lds + cbuf);
+                              });
+
+    // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
+    // slots]; x leaves by segments through the stages
     if (wave == 0) {
         T z[MAXR];   // the pad entries t[a >= r] are dropped: W z must not see them
 #pragma unroll
         for (int a = 0; a < MAXR; ++a) z[a] = cn::zero<T>();
 #pragma unroll
         for (int a = 0; a < RB; ++a) z[a] = cn::sel(a < r, t[a], cn::zero<T>());
-        if (live && r > 0) cap_solve<T>(z, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+        if (live && r > 0) {
+            if constexpr (RB <= 2) cap_solve_reg<T, RB>(z, Sr, Pr, r);
+            else cap_solve<T>(z, S + (size_t)m * r * r, spiv + (size_t)m * r, r);
+        }
 #pragma unroll
         for (int a = 0; a < RB; ++a) t[a] = z[a];
         for (int q = r * SPX * IL + lane; q < RB * SPX * IL; q += 64) {   // zero pads of W in every buffer
@@ -1068,12 +1170,6 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
         return __ballot(live) != 0 ? min(SPX, n - j0) * LaneSlot<T>::LOADS : 0;
     };
-    auto cor_copy_f = [&](int bn, char* nb) -> int { // wave 1: the W rows of block bn
-        const int j0 = bn * SPX;
-        const int cnt = min(SPX, n - j0);
-        for (int a = 0; a < r; ++a) dma16(W + (ch * r * n + (size_t)a * n + j0) * IL, nb + a * SPX * CE, cnt * CE, lane);
-        return r * dma16_count(cnt * CE);
-    };
     auto cor_store = [&](int bs) {                // x entries of block bs from its stage, whole segment runs
         const char* sb = stage + (bs & 1) * SB;
         const int j0 = bs * SPX;
@@ -1084,11 +1180,23 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         }
     };
     if (wave == 1) {
-        cor_copy_f(0, lds);
-        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_f(1, lds + cbuf) : 0);
+        if (early_w) {
+            vm_wait_le(early_c);   // block 0 complete
+        } else {
+            cor_copy_f(0, lds);
+            vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_f(1, lds + cbuf) : 0);
+        }
     } else if (wave == 2) {
-        cor_copy_l(0, lds);
-        vm_wait_le(cahead > 1 && nblk > 1 ? cor_copy_l(1, lds + cbuf) : 0);
+        // y of blocks 0 and 1: the backward sweep's last two blocks, from its stages (backward
+        // block nblk-1-f in stage (nblk-1-f) & 1, slot s = row jhi - s)
+        for (int f = 0; f < min(2, nblk); ++f) {
+            const int j0 = f * SPX, cnt = min(SPX, n - j0);
+            const char* sf = stage + ((nblk - 1 - f) & 1) * SB;
+            char* dst = lds + (f % cnb) * cbuf + cW;
+#pragma unroll
+            for (int s = 0; s < SPX; ++s)
+                if (s < cnt) LaneSlot<T>::write(dst + s * GB, lane, LaneSlot<T>::read(sf + (cnt - 1 - s) * GB, lane));
+        }
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -1096,7 +1204,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
             const int c = b + cahead < nblk ? cor_copy_f(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
             vm_wait_le(cahead > 1 ? c : 0);
         } else if (wave == 2) {
-            const int c = b + cahead < nblk ? cor_copy_l(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
+            const int c = b + cahead < nblk && b + cahead >= 2 ? cor_copy_l(b + cahead, lds + ((b + cahead) % cnb) * cbuf) : 0;
             vm_wait_le(cahead > 1 ? c : 0);
         } else if (wave == 3) {
             if (b >= 1) cor_store(b - 1);
@@ -1138,7 +1246,8 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
     T t[RB];
 #pragma unroll
     for (int q = 0; q < RB; ++q) t[q] = cn::zero<T>();
-    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, wave, lane, mlx, t);
+    bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, bufbytes, ch, n, r, US, wa, IL, wa, IL, live, wave, lane, mlx, t, 0, false,
+                              [] {});
     if (wave != 0 || !live) return;
     T* Sm = S + (size_t)m * r * r + (size_t)a * r;
 #pragma unroll

@@ -120,7 +120,9 @@ def cpu_baseline(indptr, indices, datas, b, seconds: float):
 def quiet_heap() -> None:
     """Collect the garbage of the set-up and move the survivors out of the collector's view, so
     that no full collection pauses the host inside a timed region (the timed regions submit device
-    work from a Python loop)."""
+    work from a Python loop).  Called before a leg's warm-up, not between it and the timed region:
+    the collection idles the GPU long enough for its clock to drop (the first ~100 microbench
+    solves after a 0.5 s pause ran up to 10 % slower)."""
     gc.collect()
     gc.freeze()
 
@@ -147,6 +149,7 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     if not gate <= 1e-11:
         raise RuntimeError(f"rb2d accuracy gate failed ({gate:.3e})")
     s = IVPSolver(pb, "SBDF2", 1e-4)
+    quiet_heap()
     # warm-up: replays from the initial state for about half a second (the GPU clock ramps up
     # over the first ~0.1 s of work), both ring phases captured
     tw = time.perf_counter()
@@ -160,7 +163,6 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     s.set_state(X0)
     s.run_graph(4)                       # startup order 1, then full-order graph replays
     torch.cuda.synchronize()
-    quiet_heap()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -180,6 +182,7 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     # the same problem stepped with RK443 (four stages, four solves and three explicit terms per
     # step: the scheme of the reference's published GPU timings at this size)
     rk = IVPSolver(pb, "RK443", 1e-4)
+    quiet_heap()
     tw = time.perf_counter()
     while time.perf_counter() - tw < 0.3:
         rk.set_state(X0)
@@ -189,7 +192,6 @@ def rb2d_leg(steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dic
     rk.set_state(X0)
     rk.run_graph(8)
     torch.cuda.synchronize()
-    quiet_heap()
     e0.record(stream)
     rk.run_graph(steps)
     e1.record(stream)
@@ -251,10 +253,10 @@ def channel_leg(steps: int, gate_steps: int = 10) -> dict:
     X0 = pb.initial_state(seed=0, energy=1e-4)
     s = IVPSolver(pb, "SBDF3", 5e-4)
     log("channel: " + s.explain())
-    s.set_state(X0)
-    s.run_graph(16)
-    torch.cuda.synchronize()
     quiet_heap()
+    s.set_state(X0)
+    s.run_graph(48)                      # warm-up, about 0.1 s
+    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -326,10 +328,10 @@ def channel_pt_leg(steps: int) -> dict:
     X0 = pb.initial_state(seed=0, energy=1e-4)
     s = ChannelPTSolver(pb, "SBDF3", 5e-4)
     log("channel (v-omega): " + s.explain())
-    s.set_state(X0)
-    s.run_graph(24)
-    torch.cuda.synchronize()
     quiet_heap()
+    s.set_state(X0)
+    s.run_graph(192)                     # warm-up, about 0.1 s
+    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -381,10 +383,10 @@ def single_run_ms(pb, dt: float, steps: int = 40) -> float:
     from conatus_amd.timestep import IVPSolver
     s = IVPSolver(pb, "SBDF2", dt)
     X0 = pb.initial_state(seed=0)
+    quiet_heap()
     s.set_state(X0)
     s.run_graph(steps)
     torch.cuda.synchronize()
-    quiet_heap()
     t0 = time.perf_counter()
     s.run_graph(steps)
     torch.cuda.synchronize()
@@ -410,9 +412,10 @@ def ensemble_leg(runs: int, steps: int, dt: float) -> dict:
     es = EnsembleSet(pbs, "SBDF2", dt)
     es.set_states(X0)
     log(f"ensemble set up in {time.perf_counter() - t0:.1f} s: {es.explain()}")
-    es.run_graph(3)                        # startup, capture
-    torch.cuda.synchronize()
     quiet_heap()
+    es.run_graph(3)                        # startup, capture
+    es.run_graph(20)                       # warm-up
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     es.run_graph(steps)
     torch.cuda.synchronize()
@@ -523,6 +526,45 @@ def profile_summary(path: str) -> str:
     return "\n".join(out)
 
 
+def traffic_summary(path: str) -> str:
+    """HBM bytes per launch of the microbench's solve kernel (k_solve_s) from four rocprofv3
+    ``--pmc ... -f csv`` runs under ``path``: ``prof_fetch`` and ``prof_write`` (FETCH_SIZE and
+    WRITE_SIZE of ``bench.py --no-rb2d ...``), ``cal_fetch`` and ``cal_write`` (the same counters of
+    ``bench.py --calibrate``, whose multiply kernel reads and writes 512 MiB per launch).  Each
+    counter is scaled by expected / measured KiB of the calibration kernel.  Prints the JSON record
+    that the bench line's ``roofline.traffic`` reads."""
+    import csv
+    import glob
+
+    def per_dispatch(sub: str, counter: str, want) -> list:
+        vals: dict = {}
+        for f in sorted(glob.glob(os.path.join(path, sub, "**", "*counter_collection.csv"), recursive=True)):
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter or not want(row.get("Kernel_Name", "")):
+                        continue
+                    key = (f, row.get("Dispatch_Id"))
+                    vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+        if not vals:
+            raise SystemExit(f"no {counter} rows for the kernel under {os.path.join(path, sub)}")
+        return list(vals.values())
+
+    expected_kib = (1 << 26) * 8 / 1024
+    is_cal = lambda name: "Mul" in name          # the calibration's y = 2 x kernel (not torch's fill of x)
+    is_solve = lambda name: "k_solve_s" in name
+    fs = expected_kib / float(np.mean(per_dispatch("cal_fetch", "FETCH_SIZE", is_cal)))
+    ws = expected_kib / float(np.mean(per_dispatch("cal_write", "WRITE_SIZE", is_cal)))
+    names = set()
+    for f in glob.glob(os.path.join(path, "prof_fetch", "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            names |= {r.get("Kernel_Name", "") for r in csv.DictReader(fh) if is_solve(r.get("Kernel_Name", ""))}
+    rd = float(np.mean(per_dispatch("prof_fetch", "FETCH_SIZE", is_solve))) * 1024 * fs
+    wr = float(np.mean(per_dispatch("prof_write", "WRITE_SIZE", is_solve))) * 1024 * ws
+    return json.dumps({"kernel": sorted(names)[0] if names else "k_solve_s", "hbm_bytes_per_launch": rd + wr,
+                       "read_bytes": rd, "write_bytes": wr, "fetch_scale": fs, "write_scale": ws,
+                       "source": "profiles/k_solve_microbench.md"}, indent=1)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -547,12 +589,18 @@ def main() -> int:
     ap.add_argument("--profile-summary", metavar="DIR",
                     help="print the per-kernel table (calls, average and total duration, share) of the rocprofv3 "
                          "kernel-trace database under DIR and exit (no GPU needed)")
+    ap.add_argument("--traffic-summary", metavar="DIR",
+                    help="print the HBM traffic record of the solve kernel from the rocprofv3 --pmc runs under DIR "
+                         "(prof_fetch, prof_write, cal_fetch, cal_write) and exit (no GPU needed)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain streaming kernel y = 2 x over 2^26 doubles (512 MiB read and 512 MiB "
                          "written per launch, 5 launches): the reference reading for the HBM byte counters")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary))
+        return 0
+    if args.traffic_summary:
+        print(traffic_summary(args.traffic_summary))
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -605,6 +653,7 @@ def main() -> int:
         return 1
 
     # clock warm-up (about 0.3 s of solves: the GPU clock ramps up over the first ~0.1 s of work)
+    quiet_heap()
     tw = time.perf_counter()
     while time.perf_counter() - tw < 0.3:
         for _ in range(20):
@@ -615,30 +664,36 @@ def main() -> int:
     for _ in range(args.warmup):
         h.factor(sptr, check=False)
     torch.cuda.synchronize()
-    fe = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    for e0, e1 in fe:
-        e0.record(stream)
+    f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    f0.record(stream)
+    for _ in range(args.steps):
         h.factor(sptr, check=False)
-        e1.record(stream)
+    f1.record(stream)
     torch.cuda.synchronize()
-    factor_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in fe]))
+    factor_ms = f0.elapsed_time(f1) / args.steps
     log(f"factor: {factor_ms:.4f} ms per batch")
 
-    # solve timing: W warmup + K timed steps, each one launch of the solve kernel on `stream`
+    # solve timing: W warmup + K timed steps, each one launch of the solve kernel on `stream`,
+    # back to back; HIP events on that stream at both ends of the timed region give the average
+    # launch duration (no commands between the launches)
+    tw = time.perf_counter()               # the clock back up after the factorizations
+    while time.perf_counter() - tw < 0.1:
+        for _ in range(20):
+            h.solve_ptr(d.data_ptr(), NRHS, B * N, False, sptr)
+        torch.cuda.synchronize()
     d.copy_(torch.from_numpy(b))
     for _ in range(args.warmup):
         h.solve_ptr(d.data_ptr(), NRHS, B * N, False, sptr)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    quiet_heap()
     t0 = time.perf_counter()
-    for e0, e1 in ev:
-        e0.record(stream)
+    e0.record(stream)
+    for _ in range(args.steps):
         h.solve_ptr(d.data_ptr(), NRHS, B * N, False, sptr)
-        e1.record(stream)
+    e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    kern_ms = e0.elapsed_time(e1) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     mode_solves = B * NRHS / (ms_per_step * 1e-3)
     log(f"solve: {ms_per_step:.4f} ms per step (events {kern_ms:.4f} ms) -> {mode_solves:.4g} mode-solves/s")

@@ -264,28 +264,17 @@ k_cap_lu(T* __restrict__ S, int* __restrict__ spiv, double* __restrict__ cond, i
 #undef CAP_P
 }
 
-// acc0 - sum_{d=1..WU} u(d) xw[d-1], the row step of the back substitution with U.  The terms
-// d >= 2 use solutions of earlier steps and go into up to four independent partial sums, so that
-// the dependent chain of one step is the d = 1 term alone (one complex fnms instead of WU).  Every
-// lane-mode back substitution sums in this order, so that all of them agree bit for bit.
+// acc0 - sum_{d=1..WU} u(d) xw[d-1], the row step of the back substitution with U, subtracted
+// term by term from d = WU down to d = 1 (the row order of a banded triangular solve).  The terms
+// d >= 2 use solutions of earlier steps, so only the last complex fnms waits for the previous
+// step's result; the others run while it is computed.  Every lane-mode back substitution sums in
+// this order, so that all of them agree bit for bit.
 template <class T, int WU, class F>
 __device__ __forceinline__ T band_dot(T acc0, const T (&xw)[WU], F u) {
-    static_assert(WU >= 2, "band_dot needs at least two terms");
-    constexpr int NP = WU - 1 < 4 ? WU - 1 : 4;
-    T s[NP];
+    T s = acc0;
 #pragma unroll
-    for (int d = WU; d >= 2; --d) {
-        const int q = (WU - d) % NP;
-        if (WU - d < NP)
-            s[q] = cn::mul(u(d), xw[d - 1]);
-        else
-            s[q] = cn::fmac(u(d), xw[d - 1], s[q]);
-    }
-    T tot = s[0];
-    if constexpr (NP == 4) tot = cn::add(cn::add(s[0], s[1]), cn::add(s[2], s[3]));
-    else if constexpr (NP == 3) tot = cn::add(cn::add(s[0], s[1]), s[2]);
-    else if constexpr (NP == 2) tot = cn::add(s[0], s[1]);
-    return cn::fnms(u(1), xw[0], cn::sub(acc0, tot));
+    for (int d = WU; d >= 1; --d) s = cn::fnms(u(d), xw[d - 1], s);
+    return s;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -798,11 +787,13 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 // instead of from `in`; with `keep_out` the last two blocks' results stay in their stages (block
 // b in stage b & 1, slot s = row jhi - s) and are not stored; `tail()` is called by wave 1 in the
 // last block, when no buffer but the last block's is in use, to start the next sweep's copies.
+// Block b uses buffer (b + rot) mod nb; with pre_f the factor rows of block 0 are already in
+// flight (issued by wave 1 before the sweep, into buffer rot).
 template <class T, int WU, int SPX, int RB, class Tail>
 __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __restrict__ Vt, char* lds, int region,
                                           size_t ch, int n, int r, int US, const T* in, size_t in_st, T* out,
                                           size_t out_st, bool live, int wave, int lane, int mlx, T (&t)[RB],
-                                          int stage_in, bool keep_out, Tail tail) {
+                                          int stage_in, bool keep_out, Tail tail, int rot = 0, bool pre_f = false) {
     constexpr int GB = LaneSlot<T>::BYTES;
     const int nblk = (n + SPX - 1) / SPX;
     char* stage = lds + region;
@@ -814,23 +805,24 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
     T xw[WU];
 #pragma unroll
     for (int d = 0; d < WU; ++d) xw[d] = cn::zero<T>();
+    auto buf = [&](int bk) { return lds + ((bk + rot) % nb) * bufbytes; };
     if (wave == 1) {   // factor rows of blocks 0 .. ahead-1 in flight, block 0 complete
-        bwd_dma_f<T, SPX, RB>(Uf, Vt, lds, ch, n, r, US, 0, lane);
-        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + bufbytes, ch, n, r, US, 1, lane) : 0);
+        if (!pre_f) bwd_dma_f<T, SPX, RB>(Uf, Vt, buf(0), ch, n, r, US, 0, lane);
+        vm_wait_le(ahead > 1 && nblk > 1 ? bwd_dma_f<T, SPX, RB>(Uf, Vt, buf(1), ch, n, r, US, 1, lane) : 0);
     } else if (wave == 2) {   // lane operands likewise: blocks 0 .. nin-1 out of the stages, the rest from `in`
         for (int bk = 0; bk < nin; ++bk) {
             int jlo, jhi;
             bwd_rows(n, SPX, bk, jlo, jhi);
             const int cnt = jhi - jlo + 1;
             const char* sf = stage + ((jlo / SPX) & 1) * SB;
-            char* dst = lds + (bk % nb) * bufbytes + bLS;
+            char* dst = buf(bk) + bLS;
 #pragma unroll
             for (int s = 0; s < SPX; ++s)
                 if (s < cnt) LaneSlot<T>::write(dst + s * GB, lane, LaneSlot<T>::read(sf + (cnt - 1 - s) * GB, lane));
         }
-        const int c0 = nin > 0 ? 0 : bwd_dma_l<T, SPX, RB>(lds, n, US, 0, in, in_st, live);
+        const int c0 = nin > 0 ? 0 : bwd_dma_l<T, SPX, RB>(buf(0), n, US, 0, in, in_st, live);
         (void)c0;
-        vm_wait_le(ahead > 1 && nblk > 1 && nin < 2 ? bwd_dma_l<T, SPX, RB>(lds + bufbytes, n, US, 1, in, in_st, live) : 0);
+        vm_wait_le(ahead > 1 && nblk > 1 && nin < 2 ? bwd_dma_l<T, SPX, RB>(buf(1), n, US, 1, in, in_st, live) : 0);
     }
     wg_sync();
     for (int b = 0; b < nblk; ++b) {
@@ -839,15 +831,14 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
         const int cnt = jhi - jlo + 1;
         if (wave == 1) {
             // factor rows of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
-            const int c = b + ahead < nblk ? bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + ((b + ahead) % nb) * bufbytes, ch, n, r,
-                                                                    US, b + ahead, lane)
+            const int c = b + ahead < nblk ? bwd_dma_f<T, SPX, RB>(Uf, Vt, buf(b + ahead), ch, n, r, US, b + ahead, lane)
                                            : 0;
             vm_wait_le(ahead > 1 ? c : 0);
             if (b == nblk - 1) tail();
         } else if (wave == 2) {
             // lane operands of block b+ahead into the buffer block b-1 used; then block b+1 must be complete
             const int c = b + ahead < nblk && b + ahead >= nin
-                              ? bwd_dma_l<T, SPX, RB>(lds + ((b + ahead) % nb) * bufbytes, n, US, b + ahead, in, in_st, live)
+                              ? bwd_dma_l<T, SPX, RB>(buf(b + ahead), n, US, b + ahead, in, in_st, live)
                               : 0;
             vm_wait_le(ahead > 1 ? c : 0);
         } else if (wave == 3) {
@@ -860,7 +851,7 @@ __device__ __forceinline__ void bwd_sweep(const T* __restrict__ Uf, const T* __r
                     if (jhi1 - s >= jlo1) out[(size_t)(jhi1 - s) * out_st] = LaneSlot<T>::read(sb + s * GB, lane);
             }
         } else {
-            const char* bb = lds + (b % nb) * bufbytes;
+            const char* bb = buf(b);
             char* sb = stage + (b & 1) * SB;
             if (cnt == SPX) {   // operands of step s+1 in flight during step s
                 BwdOps<T, WU, RB> op[2];
@@ -1015,6 +1006,16 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     // ---- forward sweep: y_c = L^-1 P_B P b into the work space
     const int fbuf = fwd_bytes(SPX, KL, (int)sizeof(T));
     const int fnb = sweep_nbuf(fbuf, region), fahead = fnb - 1;
+    // The backward sweep's first factor rows are copied during the last forward block, into a
+    // backward buffer (the sweep's buffer rotation brot) that does not overlap that block's buffer.
+    const int bbuf = bwd_bytes(SPX, US, RB, (int)sizeof(T));
+    const int bnb = sweep_nbuf(bbuf, region);
+    const int flo = ((nblk - 1) % fnb) * fbuf, fhi = flo + fbuf;
+    int brot = -1;
+    for (int q = 0; q < bnb && brot < 0; ++q)
+        if (q * bbuf >= fhi || (q + 1) * bbuf <= flo) brot = q;
+    const bool pre_u = brot >= 0;
+    if (brot < 0) brot = 0;
     T y[KL + 1];
     // wave 2: right-hand-side segments of forward block bn into buffer nb.  Entry e of the block's
     // segments is right-hand-side row perm[bn * SPX + KL + 1 + e] (or zero beyond n); the SPX
@@ -1079,6 +1080,7 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
         if (wave == 1) {
             const int c = b + fahead < nblk ? fwd_copy_f(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
             vm_wait_le(fahead > 1 ? c : 0);
+            if (b == nblk - 1 && pre_u) bwd_dma_f<T, SPX, RB>(Uf, Vt, lds + brot * bbuf, ch, n, r, US, 0, lane);
         } else if (wave == 2) {
             const int c = b + fahead < nblk ? fwd_copy_l(b + fahead, lds + ((b + fahead) % fnb) * fbuf) : 0;
             vm_wait_le(fahead > 1 ? c : 0);
@@ -1131,16 +1133,16 @@ k_solve_s(const T* __restrict__ Uf, const T* __restrict__ Lf, const int* __restr
     };
     // The W rows of the correction's first blocks are copied during the backward sweep's last
     // block when their buffers lie below that block's buffer (the only one still in use).
-    const int bbuf = bwd_bytes(SPX, US, RB, (int)sizeof(T));
     const int npre = min(cahead, nblk);
-    const bool early_w = npre * cbuf <= ((nblk - 1) % sweep_nbuf(bbuf, region)) * bbuf;
+    const bool early_w = npre * cbuf <= ((nblk - 1 + brot) % bnb) * bbuf;
     int early_c = 0;   // (wave 1) instructions of the second early block
     bwd_sweep<T, WU, SPX, RB>(Uf, Vt, lds, region, ch, n, r, US, wl, NL, wl, NL, live, wave, lane, mlx, t, 2, true,
                               [&] {
                                   if (!early_w) return;
                                   cor_copy_f(0, lds);
                                   if (npre > 1) early_c = cor_copy_f(1, lds + cbuf);
-                              });
+                              },
+                              brot, pre_u);
 
     // ---- z = S^-1 t, then x = y - W z: block [W: RB x SPX (rows >= r: zero pads)][y: SPX lane
     // slots]; x leaves by segments through the stages

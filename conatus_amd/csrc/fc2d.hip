@@ -234,16 +234,23 @@ k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
         }
         __syncthreads();
     }
-    for (int i = tid; i < Hx; i += blockDim.x) {
+    // x-backward: the mode sum of point i is split over PS consecutive lanes (PS = 1, 2 or 4, the
+    // most that keep PS * Hx within the workgroup), each over a contiguous run of mode pairs
+    // (k, k+1), k odd; the partial sums meet by lane exchanges, in a fixed order.
+    const int PS = 4 * Hx <= (int)blockDim.x ? 4 : (2 * Hx <= (int)blockDim.x ? 2 : 1);
+    const int npair = K / 2;                         // pairs (2q+1, 2q+2), q < npair (the last may lack k+1)
+    for (int pt = tid; pt < PS * Hx; pt += blockDim.x) {
+        const int i = pt / PS, part = pt - i * PS;
+        const int q0 = part * npair / PS, q1 = (part + 1) * npair / PS;
         double ev[8], od[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             ev[q] = 0.0;
             od[q] = 0.0;
         }
-        fc_term(ev, W[0], Zj, K, 0, 0.0);
-        int m = i;                                   // k i mod Mx at k = 1
-        for (int k = 1; k < K; k += 2) {
+        if (part == 0) fc_term(ev, W[0], Zj, K, 0, 0.0);
+        int m = (int)(((long long)(2 * q0 + 1) * i) % Mx);   // k i mod Mx at k = 2 q0 + 1
+        for (int k = 2 * q0 + 1; k < 2 * q1 + 1 && k < K; k += 2) {
             const zc t = W[m];
             fc_term(od, zc{2.0 * t.re, 2.0 * t.im}, Zj, K, k, k * kappa);
             m += i;
@@ -255,6 +262,14 @@ k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
                 if (m >= Mx) m -= Mx;
             }
         }
+        for (int mask = 1; mask < PS; mask <<= 1) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                ev[q] += __shfl_xor(ev[q], mask, 64);
+                od[q] += __shfl_xor(od[q], mask, 64);
+            }
+        }
+        if (part != 0) continue;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             double f[8];

@@ -420,9 +420,36 @@ def ensemble_leg(runs: int, steps: int, dt: float) -> dict:
     es.run_graph(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    fin = all(np.all(np.isfinite(x)) for x in es.states())
-    if not fin:
+    got = es.states()
+    if not all(np.all(np.isfinite(x)) for x in got):
         raise RuntimeError("ensemble state not finite after the timed steps")
+    # parity gate at this workload: the first run of every size class against the same run
+    # stepped alone (bit for bit) and against the CPU oracle (relative difference reported)
+    from conatus_amd.ensembles import discretization_key
+    from conatus_amd.timestep import IVPSolver
+    from oracle.rb2d import RB2DOracle
+    total = 3 + 20 + steps
+    first: dict = {}
+    for i, pb in enumerate(pbs):
+        first.setdefault(discretization_key(pb), i)
+    gate = {}
+    for key, i in sorted(first.items(), key=lambda kv: pbs[kv[1]].Nz):
+        pb = pbs[i]
+        gi, _ = es.where[i]
+        one = IVPSolver(pb, "SBDF2", dt, mode=es.groups[gi].mode)
+        one.set_state(X0[i])
+        one.run_graph(3)
+        one.run_graph(20)
+        one.run_graph(steps)
+        if not np.array_equal(got[i], one.state()):
+            raise RuntimeError(f"ensemble run {i} ({pb.Nx}x{pb.Nz}) differs from the same run stepped alone")
+        ref = RB2DOracle(pb, "SBDF2", dt).run(X0[i], total)
+        r = float(np.max(abs(got[i] - ref)) / np.max(abs(ref)))
+        if not r <= 1e-10:
+            raise RuntimeError(f"ensemble run {i} ({pb.Nx}x{pb.Nz}): oracle difference {r:.2e}")
+        gate[f"{pb.Nx}x{pb.Nz}"] = r
+    log("ensemble gate: one run per class bit for bit as stepped alone; oracle differences after "
+        f"{total} steps " + ", ".join(f"{k} {v:.1e}" for k, v in gate.items()))
     per_class = {}
     for ny in sorted(set(int(v) for v in nys)):
         pb = RayleighBenard2D(Nx=2 * ny, Nz=ny, Lx=4.0, Ra=1e5, Pr=1.0)
@@ -441,6 +468,7 @@ def ensemble_leg(runs: int, steps: int, dt: float) -> dict:
             "sequential_single_run_ms_per_ensemble_step": seq_ms,
             "speedup_vs_sequential_single_runs": seq_ms / ms,
             "single_run_ms_per_step": {str(k): v for k, v in per_class.items()},
+            "gate": {"bitwise_vs_single_runs": True, "steps": total, "oracle_relative_difference": gate},
             "tuning": es.tuning}
 
 

@@ -110,7 +110,8 @@ def _ensemble_problems(runs=256, seed=3):
                          "replay (while the library also held a multi-batch wave solve and the classes ran as "
                          "parallel graph branches; both were then removed); no cause was shown, so the case "
                          "stays off the GPU; EnsembleSet members are checked bit for bit against single runs by "
-                         "tests/test_gpu_ensembles.py at smaller sizes")
+                         "tests/test_gpu_ensembles.py at smaller sizes, and at this workload by the gate of "
+                         "bench.py's ensemble leg (one run per class, bit for bit alone and against the oracle)")
 def test_uneven_256_run_ensemble_members():
     from conatus_amd.ensembles import EnsembleSet, discretization_key
     from conatus_amd.timestep import IVPSolver

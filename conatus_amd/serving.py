@@ -32,6 +32,10 @@ them page by page, each page (a batch of runs of one size class) at its own pace
   the issue of its page's step to the moment the loop sees it complete.  With clock='ticks' the
   loop instead waits for every issued step before the next round (one round = one tick), which
   makes admission times reproducible in tests; the arithmetic is the same either way.
+* Hardware queues: HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+  default); with more stepping pages than queues, a small page's step can wait in a shared queue
+  behind a large page's.  A serving process sets GPU_MAX_HW_QUEUES (16 in the benchmark: per-class
+  p50 0.3-1.4 ms instead of 2-3 ms for the RB classes) before its first HIP call.
 
 Moving a run only copies values: every run's arithmetic stays the one of its single run (tested:
 each result is the same alone and in the mix).  There is no counterpart in the reference (it has
@@ -422,10 +426,18 @@ class Server:
     def stats(self, wall: float) -> dict:
         lat = np.array([x for q in self.finished for x in q.step_latencies]) * 1e3
         steps = int(lat.size)
+        by_class: dict = {}
+        for q in self.finished:
+            by_class.setdefault(_grid(discretization_key(q.problem)), []).extend(q.step_latencies)
+        per_class = {g: {"requests": sum(1 for q in self.finished if _grid(discretization_key(q.problem)) == g),
+                         "steps": len(v), "p50_step_ms": float(np.percentile(v, 50)) * 1e3,
+                         "p99_step_ms": float(np.percentile(v, 99)) * 1e3}
+                     for g, v in sorted(by_class.items()) if v}
         return {"requests": len(self.finished), "steps": steps, "wall_s": wall,
                 "steps_per_s": steps / wall if wall > 0 else float("nan"),
                 "p50_step_ms": float(np.percentile(lat, 50)) if steps else float("nan"),
                 "p99_step_ms": float(np.percentile(lat, 99)) if steps else float("nan"),
+                "per_class": per_class,
                 "page_steps": len(self.page_steps), "joins": self.joins, "pages_built": self.pages_built,
                 "pages": {f"{_grid(k[0])}{' startup' if k[1] else ''}": len(v) for k, v in self.pages.items()},
                 "modes": {_grid(k): v for k, v in self._modes.items()}}

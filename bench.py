@@ -633,8 +633,13 @@ def main() -> int:
     # The served leg steps up to ~30 pages on their own streams; HIP maps streams onto
     # GPU_MAX_HW_QUEUES hardware queues (4 by default), so that a small page's step can wait behind
     # a large one in a shared queue (per-class p50 2-3 ms at 4 queues, 0.3-1.4 ms at 16).  Set
-    # before the first HIP call; the single-stream legs are unaffected.
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+    # before the first HIP call (raising a smaller value from the environment, 4 on the GPU box);
+    # the single-stream legs are unaffected.
+    try:
+        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        hwq = 4
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(hwq, 16), 32))
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the "

@@ -22,7 +22,7 @@ EXPORTS = ("cn_last_error", "cn_device_count", "cn_version", "cn_bb_create", "cn
            "cn_bb_prepare_transpose", "cn_bb_props_get", "cn_bb_cond", "cn_bb_destroy",
            "cn_csr_create", "cn_csr_destroy", "cn_csr_matvec", "cn_sbdf_stage", "cn_rk_rhs",
            "cn_pointwise", "cn_rb2d_explicit", "cn_rb2d_explicit_many", "cn_bb_request_mode",
-           "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit",
+           "cn_bb_copy_systems", "cn_ch3d_workspace_bytes", "cn_ch3d_explicit", "cn_ch3d_explicit_many",
            "cn_bb_solve_noalloc", "cn_chpt_workspace_bytes", "cn_chpt_explicit", "cn_chpt_sbdf_stage",
            "cn_tr_rows", "cn_imex_create", "cn_imex_set_pointwise", "cn_imex_run", "cn_imex_reset",
            "cn_imex_info", "cn_imex_destroy")
@@ -91,6 +91,8 @@ def _declare(lib):
                                + [ctypes.c_double] * 2 + [_vp]),
         "cn_ch3d_explicit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]
                              + [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_double, _vp]),
+        "cn_ch3d_explicit_many": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]
+                                  + [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp]),
         "cn_tr_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
                                       _vp]),
         "cn_imex_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]

@@ -48,8 +48,10 @@ struct ChDims {
 // the values at both nodes of the pair (the middle node, y = 0 for odd My, on its own).
 __global__ void __launch_bounds__(256)
 k_ch_yback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ By,
-           zc* __restrict__ H1, int Kx, int Kz, int N, int My, int n) {
+           zc* __restrict__ H1, int Kx, int Kz, int N, int My, int n, size_t sx, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    X += blockIdx.z * sx;                            // member blockIdx.z: its state and workspace
+    H1 += blockIdx.z * sw;
     zc* c = reinterpret_cast<zc*>(lds);              // [3][N][MA]
     const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x;
     for (int q = tid; q < 3 * N * MA; q += blockDim.x) {
@@ -95,8 +97,10 @@ k_ch_yback(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
 
 __global__ void __launch_bounds__(256)
 k_ch_zback(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, int Kx, int Kz, int Nz,
-           int My, int Mz, double kz0) {
+           int My, int Mz, double kz0, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H1 += blockIdx.z * sw;
+    H2 += blockIdx.z * sw;
     zc* a = reinterpret_cast<zc*>(lds);              // [KB][6][Kz]
     zc* W = a + (size_t)KB * 6 * Kz;                 // [Mz]
     const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
@@ -143,8 +147,10 @@ k_ch_zback(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict_
 
 __global__ void __launch_bounds__(256)
 k_ch_xline(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, int Kx, int Mz, int Mx,
-           double kx0) {
+           double kx0, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H2 += blockIdx.z * sw;
+    H3 += blockIdx.z * sw;
     zc* z = reinterpret_cast<zc*>(lds);              // [9][Kx]: (val, dz, dy) of u, v, w per kx
     zc* W = z + 9 * Kx;                              // [Mx]
     double* G = reinterpret_cast<double*>(W + Mx);   // [3][Mx] products on the line
@@ -235,8 +241,10 @@ __device__ __forceinline__ void xback_term(double (&f)[12], const zc e, const zc
 
 __global__ void __launch_bounds__(256)
 k_ch_zback2(const zc* __restrict__ H1, const zc* __restrict__ Wz, zc* __restrict__ H2, int Kx, int Kz, int Nz,
-            int My, int Mz, double kz0) {
+            int My, int Mz, double kz0, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H1 += blockIdx.z * sw;
+    H2 += blockIdx.z * sw;
     zc* a = reinterpret_cast<zc*>(lds);              // [KB][6][Kz]
     zc* W = a + (size_t)KB * 6 * Kz;                 // [Mz]
     const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
@@ -286,8 +294,10 @@ constexpr int XL = 2;   // grid lines per workgroup of k_ch_xline2
 
 __global__ void __launch_bounds__(192)
 k_ch_xline2(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict__ H3, int Kx, int Mz, int Mx,
-            double kx0) {
+            double kx0, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H2 += blockIdx.z * sw;
+    H3 += blockIdx.z * sw;
     zc* z = reinterpret_cast<zc*>(lds);              // [XL][9][Kx]
     zc* W = z + XL * 9 * Kx;                         // [Mx]
     double* G = reinterpret_cast<double*>(W + Mx);   // [XL][3][Mx]
@@ -362,8 +372,10 @@ k_ch_xline2(const zc* __restrict__ H2, const zc* __restrict__ Wx, zc* __restrict
 
 __global__ void __launch_bounds__(256)
 k_ch_zfwd2(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz,
-           int Mz) {
+           int Mz, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H3 += blockIdx.z * sw;
+    H4 += blockIdx.z * sw;
     zc* g = reinterpret_cast<zc*>(lds);              // [KB][3][Mz]: sums in [0, Hz), differences in [Hz, Mz)
     zc* W = g + (size_t)KB * 3 * Mz;                 // [Mz]
     const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x, Hz = Mz / 2;
@@ -402,8 +414,11 @@ k_ch_zfwd2(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict_
 }
 
 __global__ void __launch_bounds__(256)
-k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz, int Mz) {
+k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__ H4, int Kx, int Kz, int Nz, int Mz,
+          size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H3 += blockIdx.z * sw;
+    H4 += blockIdx.z * sw;
     zc* g = reinterpret_cast<zc*>(lds);              // [KB][3][Mz]
     zc* W = g + (size_t)KB * 3 * Mz;                 // [Mz]
     const int kxb = blockIdx.x * KB, j = blockIdx.y, tid = threadIdx.x;
@@ -438,8 +453,10 @@ k_ch_zfwd(const zc* __restrict__ H3, const zc* __restrict__ Wz, zc* __restrict__
 // plus the middle node for odd My.
 __global__ void __launch_bounds__(256)
 k_ch_yfwd(const zc* __restrict__ H4, const double* __restrict__ Ay, const zc* __restrict__ base, zc* __restrict__ F,
-          int Kx, int Kz, int N, int My, int n, int R) {
+          int Kx, int Kz, int N, int My, int n, int R, size_t sx, size_t sw) {
     extern __shared__ __align__(16) char lds[];
+    H4 += blockIdx.z * sw;
+    F += blockIdx.z * sx;
     zc* g = reinterpret_cast<zc*>(lds);              // [My][3][MA]; after the fold: sums at j < Hy, differences at My-1-j
     const int kx = blockIdx.y, iz0 = blockIdx.x * MA, tid = threadIdx.x, Hy = My / 2;
     for (int q = tid; q < My * 3 * MA; q += blockDim.x) {
@@ -488,21 +505,21 @@ size_t cn_ch3d_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz) {
     return ChDims{Nx / 2, Nz - 1, Ny, Mx, My, Mz}.bytes();
 }
 
-int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
-                     const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
-                     int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream) {
+int cn_ch3d_explicit_many(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
+                          const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
+                          int Mx, int My, int Mz, int n, int R, double kx0, double kz0, int members, void* stream) {
     if (!X || !fidx || !base || !F || !ws || !By || !Wx || !Wz || !Ay) {
         cn::set_error("cn_ch3d_explicit: null pointer");
         return CN_EINVAL;
     }
     if (Nx < 2 || Nx % 2 || Nz < 2 || Nz % 2 || Ny < 2 || Mx < Nx || My < Ny || Mz < Nz || R < 0 ||
-        n < R + 4 * Ny) {
+        n < R + 4 * Ny || members < 1 || members > 65535) {
         cn::set_error("cn_ch3d_explicit: inconsistent sizes");
         return CN_EINVAL;
     }
     const ChDims d{Nx / 2, Nz - 1, Ny, Mx, My, Mz};
-    if (ws_bytes < d.bytes()) {
-        cn::set_error("cn_ch3d_explicit: workspace smaller than cn_ch3d_workspace_bytes");
+    if (ws_bytes / (size_t)members < d.bytes()) {
+        cn::set_error("cn_ch3d_explicit: workspace smaller than members x cn_ch3d_workspace_bytes");
         return CN_EINVAL;
     }
     const size_t lds_a = (size_t)3 * d.N * MA * sizeof(zc);
@@ -517,36 +534,48 @@ int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, 
         cn::set_error("cn_ch3d_explicit: grid too large for the transform kernels (LDS)");
         return CN_EUNSUPPORTED;
     }
+    // member m: state and F at m * sx, workspace (both regions) at m * sw (grid z = member)
+    const size_t sx = (size_t)d.Kx * d.Kz * n, sw = d.region1() + d.region2();
     zc* r1 = static_cast<zc*>(ws);
     zc* r2 = r1 + d.region1();
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 gy((d.Kz + MA - 1) / MA, d.Kx), gz((d.Kx + KB - 1) / KB, My), gx(Mz, My);
+    const unsigned mz = (unsigned)members;
+    const dim3 gy((d.Kz + MA - 1) / MA, d.Kx, mz), gz((d.Kx + KB - 1) / KB, My, mz), gx(Mz, My, mz);
     hipLaunchKernelGGL(k_ch_yback, gy, dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, By, r1, d.Kx, d.Kz,
-                       d.N, My, n);
+                       d.N, My, n, sx, sw);
     if (split_z)
         hipLaunchKernelGGL(k_ch_zback2, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
-                           My, Mz, kz0);
+                           My, Mz, kz0, sw);
     else
         hipLaunchKernelGGL(k_ch_zback, gz, dim3(256), lds_b, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
-                           My, Mz, kz0);
+                           My, Mz, kz0, sw);
     if (split_x)
-        hipLaunchKernelGGL(k_ch_xline2, dim3(Mz / XL, My), dim3(192), lds_c2, s, r2, static_cast<const zc*>(Wx), r1,
-                           d.Kx, Mz, Mx, kx0);
+        hipLaunchKernelGGL(k_ch_xline2, dim3(Mz / XL, My, mz), dim3(192), lds_c2, s, r2, static_cast<const zc*>(Wx),
+                           r1, d.Kx, Mz, Mx, kx0, sw);
     else
-        hipLaunchKernelGGL(k_ch_xline, gx, dim3(256), lds_c, s, r2, static_cast<const zc*>(Wx), r1, d.Kx, Mz, Mx, kx0);
+        hipLaunchKernelGGL(k_ch_xline, gx, dim3(256), lds_c, s, r2, static_cast<const zc*>(Wx), r1, d.Kx, Mz, Mx, kx0,
+                           sw);
     if (split_z)
         hipLaunchKernelGGL(k_ch_zfwd2, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz,
-                           Mz);
+                           Mz, sw);
     else
-        hipLaunchKernelGGL(k_ch_zfwd, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, Mz);
+        hipLaunchKernelGGL(k_ch_zfwd, gz, dim3(256), lds_d, s, r1, static_cast<const zc*>(Wz), r2, d.Kx, d.Kz, Nz, Mz,
+                           sw);
     hipLaunchKernelGGL(k_ch_yfwd, gy, dim3(256), lds_e, s, r2, Ay, static_cast<const zc*>(base), static_cast<zc*>(F),
-                       d.Kx, d.Kz, d.N, My, n, R);
+                       d.Kx, d.Kz, d.N, My, n, R, sx, sw);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         cn::set_error(std::string("cn_ch3d_explicit: ") + hipGetErrorString(e));
         return CN_EHIP;
     }
     return CN_OK;
+}
+
+int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
+                     const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
+                     int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream) {
+    return cn_ch3d_explicit_many(X, fidx, base, F, ws, ws_bytes, By, Wx, Wz, Ay, Nx, Ny, Nz, Mx, My, Mz, n, R, kx0,
+                                 kz0, 1, stream);
 }
 
 }  // extern "C"

@@ -26,7 +26,7 @@ from .errors import OptionError
 from .problems import RayleighBenard2D
 from .runtime.banded import BatchHandle, csr_args
 from .runtime.imex import Imex, capture_graph, sbdf_coefficients
-from .timestep import IVPSolver, RB2DExplicit, explicit_term
+from .timestep import Channel3DExplicit, IVPSolver, RB2DExplicit, explicit_term
 
 __all__ = ["Ensemble", "EnsembleSet", "autotune_mode", "discretization_key"]
 
@@ -132,10 +132,13 @@ class Ensemble:
         self.problems = problems
         self.B, self.K, self.n = len(problems), p0.K, p0.n
         self.scheme, self.dt, self.pivot, self.mode = scheme, float(dt), pivot, mode
-        if not isinstance(p0, RayleighBenard2D):
-            rhs = "loop"                  # the channel's explicit term has no member dimension
         self.rhs_mode = rhs
-        self.explicit = RB2DExplicit(p0, self.B) if rhs == "batched" else _LoopExplicit(p0, self.B)
+        if rhs == "loop":
+            self.explicit = _LoopExplicit(p0, self.B)
+        elif isinstance(p0, RayleighBenard2D):
+            self.explicit = RB2DExplicit(p0, self.B)
+        else:
+            self.explicit = Channel3DExplicit(p0, self.B)
         Ms = [M for pb in problems for (M, L) in pb.mats]
         Ls = [L for pb in problems for (M, L) in pb.mats]
         self.imex = Imex(Ms, Ls, p0.border, scheme, dt, pivot=pivot, dtype=np.complex128,
@@ -196,7 +199,8 @@ class Ensemble:
         return (f"Ensemble({self.B} member(s) of {pb.describe()}, parameters {ps[0]}..{ps[-1]}, "
                 f"{self.scheme}, dt={self.dt:g}, rhs={self.rhs_mode}): {self.B * self.K} mode systems of "
                 f"n={self.n} in one batched solve per stage; F in "
-                + ("one launch pair for all members" if self.rhs_mode == "batched" else "one launch pair per member")
+                + ("one launch sequence for all members" if self.rhs_mode == "batched"
+                   else "one launch sequence per member")
                 + "; " + self.imex.explain())
 
 

@@ -63,13 +63,15 @@ class RB2DExplicit:
 
 
 class Channel3DExplicit:
-    """F(X) of the 3D plane channel on the device (cn_ch3d_explicit: five transform launches,
-    one workspace).  The constant part (the driving pressure gradient) is added by the last
-    launch; the other rows of F keep base()."""
+    """F(X) of the 3D plane channel on the device (cn_ch3d_explicit_many: five transform launches
+    for all `members` states of one grid stored one after another, a workspace region per member).
+    The constant part (the driving pressure gradient) is added by the last launch; the other rows
+    of F keep base()."""
 
-    def __init__(self, pb: Channel3D):
+    def __init__(self, pb: Channel3D, members: int = 1):
         torch = _torch()
         self.pb = pb
+        self.members = int(members)
         N, Mx, My, Mz = pb.N, pb.Mx, pb.My, pb.Mz
         dev = "cuda"
         By = np.stack([S.cheb_backward_matrix(N, My, 0), S.cheb_backward_matrix(N, My, 1, 2.0)], axis=-1)
@@ -78,11 +80,12 @@ class Channel3DExplicit:
         self.Wz = torch.from_numpy(np.exp(2j * np.pi * np.arange(Mz) / Mz)).to(dev)
         Ay = (pb.S02 @ S.cheb_forward_matrix(N, My)).T
         self.Ay = torch.from_numpy(np.ascontiguousarray(Ay)).to(dev)                       # (My, N)
-        self._base = torch.from_numpy(pb.b0().ravel()).to(dev)
+        self._one_base = torch.from_numpy(pb.b0().ravel()).to(dev)
+        self._base = self._one_base if self.members == 1 else torch.cat([self._one_base] * self.members)
         fidx = np.stack([pb.field_index(f) for f in (0, 1, 2)]).astype(np.int32)
         self.fidx = torch.from_numpy(np.ascontiguousarray(fidx)).to(dev)
         nbytes = _lib.lib().cn_ch3d_workspace_bytes(pb.Nx, pb.Ny, pb.Nz, Mx, My, Mz)
-        self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+        self.ws = torch.empty(max(int(nbytes) * self.members, 1), dtype=torch.uint8, device=dev)
 
     def base(self):
         return self._base
@@ -91,10 +94,11 @@ class Channel3DExplicit:
         pb = self.pb
         v = ctypes.c_void_p
         p = lambda t: v(t.data_ptr())   # noqa: E731
-        st = _lib.lib().cn_ch3d_explicit(p(x), p(self.fidx), p(self._base), p(out), p(self.ws), self.ws.numel(),
-                                         p(self.By), p(self.Wx), p(self.Wz), p(self.Ay), pb.Nx, pb.Ny, pb.Nz,
-                                         pb.Mx, pb.My, pb.Mz, pb.n, NBORDER, float(pb.kx0), float(pb.kz0),
-                                         v(_torch().cuda.current_stream().cuda_stream))
+        st = _lib.lib().cn_ch3d_explicit_many(p(x), p(self.fidx), p(self._one_base), p(out), p(self.ws),
+                                              self.ws.numel(), p(self.By), p(self.Wx), p(self.Wz), p(self.Ay), pb.Nx,
+                                              pb.Ny, pb.Nz, pb.Mx, pb.My, pb.Mz, pb.n, NBORDER, float(pb.kx0),
+                                              float(pb.kz0), self.members,
+                                              v(_torch().cuda.current_stream().cuda_stream))
         _lib.check(st, "Channel3DExplicit")
 
 

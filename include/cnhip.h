@@ -213,6 +213,15 @@ size_t cn_ch3d_workspace_bytes(int Nx, int Ny, int Nz, int Mx, int My, int Mz);
 int cn_ch3d_explicit(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
                      const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
                      int Mx, int My, int Mz, int n, int R, double kx0, double kz0, void* stream);
+/* The same for `members` (1 .. 65535) states of one grid stored one after another (X and F:
+ * members x Kx*Kz x n; base: one member's, shared), each with its own workspace region of
+ * cn_ch3d_workspace_bytes(...) (ws_bytes >= members times that): one launch sequence for all
+ * members, member m's arithmetic that of its own cn_ch3d_explicit call.  No counterpart in the
+ * reference, whose ensembles batch equal right-hand sides only (ensembles/batched.py:55-613);
+ * for ensembles and server pages of channel runs. */
+int cn_ch3d_explicit_many(const void* X, const int* fidx, const void* base, void* F, void* ws, size_t ws_bytes,
+                          const double* By, const void* Wx, const void* Wz, const double* Ay, int Nx, int Ny, int Nz,
+                          int Mx, int My, int Mz, int n, int R, double kx0, double kz0, int members, void* stream);
 
 /* ---- 3D plane channel, poloidal-toroidal (velocity-vorticity) form ---------------------------
  * The reference's channel_3d_pt (tests/channel3d/ch3d_cases.py:90-217).  State X (float64): the

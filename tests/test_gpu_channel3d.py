@@ -53,6 +53,29 @@ def test_explicit_term_matches_oracle(shape):
     assert np.array_equal(got[:, other], pb.b0()[:, other])
 
 
+@pytest.mark.parametrize("shape,members", [((8, 9, 6), 3), ((16, 17, 12), 2)])
+def test_explicit_term_of_many_members_equals_single_members(shape, members):
+    """cn_ch3d_explicit_many: each member's F bit for bit as its own single-member call, and
+    (through the single call) within 1e-13 of the oracle."""
+    import torch
+    from conatus_amd.timestep import Channel3DExplicit
+    nx, ny, nz = shape
+    print(f"[ch3d explicit many] {shape} x {members}", flush=True)
+    pb = Channel3D(Nx=nx, Ny=ny, Nz=nz, Re=400.0, Lx=2 * np.pi, Lz=np.pi)
+    Xs = [_rough_state(pb, seed=10 + b) for b in range(members)]
+    many = Channel3DExplicit(pb, members)
+    out = many.base().clone()
+    many.write(torch.from_numpy(np.concatenate([X.ravel() for X in Xs])).cuda(), out)
+    got = out.view(members, pb.K, pb.n).cpu().numpy()
+    one = Channel3DExplicit(pb)
+    rows = np.concatenate([pb.equation_rows(e) for e in range(3)])
+    for b, X in enumerate(Xs):
+        o = one.base().clone()
+        one.write(torch.from_numpy(X.ravel()).cuda(), o)
+        assert np.array_equal(got[b], o.view(pb.K, pb.n).cpu().numpy()), b
+        assert rel(got[b][:, rows], Channel3DOracle(pb).explicit(X)[:, rows]) < 1e-13
+
+
 @pytest.mark.parametrize("scheme,steps,dt", [("SBDF3", 12, 2e-3), ("SBDF2", 8, 2e-3), ("RK443", 4, 2e-3)])
 def test_trajectory_matches_oracle(scheme, steps, dt):
     from conatus_amd.timestep import IVPSolver

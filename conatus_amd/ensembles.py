@@ -10,8 +10,8 @@ Members never interact: each member's arithmetic is the one of its single run, s
 bit-identical to the same run stepped alone by ``IVPSolver`` with the same kernel mode (tested).
 
 ``EnsembleSet`` advances runs of uneven sizes (several discretizations): one Ensemble per size
-class, and every full-order step of all classes is one captured HIP graph, so that a step of the
-whole set is one graph launch.  The kernel mode of each class (lane mapping: one lane per system
+class, each stepped from its own captured HIP graphs on its own stream, so that the classes'
+kernels overlap on the device.  The kernel mode of each class (lane mapping: one lane per system
 and right-hand side with streamed sweeps; wave mapping: one wavefront per system and right-hand
 side) is picked by timing both on the class's own factored systems (``autotune_mode``).
 """
@@ -25,7 +25,7 @@ import numpy as np
 from .errors import OptionError
 from .problems import RayleighBenard2D
 from .runtime.banded import BatchHandle, csr_args
-from .runtime.imex import Imex, capture_graph, sbdf_coefficients
+from .runtime.imex import Imex, sbdf_coefficients
 from .timestep import Channel3DExplicit, IVPSolver, RB2DExplicit, explicit_term
 
 __all__ = ["Ensemble", "EnsembleSet", "autotune_mode", "discretization_key"]
@@ -242,9 +242,10 @@ def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, 
 
 
 class EnsembleSet:
-    """Runs of uneven sizes advanced together: one Ensemble per discretization (size class), one
-    captured HIP graph per full-order step of the whole set (module doc), the classes' launches one
-    after another on the capturing stream."""
+    """Runs of uneven sizes advanced together: one Ensemble per discretization (size class), each
+    class stepped from its own captured HIP graphs on its own stream (module doc), so that the
+    classes' kernels overlap on the device: a small class's launches fill the gaps the large
+    classes leave instead of waiting for them."""
 
     def __init__(self, problems: Sequence[RayleighBenard2D], scheme: str, dt: float, *,
                  autotune: bool = True, mode: str | None = None, pivot: bool = True):
@@ -271,7 +272,7 @@ class EnsembleSet:
         self.runs = len(problems)
         self.t = 0.0
         self.iteration = 0
-        self._graphs: dict = {}
+        self._streams: list | None = None
 
     def set_states(self, states: Sequence) -> None:
         """Initial (K, n) state of every run, in the order of the problems; resets every history."""
@@ -294,34 +295,26 @@ class EnsembleSet:
         self.iteration += int(steps)
 
     def run_graph(self, steps: int) -> None:
-        """Full-order steps of all classes replayed from one captured graph per ring phase; the
-        lower-order startup steps run eagerly, class after class."""
-        done = steps
-        for g in self.groups:
-            g.imex._check_state(g._flat())
-            g.imex._continuation(g._flat(), g.t)
-            g.t, done = g.imex._graph_startup(g._flat(), g.t, steps)
-        while done < steps:
-            parts = [g.imex._graph_step(g._flat()) for g in self.groups]
-            key = tuple(p[0] for p in parts)
-            graph = self._graphs.get(key)
-            if graph is None:
-                fns = [p[1] for p in parts]
-                graph = capture_graph(lambda: [f() for f in fns])
-                self._graphs[key] = graph
-            graph.replay()
-            for g, p in zip(self.groups, parts):
-                p[2]()
-                g.t += g.dt
-            done += 1
-        for g in self.groups:
-            g.imex._remember(g._flat(), g.t)
+        """Every class runs ``steps`` steps (lower-order startup eagerly, then full-order steps
+        replayed from its captured graphs) on its own stream, ordered after the work already on
+        the current stream; the current stream then waits for all of them.  The classes share
+        nothing, so each one's arithmetic is that of its own Ensemble.run_graph."""
+        torch = _torch()
+        cur = torch.cuda.current_stream()
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream() for _ in self.groups]
+        for g, sc in zip(self.groups, self._streams):
+            sc.wait_stream(cur)
+            with torch.cuda.stream(sc):
+                g.run_graph(steps)
+        for sc in self._streams:
+            cur.wait_stream(sc)
         self.t = self.groups[0].t
         self.iteration += int(steps)
 
     def explain(self) -> str:
         lines = [f"EnsembleSet({self.runs} runs in {len(self.groups)} size classes, {self.scheme}, dt={self.dt:g}): "
-                 "one captured graph per full-order step of all classes"]
+                 "each class from its own captured graphs on its own stream"]
         for g, tu in zip(self.groups, self.tuning):
             lines.append(f"  class {tu['class']}: {tu['runs']} runs, mode {tu['mode'] or 'automatic'} "
                          + (f"(ms per solve: {', '.join(f'{k} {v:.3f}' for k, v in tu['ms_per_solve'].items())})"

@@ -333,7 +333,7 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
 int wave_steps(size_t waves) { return waves > 512 ? SPW_MANY : SPW_FEW; }
 int wave_buf(const cn_bb& h, int sp) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz, sp); }
 size_t wave_lds_bytes(const cn_bb& h, int sp) {
-    return 2 * (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h, sp);
+    return (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h, sp);
 }
 
 // Block length of the streamed sweeps: 16 steps when two buffers of buf16 bytes (the largest sweep
@@ -574,7 +574,7 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
         const int lane_buf = std::max({fwd_bytes(8, h.KLP, e), bwd_bytes(8, h.US, rb, e), cor_bytes(8, rb, e)});
         const bool lane_fast = h.KLP + 1 <= 8 && h.WU <= 16 && 2 * lane_buf + 2 * stage_bytes(8, e) <= LDS_CAP;
         const bool wave_ok =
-            2 * (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e, SPW_FEW) <=
+            (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e, SPW_FEW) <=
                 (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
         const std::string want = g_mode_request == 1 ? "lane" : g_mode_request == 2 ? "wave" : em ? em : "";

@@ -1453,7 +1453,8 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
 // finished row value y_j (lane 0), every window lane subtracts Uc[j][d] y_j, one wave rotation
 // (y_j moves to lane 63).  Groups, flushes and the partial last block as in the forward sweep;
 // the division x_j = y_j / U(j, j) is applied to the finished rows when they are stored, from the
-// diagonal dv (n entries in LDS).
+// column's entry 0 (1 / U(j, j)), read from the staged block when the row is in it (always for the
+// groups of full blocks) and from the factors otherwise.
 template <class T>
 __device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
     const T yj = bcast(y, 0);
@@ -1462,14 +1463,17 @@ __device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
 }
 
 template <class T, int SB, int SP>
-__device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot, char* bufs,
+__device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, const char* zslot, char* bufs,
                                               int bufbytes, int n, int w, int US, int lane) {
     constexpr int ES = (int)sizeof(T);
     const int R = 64 - w;
     const int nblk = (n + SP - 1) / SP;
     const bool upd = lane >= 1 && lane < w;
     const bool above = lane >= w;
-    wave_load<T>(dv, Uc0, US, n, lane);
+    // 1 / U(row, row): entry 0 of column `row`, staged at (row - jlo) * US when jlo <= row <= jhi
+    auto diag = [&](const char* cur, int jlo, int jhi, int row) -> T {
+        return (row >= jlo && row <= jhi) ? lds_at<T>(cur + (row - jlo) * US * ES) : Uc0[(size_t)row * US];
+    };
     T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
     int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
     wave_stage_bwd<T, SP>(Uc0, n, US, 0, bufs, lane);
@@ -1492,9 +1496,9 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
 #pragma unroll
                 for (int u = 0; u < SB; ++u) bwd_step<T>(y, uv[q * SB + u], lane);
                 const int jn = jhi - (q + 1) * SB;    // row of lane 0 in the new window
-                if (lane >= 64 - SB) {
+                if (lane >= 64 - SB) {   // rows of this block: their columns are staged in cur
                     const int row = jf - (lane - (64 - SB));
-                    yv[row] = cn::mul(y, dv[row]);
+                    yv[row] = cn::mul(y, lds_at<T>(cur + (row - jlo) * US * ES));
                 }
                 if (above) y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
                 jf -= SB;
@@ -1506,7 +1510,7 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
                     const int jn = jhi - sidx - 1;
                     if (above) {
                         const int row = jf - (lane - w);
-                        yv[row] = cn::mul(y, dv[row]);
+                        yv[row] = cn::mul(y, diag(cur, jlo, jhi, row));
                         y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
                     }
                     jf -= R;
@@ -1517,9 +1521,10 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
         vm_wait_le(c2);           // block b+1 has landed
         wave_sync();
     }
-    if (lane >= 64 - t) {
+    if (lane >= 64 - t) {   // the last block's buffer is still staged
+        const int jhi = n - 1 - (nblk - 1) * SP, jlo = max(0, jhi - SP + 1);
         const int row = jf - (lane - (64 - t));
-        yv[row] = cn::mul(y, dv[row]);
+        yv[row] = cn::mul(y, diag(bufs + ((nblk - 1) % 3) * bufbytes, jlo, jhi, row));
     }
     wave_sync();
 }
@@ -1531,17 +1536,17 @@ __host__ __device__ constexpr int bwd_group(int w, int SP) {
 static_assert(SPW_FEW % 32 == 0 && SPW_MANY % 8 == 0, "the step groups divide a staged block");
 
 template <class T, int SP>
-__device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, T* dv, const char* zslot,
+__device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, const char* zslot,
                                                   char* bufs, int bufbytes, int n, int w, int US, int lane) {
     switch (bwd_group(w, SP)) {
         case 32:
-            if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane);
+            if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane);
             break;
         case 16:
-            if constexpr (SP >= 16) wave_backward<T, 16, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane);
+            if constexpr (SP >= 16) wave_backward<T, 16, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane);
             break;
-        case 8: wave_backward<T, 8, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
-        default: wave_backward<T, 0, SP>(Uc0, yv, dv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 8: wave_backward<T, 8, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        default: wave_backward<T, 0, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane); break;
     }
 }
 
@@ -1580,8 +1585,8 @@ __device__ __forceinline__ void wave_vty(const T* __restrict__ V0, const T* yv, 
     }
 }
 
-// Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][dv: n entries][zero
-// slot, S, pivots of S][3 staging buffers].
+// Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][zero slot, S,
+// pivots of S][3 staging buffers].
 template <class T, int SP>
 __global__ void __launch_bounds__(64)
 k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv,
@@ -1593,8 +1598,7 @@ k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
     const int m = blockIdx.x, k = blockIdx.y;
     const size_t mn = (size_t)m * n;
     T* yv = reinterpret_cast<T*>(lds);
-    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
-    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    char* zslot = lds + align16(n * (int)sizeof(T));
     T* Sl = reinterpret_cast<T*>(zslot + 16);                  // capacitance LU (r x r) and pivots
     int* spl = reinterpret_cast<int*>(Sl + MAXR * MAXR);
     char* bufs = zslot + WAVE_SMALL;
@@ -1611,7 +1615,7 @@ k_solve_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
     if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
     wave_sync();
     wave_forward<T, (SP < 32 ? SP : 32), SP>(Lf + mn * KL, piv + mn, yv, zslot, bufs, bufbytes, n, KL, lane);   // kl + 1 <= 17 lanes
-    wave_backward_any<T, SP>(Uc + mn * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    wave_backward_any<T, SP>(Uc + mn * US, yv, zslot, bufs, bufbytes, n, w2, US, lane);
     if (r == 0) {
         for (int j = lane; j < n; j += 64) io_store<T>(io, mn, k, j, yv[j]);
         return;
@@ -1653,14 +1657,13 @@ k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W
     const int lane = threadIdx.x;
     const int m = blockIdx.x, a = blockIdx.y;
     T* yv = reinterpret_cast<T*>(lds);
-    T* dv = reinterpret_cast<T*>(lds + align16(n * (int)sizeof(T)));
-    char* zslot = lds + 2 * align16(n * (int)sizeof(T));
+    char* zslot = lds + align16(n * (int)sizeof(T));
     char* bufs = zslot + WAVE_SMALL;
     T* Wa = W + ((size_t)m * r + a) * n;
     wave_load<T>(yv, Wa, 1, n, lane);
     if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
     wave_sync();
-    wave_backward_any<T, SP>(Uc + (size_t)m * n * US, yv, dv, zslot, bufs, bufbytes, n, w2, US, lane);
+    wave_backward_any<T, SP>(Uc + (size_t)m * n * US, yv, zslot, bufs, bufbytes, n, w2, US, lane);
     for (int j = lane; j < n; j += 64) Wa[j] = yv[j];
     T t[MAXR];
     wave_vty<T>(Vt + (size_t)m * r * n, yv, n, r, lane, t);

@@ -333,7 +333,7 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
 int wave_steps(size_t waves) { return waves > 512 ? SPW_MANY : SPW_FEW; }
 int wave_buf(const cn_bb& h, int sp) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz, sp); }
 size_t wave_lds_bytes(const cn_bb& h, int sp) {
-    return (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + 3 * (size_t)wave_buf(h, sp);
+    return (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + wave_nbuf(sp) * (size_t)wave_buf(h, sp);
 }
 
 // Block length of the streamed sweeps: 16 steps when two buffers of buf16 bytes (the largest sweep

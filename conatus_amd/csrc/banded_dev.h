@@ -1277,6 +1277,10 @@ k_border_s(const T* __restrict__ Uf, const T* __restrict__ Vt, T* W, T* __restri
 // (fewer block hand-overs per sweep), 8 when the batch fills the chip (three buffers of 8 steps
 // leave LDS for 6 one-wavefront workgroups per CU at kl = 12, ku = 18, n = 395, instead of 2).
 constexpr int SPW_FEW = 32, SPW_MANY = 8;
+// Staging buffers of the wave-mode sweeps: three (copies issued two blocks ahead) for few
+// wavefronts, two (one block ahead) when many share a CU, where the other wavefronts hide the
+// copies' latency and the smaller LDS footprint lets more of them reside.
+__host__ __device__ constexpr int wave_nbuf(int SP) { return SP <= SPW_MANY ? 2 : 3; }
 
 // Ordering point of a one-wavefront workgroup: LDS operations of a wavefront execute in order,
 // and the staging copies are waited for explicitly (vm_wait_le), so only the compiler has to be
@@ -1401,14 +1405,16 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
     T y = lane < n ? yv[lane] : cn::zero<T>();
     int t = 0, jf = 0;            // steps since the last flush, first row finished since then
     const int Poff = SP * KL * ES;
-    // three staging buffers, copies issued two blocks ahead
+    // NB staging buffers, copies issued NB - 1 blocks ahead
+    constexpr int NB = wave_nbuf(SP), AH = NB - 1;
     wave_stage_fwd<T, SP>(L0, P0, n, KL, 0, bufs, lane);
-    const int c1 = nblk > 1 ? wave_stage_fwd<T, SP>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
+    const int c1 = AH > 1 && nblk > 1 ? wave_stage_fwd<T, SP>(L0, P0, n, KL, 1, bufs + bufbytes, lane) : 0;
     vm_wait_le(c1);
     wave_sync();
     for (int b = 0; b < nblk; ++b) {
-        char* cur = bufs + (b % 3) * bufbytes;
-        const int c2 = b + 2 < nblk ? wave_stage_fwd<T, SP>(L0, P0, n, KL, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        char* cur = bufs + (b % NB) * bufbytes;
+        const int c2 = b + AH < nblk ? wave_stage_fwd<T, SP>(L0, P0, n, KL, b + AH, bufs + ((b + AH) % NB) * bufbytes, lane)
+                                     : 0;
         const int j0 = b * SP, cnt = min(SP, n - j0);
         const int pv = lane < cnt ? lds_at<int>(cur + Poff + lane * 4) : 0;   // pivot of step lane
         const char* lb = upd ? cur + (lane - 1) * ES : zslot;
@@ -1440,7 +1446,7 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
                 }
             }
         }
-        vm_wait_le(c2);           // block b+1 has landed
+        vm_wait_le(AH > 1 ? c2 : 0);   // block b+1 has landed
         wave_sync();
     }
     if (lane >= 64 - t) yv[jf + lane - (64 - t)] = y;   // the last t finished rows
@@ -1476,13 +1482,15 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
     };
     T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
     int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
+    constexpr int NB = wave_nbuf(SP), AH = NB - 1;
     wave_stage_bwd<T, SP>(Uc0, n, US, 0, bufs, lane);
-    const int c1 = nblk > 1 ? wave_stage_bwd<T, SP>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
+    const int c1 = AH > 1 && nblk > 1 ? wave_stage_bwd<T, SP>(Uc0, n, US, 1, bufs + bufbytes, lane) : 0;
     vm_wait_le(c1);
     wave_sync();
     for (int b = 0; b < nblk; ++b) {
-        char* cur = bufs + (b % 3) * bufbytes;
-        const int c2 = b + 2 < nblk ? wave_stage_bwd<T, SP>(Uc0, n, US, b + 2, bufs + ((b + 2) % 3) * bufbytes, lane) : 0;
+        char* cur = bufs + (b % NB) * bufbytes;
+        const int c2 = b + AH < nblk ? wave_stage_bwd<T, SP>(Uc0, n, US, b + AH, bufs + ((b + AH) % NB) * bufbytes, lane)
+                                     : 0;
         const int jhi = n - 1 - b * SP, jlo = max(0, jhi - SP + 1);
         const int cnt = jhi - jlo + 1;
         const char* ub = upd ? cur + lane * ES : zslot;
@@ -1518,13 +1526,13 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
                 }
             }
         }
-        vm_wait_le(c2);           // block b+1 has landed
+        vm_wait_le(AH > 1 ? c2 : 0);   // block b+1 has landed
         wave_sync();
     }
     if (lane >= 64 - t) {   // the last block's buffer is still staged
         const int jhi = n - 1 - (nblk - 1) * SP, jlo = max(0, jhi - SP + 1);
         const int row = jf - (lane - (64 - t));
-        yv[row] = cn::mul(y, diag(bufs + ((nblk - 1) % 3) * bufbytes, jlo, jhi, row));
+        yv[row] = cn::mul(y, diag(bufs + ((nblk - 1) % NB) * bufbytes, jlo, jhi, row));
     }
     wave_sync();
 }

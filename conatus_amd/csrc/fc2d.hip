@@ -159,35 +159,49 @@ __device__ __forceinline__ void fc_term(double (&f)[8], const zc e, const zc* Zj
 // k_fc_line for even Mx: the points x_i and x_{i+Mx/2} from one pass over the modes
 // (e^{i k kappa x_{i+Mx/2}} = (-1)^k e^{i k kappa x_i}: even-k and odd-k terms summed apart), and
 // the x-forward over i < Mx/2 of G_i + G_{i+Mx/2} (even k) or G_i - G_{i+Mx/2} (odd k).
-// z-backward of every grid line at once for large batches: one workgroup per (mode, member) reads
-// the mode's coefficients once, Zg[member][j][2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)] (the
-// same sums, in the same order, as the fused form inside k_fc_line2).
+// z-backward of every grid line at once for large batches: one workgroup per ZB modes of a member
+// reads those modes' coefficients once, Zg[member][j][2f + v][k] = sum_n Bz_v[n][j] X[k][pos_f(n)]
+// (the same sums, in the same order, as the fused form inside k_fc_line2); each Bz entry read
+// from L2 serves the ZB modes (one workgroup per mode read the whole table once per mode).
+constexpr int ZB = 4;
 __global__ void __launch_bounds__(256)
 k_fc_zback(const zc* __restrict__ X, const int* __restrict__ fidx, const double* __restrict__ Bz,
            zc* __restrict__ Zg, int K, int n, int N, int Mz) {
     extern __shared__ __align__(16) char lds[];
-    zc* c = reinterpret_cast<zc*>(lds);             // [3][N]
-    const int k = blockIdx.x, tid = threadIdx.x;
-    const zc* Xk = X + ((size_t)blockIdx.y * K + k) * n;
+    zc* c = reinterpret_cast<zc*>(lds);             // [ZB][3][N]
+    const int k0 = blockIdx.x * ZB, tid = threadIdx.x;
+    const int nk = min(ZB, K - k0);
+    const zc* X0 = X + ((size_t)blockIdx.y * K + k0) * n;
     Zg += (size_t)blockIdx.y * Mz * 6 * K;
-    for (int q = tid; q < 3 * N; q += blockDim.x) c[q] = Xk[fidx[q]];
+    for (int q = tid; q < ZB * 3 * N; q += blockDim.x) {
+        const int kk = q / (3 * N), r = q - kk * 3 * N;
+        c[q] = kk < nk ? X0[(size_t)kk * n + fidx[r]] : zc{0.0, 0.0};
+    }
     __syncthreads();
     for (int p = tid; p < 3 * Mz; p += blockDim.x) {
         const int f = p / Mz, j = p - f * Mz;
-        const zc* cf = c + (size_t)f * N;
-        double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
-#pragma unroll 8
+        double r0[ZB], i0[ZB], r1[ZB], i1[ZB];
+#pragma unroll
+        for (int kk = 0; kk < ZB; ++kk) r0[kk] = i0[kk] = r1[kk] = i1[kk] = 0.0;
+#pragma unroll 4
         for (int nn = 0; nn < N; ++nn) {
-            const zc x = cf[nn];
             const double b0 = Bz[(size_t)nn * Mz + j], b1 = Bz[((size_t)N + nn) * Mz + j];
-            r0 = fma(b0, x.re, r0);
-            i0 = fma(b0, x.im, i0);
-            r1 = fma(b1, x.re, r1);
-            i1 = fma(b1, x.im, i1);
+#pragma unroll
+            for (int kk = 0; kk < ZB; ++kk) {
+                const zc x = c[((size_t)kk * 3 + f) * N + nn];
+                r0[kk] = fma(b0, x.re, r0[kk]);
+                i0[kk] = fma(b0, x.im, i0[kk]);
+                r1[kk] = fma(b1, x.re, r1[kk]);
+                i1[kk] = fma(b1, x.im, i1[kk]);
+            }
         }
-        zc* z = Zg + ((size_t)j * 6 + 2 * f) * K + k;
-        z[0] = zc{r0, i0};
-        z[K] = zc{r1, i1};
+#pragma unroll
+        for (int kk = 0; kk < ZB; ++kk) {
+            if (kk >= nk) break;
+            zc* z = Zg + ((size_t)j * 6 + 2 * f) * K + k0 + kk;
+            z[0] = zc{r0[kk], i0[kk]};
+            z[K] = zc{r1[kk], i1[kk]};
+        }
     }
 }
 
@@ -360,9 +374,9 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
     // many modes: the z-backward of each mode once (k_fc_zback) into the scratch after the line
     // products, then the lines read their values; few modes: fused into the line kernel
     zc* Zg = static_cast<zc*>(Gh) + (size_t)members * 3 * K * Mz;
-    const bool pre = Mx % 2 == 0 && (long long)members * K >= 4096 && (size_t)3 * Nz * sizeof(zc) <= 64 * 1024;
+    const bool pre = Mx % 2 == 0 && (long long)members * K >= 4096 && (size_t)ZB * 3 * Nz * sizeof(zc) <= 64 * 1024;
     if (pre) {
-        hipLaunchKernelGGL(k_fc_zback, dim3(K, members), dim3(256), (size_t)3 * Nz * sizeof(zc), s,
+        hipLaunchKernelGGL(k_fc_zback, dim3((K + ZB - 1) / ZB, members), dim3(256), (size_t)ZB * 3 * Nz * sizeof(zc), s,
                            static_cast<const zc*>(X), fidx, Bz, Zg, K, n, Nz, Mz);
         hipLaunchKernelGGL(k_fc_line2<true>, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx,
                            Bz, static_cast<const zc*>(Wx), static_cast<zc*>(Gh), static_cast<const zc*>(Zg), K, n, Nz,

@@ -322,30 +322,43 @@ k_fc_line2(const zc* __restrict__ X, const int* __restrict__ fidx, const double*
     }
 }
 
+// (ZM modes of a member per workgroup: each Az entry read from L2 serves the ZM modes; the sums
+// of every output keep the order of the one-mode form)
+template <int ZM>
 __global__ void __launch_bounds__(256)
 k_fc_mode_rows(const zc* __restrict__ Gh, const double* __restrict__ Az, zc* __restrict__ F, int K, int Mz, int N,
                int n, int R) {
     extern __shared__ __align__(16) char lds[];
-    zc* g = reinterpret_cast<zc*>(lds);              // [3][Mz]: Gh[e][k][:]
-    const int k = blockIdx.x, tid = threadIdx.x;
+    zc* g = reinterpret_cast<zc*>(lds);              // [ZM][3][Mz]: Gh[e][k0 + kk][:]
+    const int k0 = blockIdx.x * ZM, tid = threadIdx.x;
+    const int nk = min(ZM, K - k0);
     Gh += (size_t)blockIdx.y * 3 * K * Mz;
     F += (size_t)blockIdx.y * K * n;
-    for (int q = tid; q < 3 * Mz; q += blockDim.x) {
-        const int e = q / Mz, jj = q - e * Mz;
-        g[q] = Gh[((size_t)e * K + k) * Mz + jj];
+    for (int q = tid; q < ZM * 3 * Mz; q += blockDim.x) {
+        const int kk = q / (3 * Mz), r = q - kk * 3 * Mz;
+        const int e = r / Mz, jj = r - e * Mz;
+        g[q] = kk < nk ? Gh[((size_t)e * K + k0 + kk) * Mz + jj] : zc{0.0, 0.0};
     }
     __syncthreads();
     for (int p = tid; p < 3 * N; p += blockDim.x) {
         const int e = p / N, nn = p - e * N;
-        const zc* ge = g + (size_t)e * Mz;
-        double re = 0.0, im = 0.0;
+        double re[ZM], im[ZM];
+#pragma unroll
+        for (int kk = 0; kk < ZM; ++kk) re[kk] = im[kk] = 0.0;
         for (int jj = 0; jj < Mz; ++jj) {
             const double a = Az[(size_t)jj * N + nn];
-            const zc v = ge[jj];
-            re = fma(a, v.re, re);
-            im = fma(a, v.im, im);
+#pragma unroll
+            for (int kk = 0; kk < ZM; ++kk) {
+                const zc v = g[((size_t)kk * 3 + e) * Mz + jj];
+                re[kk] = fma(a, v.re, re[kk]);
+                im[kk] = fma(a, v.im, im[kk]);
+            }
         }
-        F[(size_t)k * n + R + 4 * nn + e] = zc{re, im};
+#pragma unroll
+        for (int kk = 0; kk < ZM; ++kk) {
+            if (kk >= nk) break;
+            F[(size_t)(k0 + kk) * n + R + 4 * nn + e] = zc{re[kk], im[kk]};
+        }
     }
 }
 
@@ -388,8 +401,12 @@ int cn_rb2d_explicit_many(const void* X, const int* fidx, void* F, void* Gh, con
     else
         hipLaunchKernelGGL(k_fc_line, dim3(Mz, members), dim3(256), lds_a, s, static_cast<const zc*>(X), fidx, Bz,
                            static_cast<const zc*>(Wx), static_cast<zc*>(Gh), K, n, Nz, Mz, Mx, kappa);
-    hipLaunchKernelGGL(k_fc_mode_rows, dim3(K, members), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
-                       static_cast<zc*>(F), K, Mz, Nz, n, R);
+    if (pre && (size_t)ZB * lds_b <= 64 * 1024)
+        hipLaunchKernelGGL(k_fc_mode_rows<ZB>, dim3((K + ZB - 1) / ZB, members), dim3(256), ZB * lds_b, s,
+                           static_cast<const zc*>(Gh), Az, static_cast<zc*>(F), K, Mz, Nz, n, R);
+    else
+        hipLaunchKernelGGL(k_fc_mode_rows<1>, dim3(K, members), dim3(256), lds_b, s, static_cast<const zc*>(Gh), Az,
+                           static_cast<zc*>(F), K, Mz, Nz, n, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         cn::set_error(std::string("cn_rb2d_explicit: ") + hipGetErrorString(e));

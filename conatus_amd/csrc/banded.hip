@@ -284,12 +284,15 @@ struct cn_bb {
     int streamed = 1;   // 0: always use the direct solve kernel (CNHIP_SOLVE=direct)
     int mode = 0;       // 0: lane mode (one lane per system and rhs), 1: wave mode (one wavefront each)
     int il = 16;        // interleave of the factor arrays: 16 systems (lane mode) or 1 (wave mode)
+    int parts = 1;      // wave mode: wavefronts per system of the solve (>= 2: partitioned sweeps, spikes)
+    int plen = 0, ls = 0, lsu = 0;   // longest chunk; row strides of the forward / backward spikes
     int nrhs_cap = 0;
     size_t esz = 16;
     double cond_max = 1.0, factor_ms = 0.0;
     int device = 0;
     DevBuf Ab, Uf, Lf, piv, Vt, W, S, spiv, cond, info, perm, ws;
     DevBuf Wt, St, stpiv, cond_t, info_t;   // transposed Woodbury data (built on demand)
+    DevBuf SpL, SpU;                        // partitioned wave mode: the chunks' coupling responses
     bool tr_ready = false;
     std::vector<double> hcond;
     std::vector<int> hperm;   // host copy of the row permutation (border rows first)
@@ -331,9 +334,43 @@ void layout_one(const cn_bb& h, int m, const int32_t* ip, const int32_t* ix, con
 // staged block of the wave-mode sweeps for a launch of `waves` wavefronts: the longer block while
 // the batch leaves CUs idle at two workgroups per CU, the shorter one (more resident) beyond
 int wave_steps(size_t waves) { return waves > 512 ? SPW_MANY : SPW_FEW; }
+constexpr int PART_MAX_B = 256;   // automatic partitioned sweeps up to this many systems
 int wave_buf(const cn_bb& h, int sp) { return wave_buf_bytes(h.KLP, h.US, (int)h.esz, sp); }
 size_t wave_lds_bytes(const cn_bb& h, int sp) {
     return (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + wave_nbuf(sp) * (size_t)wave_buf(h, sp);
+}
+
+// Partitioned wave mode.  LDS of one wavefront's chunk rows (the chunk and its coupling rows)
+int part_yw_bytes(const cn_bb& h) { return align16((h.plen + std::max(h.KLP, h.M - 1)) * (int)h.esz); }
+// LDS of the partitioned solve with staged blocks of sp steps
+size_t part_lds_bytes(const cn_bb& h, int P, int sp) {
+    return (size_t)align16(h.n * (int)h.esz) + WAVE_SMALL + (size_t)P * (PART_W + MAXR) * h.esz +
+           (size_t)P * (part_yw_bytes(h) + wave_nbuf(sp) * (size_t)wave_buf(h, sp));
+}
+// staged block length of the partitioned solve: the longest whose LDS fits (0: none)
+int part_steps(const cn_bb& h, int P) {
+    for (int sp : {16, SPW_MANY})
+        if (part_lds_bytes(h, P, sp) <= (size_t)LDS_CAP) return sp;
+    return 0;
+}
+// Wavefronts per system: 8, else 4, when the chunks are at least one coupling width (max(kl,
+// kl + ku)) long; CNHIP_PARTS = 2, 4 or 8 asks for that count (2 splits the chain too little to pay
+// for the coupling: measured no faster than one wavefront); 1 when none applies or the LDS does not
+// fit.
+int pick_parts(cn_bb& h) {
+    const int wmax = std::max(h.KLP, h.M - 1);
+    if (h.KLP < 1 || wmax > PART_W) return 1;
+    const char* e = std::getenv("CNHIP_PARTS");
+    const int want = e ? std::atoi(e) : 0;
+    for (int P : {8, 4, 2}) {
+        if (want > 0 ? P != want : P == 2) continue;
+        if (h.n / P < wmax) continue;
+        h.plen = (h.n + P - 1) / P;
+        if (part_steps(h, P) == 0) continue;
+        return P;
+    }
+    h.plen = 0;
+    return 1;
 }
 
 // Block length of the streamed sweeps: 16 steps when two buffers of buf16 bytes (the largest sweep
@@ -386,6 +423,15 @@ void run_factor(cn_bb& h, hipStream_t s) {
         hipLaunchKernelGGL((k_scale_ucol<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                            static_cast<T*>(h.Uf.p), tot, h.US);
         hip_check(hipGetLastError(), "k_scale_ucol launch");
+        if (h.parts > 1) {
+            const int ywb = part_yw_bytes(h), bb = wave_buf(h, SPW_FEW);
+            const size_t lds = (size_t)ywb + 16 + wave_nbuf(SPW_FEW) * (size_t)bb;
+            hipLaunchKernelGGL((k_spikes_w<T, SPW_FEW>), dim3(h.B, h.parts, std::max(h.KLP, h.M - 1)), dim3(64), lds,
+                               s, static_cast<const T*>(h.Uf.p), static_cast<const T*>(h.Lf.p),
+                               static_cast<const int*>(h.piv.p), static_cast<T*>(h.SpL.p), static_cast<T*>(h.SpU.p),
+                               h.n, h.parts, h.KLP, h.M, h.US, h.ls, h.lsu, ywb, bb);
+            hip_check(hipGetLastError(), "k_spikes_w launch");
+        }
     }
     h.tr_ready = false;   // the transposed Woodbury data follow the factors (built on demand)
     if (h.r == 0) return;
@@ -577,11 +623,18 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
             (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e, SPW_FEW) <=
                 (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
-        const std::string want = g_mode_request == 1 ? "lane" : g_mode_request == 2 ? "wave" : em ? em : "";
-        if (want == "wave") h.mode = wave_ok ? 1 : 0;
+        const std::string want = g_mode_request == 1   ? "lane"
+                                 : g_mode_request == 2 ? "wave"
+                                 : g_mode_request == 3 ? "part"
+                                 : em                  ? em
+                                                       : "";
+        if (want == "wave" || want == "part") h.mode = wave_ok ? 1 : 0;
         else if (want == "lane") h.mode = 0;
         else h.mode = (lane_fast && B >= 1024) || !wave_ok ? 0 : 1;
         h.il = h.mode == 1 ? 1 : IL;
+        // partitioned sweeps when asked for, and by default while the batch has fewer systems than
+        // a quarter of the chip's SIMDs (one wavefront per system would leave most of them idle)
+        if (h.mode == 1 && (want == "part" || (want.empty() && B <= PART_MAX_B))) h.parts = pick_parts(h);
     }
     const size_t es = sizeof(T);
     const size_t nAb = checked_mul({(size_t)h.Bp, (size_t)n, (size_t)h.M});
@@ -603,6 +656,12 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
     h.cond.alloc((size_t)h.Bp * sizeof(double));
     h.info.alloc((size_t)h.Bp * sizeof(int));
     h.perm.alloc((size_t)n * sizeof(int));
+    if (h.parts > 1) {
+        h.ls = h.plen + h.KLP;
+        h.lsu = h.plen + h.M - 1;
+        h.SpL.alloc(checked_mul({(size_t)B, (size_t)h.parts, (size_t)h.KLP, (size_t)h.ls, es}));
+        h.SpU.alloc(checked_mul({(size_t)B, (size_t)h.parts, (size_t)(h.M - 1), (size_t)h.lsu, es}));
+    }
     h.Wt.alloc(std::max<size_t>(nV, 1) * es);
     h.St.alloc(std::max<size_t>((size_t)h.Bp * r * r, 1) * es);
     h.stpiv.alloc(std::max<size_t>((size_t)h.Bp * r, 1) * sizeof(int));
@@ -704,6 +763,30 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s, void* ws = nullpt
         reserve_impl(h, nrhs);
     }
     if (ws == nullptr) ws = h.ws.p;
+    if (h.mode == 1 && h.parts > 1) {
+        const int P = h.parts, sp = part_steps(h, P);
+        const size_t lds = part_lds_bytes(h, P, sp);
+        const int ywb = part_yw_bytes(h), bb = wave_buf(h, sp);
+        if (h.dtype == CN_C128)
+            hipLaunchKernelGGL((sp == 16 ? k_solve_p<zc, 16> : k_solve_p<zc, SPW_MANY>), dim3(h.B, nrhs), dim3(64 * P),
+                               lds, s, static_cast<const zc*>(h.Uf.p), static_cast<const zc*>(h.Lf.p),
+                               static_cast<const int*>(h.piv.p), static_cast<const zc*>(h.Vt.p),
+                               static_cast<const zc*>(h.W.p), static_cast<const zc*>(h.S.p),
+                               static_cast<const int*>(h.spiv.p), static_cast<const int*>(h.perm.p),
+                               static_cast<const zc*>(h.SpL.p), static_cast<const zc*>(h.SpU.p), io, h.n, h.r, h.KLP,
+                               h.M, h.US, P, h.ls, h.lsu, ywb, bb);
+        else
+            hipLaunchKernelGGL((sp == 16 ? k_solve_p<double, 16> : k_solve_p<double, SPW_MANY>), dim3(h.B, nrhs),
+                               dim3(64 * P), lds, s, static_cast<const double*>(h.Uf.p),
+                               static_cast<const double*>(h.Lf.p), static_cast<const int*>(h.piv.p),
+                               static_cast<const double*>(h.Vt.p), static_cast<const double*>(h.W.p),
+                               static_cast<const double*>(h.S.p), static_cast<const int*>(h.spiv.p),
+                               static_cast<const int*>(h.perm.p), static_cast<const double*>(h.SpL.p),
+                               static_cast<const double*>(h.SpU.p), io, h.n, h.r, h.KLP, h.M, h.US, P, h.ls, h.lsu,
+                               ywb, bb);
+        hip_check(hipGetLastError(), "k_solve_p launch");
+        return;
+    }
     if (h.mode == 1) {
         const int sp = wave_steps((size_t)h.B * nrhs);
         const size_t lds = wave_lds_bytes(h, sp);
@@ -933,9 +1016,9 @@ int cn_bb_props_get(cn_bb_t h, cn_bb_props* p) {
         p->transpose_ready = h->tr_ready ? 1 : 0;
         p->cond_max = h->cond_max;
         p->factor_ms = h->factor_ms;
-        p->solve_mode = h->mode;
+        p->solve_mode = h->mode == 1 && h->parts > 1 ? 2 : h->mode;
         p->factor_bytes = (int64_t)(h->Uf.bytes + h->Lf.bytes + h->piv.bytes + h->Vt.bytes + h->W.bytes + h->S.bytes +
-                                    h->spiv.bytes);
+                                    h->spiv.bytes + h->SpL.bytes + h->SpU.bytes);
     });
 }
 
@@ -954,7 +1037,8 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
         const cn_bb& S_ = *src;
         cn_bb& D = *dst;
         if (S_.dtype != D.dtype || S_.n != D.n || S_.r != D.r || S_.KLP != D.KLP || S_.M != D.M || S_.US != D.US ||
-            S_.WU != D.WU || S_.il != D.il || S_.mode != D.mode || S_.pivot != D.pivot || S_.device != D.device)
+            S_.WU != D.WU || S_.il != D.il || S_.mode != D.mode || S_.parts != D.parts || S_.pivot != D.pivot ||
+            S_.device != D.device)
             fail(CN_EINVAL, "cn_bb_copy_systems: the handles differ in type, size, band, border rank, kernel mode or device");
         if (count < 0 || src_first < 0 || dst_first < 0 || src_first + count > S_.B || dst_first + count > D.B)
             fail(CN_EINVAL, "cn_bb_copy_systems: system range out of bounds");
@@ -1001,6 +1085,10 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
         }
         major(S_.S, D.S, r * r, es);
         major(S_.spiv, D.spiv, r, sizeof(int));
+        if (D.parts > 1) {
+            major(S_.SpL, D.SpL, (size_t)D.parts * D.KLP * D.ls, es);
+            major(S_.SpU, D.SpU, (size_t)D.parts * (D.M - 1) * D.lsu, es);
+        }
         major(S_.cond, D.cond, 1, sizeof(double));
         major(S_.info, D.info, 1, sizeof(int));
         if ((int)D.hcond.size() < D.B) D.hcond.assign(D.B, 1.0);
@@ -1013,8 +1101,8 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
 }
 
 int cn_bb_request_mode(int mode) {
-    if (mode < 0 || mode > 2) {
-        g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane) or 2 (wave)";
+    if (mode < 0 || mode > 3) {
+        g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane), 2 (wave) or 3 (partitioned wave)";
         return CN_EINVAL;
     }
     g_mode_request = mode;

@@ -1395,14 +1395,16 @@ __device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
 
 template <class T, int SB, int SP>
 __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int* __restrict__ P0, T* yv,
-                                             const char* zslot, char* bufs, int bufbytes, int n, int KL, int lane) {
+                                             const char* zslot, char* bufs, int bufbytes, int n, int KL, int lane,
+                                             int nrow = -1, bool keep = false) {
     constexpr int ES = (int)sizeof(T);
     const int w = KL + 1;
     const int R = 64 - w;
     const int nblk = (n + SP - 1) / SP;
+    const int nr = nrow < 0 ? n : nrow;   // rows present in yv (n steps; a chunk also has KL rows after them)
     const bool upd = lane >= 1 && lane <= KL;
     const bool above = lane >= w;
-    T y = lane < n ? yv[lane] : cn::zero<T>();
+    T y = lane < nr ? yv[lane] : cn::zero<T>();
     int t = 0, jf = 0;            // steps since the last flush, first row finished since then
     const int Poff = SP * KL * ES;
     // NB staging buffers, copies issued NB - 1 blocks ahead
@@ -1429,7 +1431,7 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
                 for (int u = 0; u < SB; ++u) fwd_step<T>(y, lv[q * SB + u], __builtin_amdgcn_readlane(pv, q * SB + u), lane);
                 const int jn = j0 + (q + 1) * SB;     // first row of the new window
                 if (lane >= 64 - SB) yv[jf + lane - (64 - SB)] = y;
-                if (above) y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                if (above) y = (jn + lane < nr) ? yv[jn + lane] : cn::zero<T>();
                 jf += SB;
             }
         } else {
@@ -1439,7 +1441,7 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
                     const int jn = j0 + sidx + 1;
                     if (above) {
                         yv[jf + lane - w] = y;
-                        y = (jn + lane < n) ? yv[jn + lane] : cn::zero<T>();
+                        y = (jn + lane < nr) ? yv[jn + lane] : cn::zero<T>();
                     }
                     jf += R;
                     t = 0;
@@ -1450,6 +1452,7 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
         wave_sync();
     }
     if (lane >= 64 - t) yv[jf + lane - (64 - t)] = y;   // the last t finished rows
+    if (keep && lane < KL && n + lane < nr) yv[n + lane] = y;   // the rows still in the window
     wave_sync();
 }
 
@@ -1470,7 +1473,8 @@ __device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
 
 template <class T, int SB, int SP>
 __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, const char* zslot, char* bufs,
-                                              int bufbytes, int n, int w, int US, int lane) {
+                                              int bufbytes, int n, int w, int US, int lane, int rlo = 0,
+                                              bool keep = false) {
     constexpr int ES = (int)sizeof(T);
     const int R = 64 - w;
     const int nblk = (n + SP - 1) / SP;
@@ -1480,7 +1484,7 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
     auto diag = [&](const char* cur, int jlo, int jhi, int row) -> T {
         return (row >= jlo && row <= jhi) ? lds_at<T>(cur + (row - jlo) * US * ES) : Uc0[(size_t)row * US];
     };
-    T y = (n - 1 - lane >= 0) ? yv[n - 1 - lane] : cn::zero<T>();
+    T y = (n - 1 - lane >= rlo) ? yv[n - 1 - lane] : cn::zero<T>();
     int t = 0, jf = n - 1;        // steps since the last flush, first row finished since then
     constexpr int NB = wave_nbuf(SP), AH = NB - 1;
     wave_stage_bwd<T, SP>(Uc0, n, US, 0, bufs, lane);
@@ -1508,7 +1512,7 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
                     const int row = jf - (lane - (64 - SB));
                     yv[row] = cn::mul(y, lds_at<T>(cur + (row - jlo) * US * ES));
                 }
-                if (above) y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                if (above) y = (jn - lane >= rlo) ? yv[jn - lane] : cn::zero<T>();
                 jf -= SB;
             }
         } else {
@@ -1519,7 +1523,7 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
                     if (above) {
                         const int row = jf - (lane - w);
                         yv[row] = cn::mul(y, diag(cur, jlo, jhi, row));
-                        y = (jn - lane >= 0) ? yv[jn - lane] : cn::zero<T>();
+                        y = (jn - lane >= rlo) ? yv[jn - lane] : cn::zero<T>();
                     }
                     jf -= R;
                     t = 0;
@@ -1534,6 +1538,7 @@ __device__ __forceinline__ void wave_backward(const T* __restrict__ Uc0, T* yv, 
         const int row = jf - (lane - (64 - t));
         yv[row] = cn::mul(y, diag(bufs + ((nblk - 1) % NB) * bufbytes, jlo, jhi, row));
     }
+    if (keep && lane < w - 1 && -1 - lane >= rlo) yv[-1 - lane] = y;   // rows -1, -2, .. still in the window
     wave_sync();
 }
 
@@ -1545,16 +1550,17 @@ static_assert(SPW_FEW % 32 == 0 && SPW_MANY % 8 == 0, "the step groups divide a 
 
 template <class T, int SP>
 __device__ __forceinline__ void wave_backward_any(const T* __restrict__ Uc0, T* yv, const char* zslot,
-                                                  char* bufs, int bufbytes, int n, int w, int US, int lane) {
+                                                  char* bufs, int bufbytes, int n, int w, int US, int lane,
+                                                  int rlo = 0, bool keep = false) {
     switch (bwd_group(w, SP)) {
         case 32:
-            if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane);
+            if constexpr (SP >= 32) wave_backward<T, 32, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane, rlo, keep);
             break;
         case 16:
-            if constexpr (SP >= 16) wave_backward<T, 16, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane);
+            if constexpr (SP >= 16) wave_backward<T, 16, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane, rlo, keep);
             break;
-        case 8: wave_backward<T, 8, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane); break;
-        default: wave_backward<T, 0, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane); break;
+        case 8: wave_backward<T, 8, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane, rlo, keep); break;
+        default: wave_backward<T, 0, SP>(Uc0, yv, zslot, bufs, bufbytes, n, w, US, lane, rlo, keep); break;
     }
 }
 
@@ -1680,6 +1686,288 @@ k_border_w(const T* __restrict__ Uc, const T* __restrict__ Vt, T* __restrict__ W
 #pragma unroll
         for (int q = 0; q < MAXR; ++q)
             if (q < r) Sm[q] = (q == a) ? cn::fmac(cn::one<T>(), cn::one<T>(), t[q]) : t[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Partitioned wave mode: the sweeps of one system split over P wavefronts of one workgroup, for
+// launches with fewer systems than the chip has SIMDs, where one wavefront per system leaves the
+// sweep a chain of n dependent steps on a single SIMD.  Chunk c covers rows [a_c, a_{c+1}) with
+// a_c = c n / P; every chunk is at least as long as the coupling widths (KL rows ahead of the
+// forward sweep, WB = M - 1 rows behind the backward sweep).  Each wavefront sweeps its chunk
+// with the coupling rows it would receive from its neighbour set to zero; the sweeps are linear,
+// so the true result is that local result plus the chunk's response to the coupling values,
+// taken from the factor-time spikes:
+//   SpL[m][c][u][i], i < LS:  the forward sweep of chunk c from a unit value at head row u
+//                             (rows 0 .. Lc-1 of the chunk, then the KL rows after it);
+//   SpU[m][c][u][i], i < LSU: the backward sweep of chunk c from a unit value at row Lc - WB + u
+//                             (i = row + WB: the WB rows before the chunk, then its Lc rows).
+// The coupling values come from a short chain over the chunks (q_{c+1} = out_c + T_c q_c with
+// T_c the outgoing part of the spikes, KL x KL forward and WB x WB backward), then every chunk
+// adds its spikes times q_c to its rows.  The arithmetic of a system depends on (n, kl, ku, r)
+// only, never on the batch, so a system solves bit for bit alike in any batch of this mode.
+__host__ __device__ constexpr int part_lo(int c, int n, int P) { return (int)(((long long)c * n) / P); }
+constexpr int PART_W = 32;   // largest coupling width (KL or WB) of the partitioned mode
+
+// Spikes of one (system, chunk, unit): grid (B, P, max(KL, WB)), one wavefront each.  LDS: [yw:
+// ywbytes][zero slot][NB staging buffers].
+template <class T, int SP>
+__global__ void __launch_bounds__(64)
+k_spikes_w(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv, T* __restrict__ SpL,
+           T* __restrict__ SpU, int n, int P, int KL, int M, int US, int LS, int LSU, int ywbytes, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x;
+    const int m = blockIdx.x, c = blockIdx.y, u = blockIdx.z;
+    const int WB = M - 1;
+    const int a = part_lo(c, n, P), Lc = part_lo(c + 1, n, P) - a;
+    const size_t mn = (size_t)m * n;
+    T* yw = reinterpret_cast<T*>(lds);
+    char* zslot = lds + ywbytes;
+    char* bufs = zslot + 16;
+    if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    if (c >= 1 && u < KL) {
+        for (int i = lane; i < Lc + KL; i += 64) yw[i] = (i == u) ? cn::one<T>() : cn::zero<T>();
+        wave_sync();
+        wave_forward<T, (SP < 32 ? SP : 32), SP>(Lf + (mn + a) * KL, piv + mn + a, yw, zslot, bufs, bufbytes, Lc, KL,
+                                               lane, Lc + KL, true);
+        T* dst = SpL + (((size_t)m * P + c) * KL + u) * LS;
+        for (int i = lane; i < Lc + KL; i += 64) dst[i] = yw[i];
+        wave_sync();
+    }
+    if (c <= P - 2 && u < WB) {
+        for (int i = lane; i < WB + Lc; i += 64) yw[i] = (i == Lc + u) ? cn::one<T>() : cn::zero<T>();
+        wave_sync();
+        wave_backward_any<T, SP>(Uc + (mn + a) * US, yw + WB, zslot, bufs, bufbytes, Lc, M, US, lane, -WB, true);
+        T* dst = SpU + (((size_t)m * P + c) * WB + u) * LSU;
+        for (int i = lane; i < WB + Lc; i += 64) dst[i] = yw[i];
+    }
+}
+
+// acc + sum_{u < wdt} m[u] q[u]: the q values (one LDS slot of PART_W entries, zero beyond wdt, as
+// m is) read eight at a time, so that a group costs one LDS latency, into four partial sums (term u
+// into sum u mod 4, joined as acc + ((s0 + s1) + (s2 + s3))), so that the multiply-adds do not form
+// one dependent chain
+template <class T>
+__device__ __forceinline__ T part_dot(const T (&m)[PART_W], const T* q, int wdt, T acc) {
+    T ps[4] = {cn::zero<T>(), cn::zero<T>(), cn::zero<T>(), cn::zero<T>()};
+#pragma unroll
+    for (int u0 = 0; u0 < PART_W; u0 += 8) {
+        if (u0 < wdt) {
+            T qq[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) qq[j] = q[u0 + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ps[j & 3] = cn::fmac(m[u0 + j], qq[j], ps[j & 3]);
+        }
+    }
+    return addv(acc, addv(addv(ps[0], ps[1]), addv(ps[2], ps[3])));
+}
+
+// z = S^-1 t with t across the lanes (lane i holds t_i, i < r): the capacitance LU (column-major
+// in LDS, pivots sp) read once, row `lane` per lane; row interchanges in order, then the unit lower
+// and the upper triangle by columns, the finished entry broadcast from its lane at each step
+template <class T>
+__device__ __forceinline__ T cap_solve_lanes(T ti, const T* Sm, const int* sp, int r, int lane) {
+    T row[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) row[q] = (q < r && lane < r) ? Sm[q * r + lane] : cn::zero<T>();
+    int pv[MAXR];
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) pv[i] = i < r ? sp[i] : i;
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i < r && pv[i] != i) {
+            const T vi = bcast(ti, i), vp = bcast(ti, pv[i]);
+            ti = cn::sel(lane == i, vp, cn::sel(lane == pv[i], vi, ti));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) {
+        if (q < r) {
+            const T v = bcast(ti, q);
+            if (lane > q) ti = cn::fnms(row[q], v, ti);
+        }
+    }
+#pragma unroll
+    for (int q = MAXR - 1; q >= 0; --q) {
+        if (q < r) {
+            if (lane == q) ti = cn::mul(ti, cn::recip(row[q]));
+            const T v = bcast(ti, q);
+            if (lane < q) ti = cn::fnms(row[q], v, ti);
+        }
+    }
+    return ti;
+}
+
+// rows [0, Lc) of a chunk: v_i += sum_u Sp[u][off + i] q[u] (u < wdt), written to dst[i]
+template <class T>
+__device__ __forceinline__ void part_fixup(const T* __restrict__ Sp, size_t ustride, int off, int wdt, const T* q,
+                                           const T* src, T* dst, int Lc, bool spike, int lane) {
+    for (int i0 = 0; i0 < Lc; i0 += 64) {
+        const int i = i0 + lane, ic = min(i, Lc - 1);
+        T v = src[ic];
+        if (spike) {
+            T sv[PART_W];
+#pragma unroll
+            for (int u = 0; u < PART_W; ++u) sv[u] = u < wdt ? Sp[(size_t)u * ustride + off + ic] : cn::zero<T>();
+            v = part_dot<T>(sv, q, wdt, v);
+        }
+        if (i < Lc) dst[i] = v;
+    }
+}
+
+// Partitioned solve: grid (B, nrhs), P wavefronts per workgroup (wavefront c: chunk c).  LDS:
+// [y: n entries][zero slot, S, pivots of S][q: P x PART_W][partial V^T y: P x MAXR]
+// [per wavefront: yw (ywbytes), NB staging buffers].
+template <class T, int SP>
+__global__ void __launch_bounds__(512)
+k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restrict__ piv,
+          const T* __restrict__ Vt, const T* __restrict__ W, const T* __restrict__ S,
+          const int* __restrict__ spiv, const int* __restrict__ perm, const T* __restrict__ SpL,
+          const T* __restrict__ SpU, SolveIO io, int n, int r, int KL, int M, int US, int P, int LS, int LSU,
+          int ywbytes, int bufbytes) {
+    extern __shared__ __align__(16) char lds[];
+    constexpr int ES = (int)sizeof(T);
+    constexpr int NB = wave_nbuf(SP);
+    const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+    const int m = blockIdx.x, k = blockIdx.y;
+    const size_t mn = (size_t)m * n;
+    const int WB = M - 1;
+    const int a = part_lo(c, n, P), e = part_lo(c + 1, n, P), Lc = e - a;
+    T* yv = reinterpret_cast<T*>(lds);
+    char* zslot = lds + align16(n * ES);
+    T* Sl = reinterpret_cast<T*>(zslot + 16);
+    int* spl = reinterpret_cast<int*>(Sl + MAXR * MAXR);
+    T* qv = reinterpret_cast<T*>(zslot + WAVE_SMALL);
+    T* part = qv + P * PART_W;
+    char* mine = reinterpret_cast<char*>(part + P * MAXR) + (size_t)c * (ywbytes + NB * bufbytes);
+    T* yw = reinterpret_cast<T*>(mine);
+    char* bufs = mine + ywbytes;
+    const T* SpLc = SpL + ((size_t)m * P + c) * KL * LS;
+    const T* SpUc = SpU + ((size_t)m * P + c) * WB * LSU;
+    if (c == 0) {
+        if (lane < r * r) Sl[lane] = S[(size_t)m * r * r + lane];
+        if (lane < r) spl[lane] = spiv[(size_t)m * r + lane];
+        if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
+    }
+    for (int i = lane; i < PART_W; i += 64) qv[c * PART_W + i] = cn::zero<T>();   // padding of the q slots
+    // the Woodbury rows of the chunk's first 64 rows, read now so that their latency passes
+    // during the sweeps
+    const T* V0 = Vt + (size_t)m * r * n;
+    const T* W0 = W + (size_t)m * r * n;
+    T vpre[MAXR], wpre[MAXR];
+    if (r > 0) {
+        const int j = a + min(lane, Lc - 1);
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) vpre[q] = V0[(size_t)min(q, r - 1) * n + j];
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) wpre[q] = W0[(size_t)min(q, r - 1) * n + j];
+    }
+    for (int j0 = a; j0 < e; j0 += 256) {   // c = P b, this chunk's rows (batched loads)
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = io_load_b<T>(io, mn, k, perm[min(j0 + 64 * u + lane, e - 1)]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + 64 * u + lane < e) yv[j0 + 64 * u + lane] = v[u];
+    }
+    __syncthreads();
+    // forward: the chunk and the KL rows after it, head rows zero (they arrive through q)
+    for (int i = lane; i < Lc + KL; i += 64) {
+        const int row = a + i;
+        yw[i] = ((c > 0 && i < KL) || row >= n) ? cn::zero<T>() : yv[row];
+    }
+    wave_sync();
+    wave_forward<T, (SP < 32 ? SP : 32), SP>(Lf + (mn + a) * KL, piv + mn + a, yw, zslot, bufs, bufbytes, Lc, KL,
+                                           lane, Lc + KL, true);
+    __syncthreads();
+    const bool chain = c >= 1 && c <= P - 2;
+    T tr[PART_W];   // row `lane` of T_c (forward, then backward)
+#pragma unroll
+    for (int i = 0; i < PART_W; ++i)
+        tr[i] = (chain && i < KL && lane < KL) ? SpLc[(size_t)i * LS + Lc + lane] : cn::zero<T>();
+    if (c == 0 && lane < KL) qv[PART_W + lane] = yw[Lc + lane];
+    __syncthreads();
+    for (int st = 1; st <= P - 2; ++st) {   // q_{st+1} = out_st + T_st q_st
+        if (c == st && lane < KL) qv[(st + 1) * PART_W + lane] = part_dot<T>(tr, qv + st * PART_W, KL, yw[Lc + lane]);
+        __syncthreads();
+    }
+    part_fixup<T>(SpLc, LS, 0, KL, qv + c * PART_W, yw, yv + a, Lc, c >= 1, lane);
+    __syncthreads();
+    // backward: the WB rows before the chunk (zero: they leave through q) and the chunk
+    for (int i = lane; i < WB + Lc; i += 64) yw[i] = i < WB ? cn::zero<T>() : yv[a + i - WB];
+    wave_sync();
+    wave_backward_any<T, SP>(Uc + (mn + a) * US, yw + WB, zslot, bufs, bufbytes, Lc, M, US, lane, -WB, true);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PART_W; ++i)
+        tr[i] = (chain && i < WB && lane < WB) ? SpUc[(size_t)i * LSU + lane] : cn::zero<T>();
+    if (c == P - 1 && lane < WB) qv[(P - 2) * PART_W + lane] = yw[lane];
+    __syncthreads();
+    for (int st = P - 2; st >= 1; --st) {   // q_{st-1} = out_st + T_st q_st
+        if (c == st && lane < WB) qv[(st - 1) * PART_W + lane] = part_dot<T>(tr, qv + st * PART_W, WB, yw[lane]);
+        __syncthreads();
+    }
+    part_fixup<T>(SpUc, LSU, WB, WB, qv + c * PART_W, yw + WB, yv + a, Lc, c <= P - 2, lane);
+    wave_sync();
+    if (r == 0) {
+        for (int i = lane; i < Lc; i += 64) io_store<T>(io, mn, k, a + i, yv[a + i]);
+        return;
+    }
+    // Woodbury correction over this chunk's rows: partial V^T y, summed over the chunks in order
+    T t[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = cn::zero<T>();
+    for (int i0 = 0; i0 < Lc; i0 += 64) {
+        const int i = i0 + lane, j = a + min(i, Lc - 1);
+        T vv[MAXR];
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) vv[q] = i0 == 0 ? vpre[q] : V0[(size_t)min(q, r - 1) * n + j];
+        const T yj = i < Lc ? yv[j] : cn::zero<T>();
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q < r) t[q] = cn::fmac(vv[q], yj, t[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) {
+        if (q < r) {
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) t[q] = addv(t[q], xor_shfl(t[q], mask));
+        }
+    }
+    if (lane < r) {
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q)
+            if (q == lane) part[c * MAXR + q] = t[q];
+    }
+    __syncthreads();
+    // lane q < r: t_q summed over the chunks in order (all partials read at once), then z = S^-1 t
+    // across the lanes and broadcast back
+    T tq = cn::zero<T>();
+    if (lane < r) {
+        T pp[8];
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) pp[cc] = cc < P ? part[cc * MAXR + lane] : cn::zero<T>();
+        tq = pp[0];
+#pragma unroll
+        for (int cc = 1; cc < 8; ++cc)
+            if (cc < P) tq = addv(tq, pp[cc]);
+    }
+    tq = cap_solve_lanes<T>(tq, Sl, spl, r, lane);
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) t[q] = q < r ? bcast(tq, q) : cn::zero<T>();
+    for (int i0 = 0; i0 < Lc; i0 += 64) {   // x = y - W z
+        const int i = i0 + lane, j = a + min(i, Lc - 1);
+        T wv[MAXR];
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) wv[q] = i0 == 0 ? wpre[q] : W0[(size_t)min(q, r - 1) * n + j];
+        if (i < Lc) {
+            T v = yv[j];
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q)
+                if (q < r) v = cn::fnms(wv[q], t[q], v);
+            io_store<T>(io, mn, k, j, v);
+        }
     }
 }
 

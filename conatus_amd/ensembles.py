@@ -24,7 +24,7 @@ import numpy as np
 
 from .errors import OptionError
 from .problems import RayleighBenard2D
-from .runtime.banded import BatchHandle, csr_args
+from .runtime.banded import SOLVE_MODES, BatchHandle, csr_args
 from .runtime.imex import Imex, sbdf_coefficients
 from .timestep import Channel3DExplicit, IVPSolver, RB2DExplicit, explicit_term
 
@@ -207,7 +207,7 @@ class Ensemble:
 def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, reps: int = 5,
                   pivot: bool = True) -> tuple[str, dict]:
     """Kernel mode for one size class: the full-order operator a0/dt M + L of every member's modes
-    factored in lane mode and in wave mode, ``reps`` timed solves of each; returns the faster mode
+    factored in lane mode, wave mode and partitioned wave mode, ``reps`` timed solves of each; returns the faster mode
     and the timings (ms per solve).  A mode that does not fit the shape is refused by the library
     and skipped here."""
     torch = _torch()
@@ -219,9 +219,9 @@ def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, 
     times: dict = {}
     rng = np.random.default_rng(0)
     b = torch.from_numpy(rng.standard_normal((len(mats), n)) + 1j * rng.standard_normal((len(mats), n))).cuda()
-    for mode in ("lane", "wave"):
+    for mode in ("lane", "wave", "part"):
         h = BatchHandle(mats, border, np.complex128, pivot, mode=mode)
-        got = "wave" if h.props().solve_mode == 1 else "lane"
+        got = SOLVE_MODES[h.props().solve_mode]
         if got != mode:
             continue
         h.reserve(1)

@@ -12,8 +12,10 @@ sizes the library and the Python layers allocate:
 * CSR operators M and L (values, column indices, row pointers);
 * factor handles, one per implicit operator alpha M + beta L the scheme uses (SBDF-q: q orders of
   the startup; RK: one per distinct H_ii; CN: the CNAB1 startup and the scheme's), each with its
-  band rows, U, L, pivots, border V^T / W, capacitance S, their transposed twins and the solve
-  work space of one right-hand side (csrc/banded.hip, create_impl and reserve_impl);
+  band rows, U, L, pivots, border V^T / W, capacitance S, their transposed twins, the solve
+  work space of one right-hand side and, for a batch the library partitions (up to 256 systems
+  long enough to split), the chunks' coupling spikes (csrc/banded.hip, create_impl and
+  reserve_impl);
 * explicit term: transform tables, the constant part of F and its workspace (RB 2D: 3 K Mz line
   products; channel: cn_ch3d_workspace_bytes).
 
@@ -23,8 +25,8 @@ scratch), as measured on MI355X: a 64 x 65 x 64 channel run under SBDF3 holds 29
 buffers are 2712 MiB, a RB 2D 256 x 128 run under SBDF2 234 MiB for 222 MiB
 (tests/test_gpu_memory.py).  ``PROCESS_FLOOR_BYTES`` is what a process holds before any run (HIP
 context, the code objects of libcnhip and torch, and the library's fixed 64 MiB work-space array
-of the allocation-free solve, which every device holds once the library is loaded): up to 278 MiB
-measured on MI355X, 320 MiB declared (the reference declares 512 MiB for its CUDA stack).
+of the allocation-free solve, which every device holds once the library is loaded): up to 324 MiB
+measured on MI355X, 352 MiB declared (the reference declares 512 MiB for its CUDA stack).
 """
 from __future__ import annotations
 
@@ -36,9 +38,9 @@ __all__ = ["MemoryPlan", "plan", "handle_bytes", "PROCESS_FLOOR_BYTES"]
 
 MiB = 2 ** 20
 # HIP context + code objects of a process that has loaded torch and libcnhip and run one solve,
-# with the library's 64 MiB static work-space array: up to 278 MiB measured on MI355X (ROCm 7,
+# with the library's 64 MiB static work-space array: up to 324 MiB measured on MI355X (ROCm 7,
 # torch 2.10; alone and after other runs), declared rounded up
-PROCESS_FLOOR_BYTES = 320 * MiB
+PROCESS_FLOOR_BYTES = 352 * MiB
 
 _KL_SET = (1, 2, 4, 6, 8, 10, 12, 14, 16)
 _WU_SET = (4, 8, 16, 24, 32)
@@ -51,13 +53,52 @@ def _pick(v: int, options) -> int:
     raise ValueError(f"band width {v} beyond the compiled kernels")
 
 
+_LDS_CAP = 160 * 1024
+_WAVE_SMALL = 16 + 8 * 8 * 16 + 8 * 4 + 16
+
+
+def _a16(b: int) -> int:
+    return (b + 15) // 16 * 16
+
+
+def _wave_buf(KLP: int, US: int, esz: int, sp: int) -> int:
+    return max(_a16(sp * KLP * esz + sp * 4), _a16(sp * US * esz))
+
+
+def _parts(B: int, n: int, KLP: int, M: int, US: int, esz: int) -> tuple[int, int]:
+    """(wavefronts per system, longest chunk) of the automatic kernel mode (csrc/banded.hip:
+    wave mode when the one-wavefront LDS fits and the batch is not a lane-mode one, partitioned
+    for up to 256 systems by pick_parts); (1, 0) when the batch is not partitioned."""
+    lane_fast = KLP + 1 <= 8 and US - 1 <= 16
+    wave_ok = _a16(n * esz) + _WAVE_SMALL + 3 * _wave_buf(KLP, US, esz, 32) <= _LDS_CAP
+    if (lane_fast and B >= 1024) or not wave_ok or B > 256:
+        return 1, 0
+    wmax = max(KLP, M - 1)
+    if wmax > 32:
+        return 1, 0
+    for P in (8, 4):
+        if n // P < wmax:
+            continue
+        plen = (n + P - 1) // P
+        for sp in (16, 8):
+            nb = 2 if sp <= 8 else 3
+            lds = (_a16(n * esz) + _WAVE_SMALL + P * (32 + 8) * esz
+                   + P * (_a16((plen + wmax) * esz) + nb * _wave_buf(KLP, US, esz, sp)))
+            if lds <= _LDS_CAP:
+                return P, plen
+    return 1, 0
+
+
 def handle_bytes(B: int, n: int, kl: int, ku: int, r: int, esz: int = 16, nrhs: int = 1) -> int:
-    """Device bytes of one factor handle of B systems (csrc/banded.hip create_impl: the kernels'
-    padded widths KLP, WU, systems padded to whole 64-system workgroups; reserve_impl: the solve
-    work space of ``nrhs`` right-hand sides)."""
+    """Device bytes of one factor handle of B systems in the automatic kernel mode (csrc/banded.hip
+    create_impl: the kernels' padded widths KLP, WU, systems padded to whole 64-system workgroups,
+    the coupling spikes of a partitioned batch; reserve_impl: the solve work space of ``nrhs``
+    right-hand sides)."""
     KLP = _pick(max(kl, 1), _KL_SET)
     M = KLP + ku + 1
     US = _pick(KLP + ku, _WU_SET) + 1
+    P, plen = _parts(B, n, KLP, M, US, esz)
+    spikes = B * P * (KLP * (plen + KLP) + (M - 1) * (plen + M - 1)) * esz if P > 1 else 0
     Bp = (B + 63) // 64 * 64
     nV = max(Bp * r * n, 1)
     S = max(Bp * r * r, 1)
@@ -65,7 +106,7 @@ def handle_bytes(B: int, n: int, kl: int, ku: int, r: int, esz: int = 16, nrhs: 
     total = (Bp * n * M * esz + Bp * n * US * esz + Bp * n * KLP * esz + Bp * n * 4      # Ab, Uf, Lf, piv
              + 2 * nV * esz + S * esz + sp_ * 4 + Bp * 8 + Bp * 4 + n * 4                 # Vt, W, S, spiv, cond, info, perm
              + nV * esz + S * esz + sp_ * 4 + Bp * 8 + Bp * 4)                            # transposed twins
-    return total + B * nrhs * n * esz
+    return total + spikes + B * nrhs * n * esz
 
 
 def _implicit_operators(scheme: str, dt: float) -> int:

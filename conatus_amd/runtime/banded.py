@@ -29,14 +29,21 @@ _factor_dtype = factor_dtype
 _check_rhs_shape = check_rhs_shape
 
 
+#: kernel modes a handle can be asked for (cn_bb_request_mode) and the solve modes it reports
+#: (cn_bb_props.solve_mode): one lane per (system, rhs); one wavefront per (system, rhs); one
+#: workgroup per (system, rhs) with the sweeps partitioned over its wavefronts
+MODE_REQUEST = {None: 0, "lane": 1, "wave": 2, "part": 3}
+SOLVE_MODES = {0: "lane", 1: "wave", 2: "part"}
+
+
 class BatchHandle:
     """Owner of one ``cn_bb`` handle: a factored batch resident on the current HIP device."""
 
     def __init__(self, csrs: Sequence[tuple], border: Sequence[int], dtype, pivot: bool = True,
                  col_border: bool = False, mode: str | None = None):
         L = _lib.lib()
-        if mode not in (None, "lane", "wave"):
-            raise OptionError(f"BandedBatch: mode must be None, 'lane' or 'wave', got {mode!r}")
+        if mode not in MODE_REQUEST:
+            raise OptionError(f"BandedBatch: mode must be None, 'lane', 'wave' or 'part', got {mode!r}")
         self.dtype = np.dtype(dtype)
         B = len(csrs)
         if B == 0:
@@ -55,7 +62,7 @@ class BatchHandle:
         bord = np.ascontiguousarray(np.asarray([int(i) for i in border], dtype=np.int32))
         h = ctypes.c_void_p()
         code = _lib.CN_C128 if self.dtype == np.complex128 else _lib.CN_F64
-        _lib.check(L.cn_bb_request_mode({None: 0, "lane": 1, "wave": 2}[mode]), "BandedBatch")
+        _lib.check(L.cn_bb_request_mode(MODE_REQUEST[mode]), "BandedBatch")
         try:
             st = L.cn_bb_create(ctypes.byref(h), code, B, n, ip, ix, dv, nnz,
                                 bord.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(bord.size),

@@ -14,7 +14,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from ..errors import BackendError, OptionError
-from .banded import BatchHandle, _factor_dtype, csr_args
+from .banded import SOLVE_MODES, BatchHandle, _factor_dtype, csr_args
 from .checks import check_index_map, check_io_overlap
 
 __all__ = ["available", "device_name", "synchronize", "to_device", "BandedBatch"]
@@ -92,7 +92,7 @@ class BandedBatch:
         self.capacitance_cond_max = float(p.cond_max)
         self.col_border = bool(col_border)
         self.kernel = (p.kl_kernel, p.wu_kernel, p.lanes)
-        self.solve_mode = "wave" if p.solve_mode == 1 else "lane"
+        self.solve_mode = SOLVE_MODES[p.solve_mode]
         self.factor_bytes = int(p.factor_bytes)
 
     level_scheduled = False
@@ -200,4 +200,7 @@ class BandedBatch:
                 f"gfx950: factor {lanes} lanes per system (kernel kl={kl}), "
                 + ("solve one lane per (system, rhs) over 16-system interleaved factors (window "
                    f"{wu}), streamed through LDS" if self.solve_mode == "lane" else
-                   "solve one wavefront per (system, rhs), band window across lanes, U stored by columns"))
+                   "solve one wavefront per (system, rhs), band window across lanes, U stored by columns"
+                   if self.solve_mode == "wave" else
+                   "solve one workgroup per (system, rhs), its sweeps partitioned over 2-8 wavefronts "
+                   "coupled through factor-time spikes, band window across lanes"))

@@ -53,7 +53,8 @@ typedef struct cn_bb_props {
     double cond_max;    /* max over the batch of cond_1 of the capacitance matrix */
     double factor_ms;   /* device time of the last factorization */
     int64_t factor_bytes; /* device bytes held by the factors */
-    int32_t solve_mode;   /* 0: one lane per (system, rhs), streamed sweeps; 1: one wavefront per (system, rhs) */
+    int32_t solve_mode;   /* 0: one lane per (system, rhs), streamed sweeps; 1: one wavefront per (system, rhs);
+                             2: one workgroup per (system, rhs), its sweeps partitioned over 2-8 wavefronts */
 } cn_bb_props;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -140,9 +141,11 @@ void cn_bb_destroy(cn_bb_t h);
 int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, int count, void* stream);
 
 /* Kernel mode of the handles the calling thread creates from now on: 0 automatic (lane mode for
- * many narrow-band systems, wave mode otherwise), 1 lane mode (one lane per system and
- * right-hand side, streamed sweeps), 2 wave mode (one wavefront per system and right-hand side);
- * a mode that does not fit the shape falls back to the other.  For the per-size-class autotuner;
+ * many narrow-band systems, partitioned wave mode for up to 256 systems long enough to split,
+ * wave mode otherwise), 1 lane mode (one lane per system and right-hand side, streamed sweeps),
+ * 2 wave mode (one wavefront per system and right-hand side), 3 partitioned wave mode (the sweeps
+ * of a system split over 2-8 wavefronts, coupled through spikes computed at factor time); a mode
+ * that does not fit the shape falls back to the next one.  For the per-size-class autotuner;
  * no counterpart in the reference (its CUDA batch picks band vs level-scheduled solves by flags,
  * core/include/conatus/cuda.hpp:57-120). */
 int cn_bb_request_mode(int mode);

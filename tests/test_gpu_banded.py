@@ -4,9 +4,10 @@ Bar: normwise backward error <= 1e-13, the residual within
 10x the oracle's own residual plus n eps (the reference's cross-backend rule,
 tests/cross_backend/test_banded_level.py:71-76), solutions within a conditioning-scaled distance of
 the oracle's, fused gather/scatter and split-complex maps exact, bitwise determinism.  Every test
-runs in both solve modes of the library (CNHIP_MODE=lane: one lane per system and right-hand side
+runs in every solve mode of the library (CNHIP_MODE=lane: one lane per system and right-hand side
 over 16-system interleaved factors; CNHIP_MODE=wave: one wavefront per system and right-hand side,
-U stored by columns)."""
+U stored by columns; CNHIP_MODE=part: wave mode with each system's sweeps partitioned over 2-8
+wavefronts)."""
 import os
 
 import numpy as np
@@ -22,9 +23,11 @@ pytestmark = pytest.mark.gpu
 BE_TOL = 1e-13
 
 
-@pytest.fixture(autouse=True, params=["lane", "lane8", "wave"])
+@pytest.fixture(autouse=True, params=["lane", "lane8", "wave", "part"])
 def solve_mode(request, monkeypatch):
-    """lane: streamed sweeps in blocks of 16 steps where they fit in LDS; lane8: blocks of 8."""
+    """lane: streamed sweeps in blocks of 16 steps where they fit in LDS; lane8: blocks of 8;
+    part: wave mode with each system's sweeps partitioned over 2-8 wavefronts where the system is
+    long enough (else wave mode)."""
     mode = request.param.rstrip("8")
     monkeypatch.setenv("CNHIP_MODE", mode)
     if request.param == "lane8":
@@ -110,7 +113,7 @@ def test_band_widths_cover_every_kernel(kl, ku, dtype, rng, solve_mode):
         A[np.arange(r, n), np.arange(r, n)] += 0.5          # pivots still swap rows
         mats.append(sp.csr_matrix(A))
     bb = BandedBatch(mats, list(range(r)))
-    assert bb.solve_mode == solve_mode
+    assert bb.solve_mode == solve_mode or (solve_mode == "part" and bb.solve_mode == "wave")
     b = _rhs(rng, (B, n), dtype)
     d = to_device(b)
     bb.solve(d)
@@ -355,17 +358,18 @@ def test_core_shim_signatures(rng):
 
 
 def test_auto_mode_selection(rng, monkeypatch, solve_mode):
-    """Without CNHIP_MODE: wave mode for a few systems, lane mode for >= 1024 narrow-band ones."""
+    """Without CNHIP_MODE: partitioned wave mode for a few systems long enough to split, lane mode
+    for >= 1024 narrow-band ones."""
     if solve_mode != "lane" or os.environ.get("CNHIP_SPX"):
         pytest.skip("mode selection is checked once")
     monkeypatch.delenv("CNHIP_MODE")
     from conatus_amd.runtime.hip import BandedBatch
     few = BandedBatch([conditioned(rng, 64, 2, 3, 2, dtype=complex) for _ in range(3)], [0, 1])
-    assert few.solve_mode == "wave"
+    assert few.solve_mode == "part"
     A = conditioned(rng, 32, 2, 2, 2, dtype=complex)
     many = BandedBatch([A] * 1024, [0, 1])
     assert many.solve_mode == "lane"
-    assert "lane per (system, rhs)" in many.explain() and "wavefront per (system, rhs)" in few.explain()
+    assert "lane per (system, rhs)" in many.explain() and "partitioned over" in few.explain()
 
 
 @pytest.mark.parametrize("kl,ku,r", [(12, 18, 7), (4, 4, 2), (16, 16, 3)])

@@ -1799,6 +1799,25 @@ __device__ __forceinline__ T cap_solve_lanes(T ti, const T* Sm, const int* sp, i
     return ti;
 }
 
+// One step of a coupling chain, its w outputs over lane pairs: lane i and lane i + 32 (i < w) hold
+// the terms [0, 16) and [16, 32) of output i (m: that half of row i of T_c, zero beyond w; q: the
+// LDS slot of the incoming coupling values, zero beyond w); lane i returns base_i + (its half) +
+// (the other half).
+constexpr int PART_H = PART_W / 2;
+template <class T>
+__device__ __forceinline__ T part_chain_step(const T (&m)[PART_H], const T* q, T base, int lane) {
+    const int half = lane >> 5;
+    T ps[4] = {half ? cn::zero<T>() : base, cn::zero<T>(), cn::zero<T>(), cn::zero<T>()};
+    T qq[PART_H];
+#pragma unroll
+    for (int j = 0; j < PART_H; ++j) qq[j] = q[half * PART_H + j];
+#pragma unroll
+    for (int j = 0; j < PART_H; ++j) ps[j & 3] = cn::fmac(m[j], qq[j], ps[j & 3]);
+    const T mine = addv(addv(ps[0], ps[1]), addv(ps[2], ps[3]));
+    const T other = xor_shfl(mine, 32);
+    return half ? addv(other, mine) : addv(mine, other);
+}
+
 // rows [0, Lc) of a chunk: v_i += sum_u Sp[u][off + i] q[u] (u < wdt), written to dst[i]
 template <class T>
 __device__ __forceinline__ void part_fixup(const T* __restrict__ Sp, size_t ustride, int off, int wdt, const T* q,
@@ -1851,18 +1870,8 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
         if (lane == 0) *reinterpret_cast<zc*>(zslot) = cn::zero<zc>();
     }
     for (int i = lane; i < PART_W; i += 64) qv[c * PART_W + i] = cn::zero<T>();   // padding of the q slots
-    // the Woodbury rows of the chunk's first 64 rows, read now so that their latency passes
-    // during the sweeps
     const T* V0 = Vt + (size_t)m * r * n;
     const T* W0 = W + (size_t)m * r * n;
-    T vpre[MAXR], wpre[MAXR];
-    if (r > 0) {
-        const int j = a + min(lane, Lc - 1);
-#pragma unroll
-        for (int q = 0; q < MAXR; ++q) vpre[q] = V0[(size_t)min(q, r - 1) * n + j];
-#pragma unroll
-        for (int q = 0; q < MAXR; ++q) wpre[q] = W0[(size_t)min(q, r - 1) * n + j];
-    }
     for (int j0 = a; j0 < e; j0 += 256) {   // c = P b, this chunk's rows (batched loads)
         T v[4];
 #pragma unroll
@@ -1882,14 +1891,20 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
                                            lane, Lc + KL, true);
     __syncthreads();
     const bool chain = c >= 1 && c <= P - 2;
-    T tr[PART_W];   // row `lane` of T_c (forward, then backward)
+    const int crow = lane & 31, chalf = lane >> 5;   // chain output and half of its terms
+    T tr[PART_H];   // half `chalf` of row `crow` of T_c (forward, then backward)
 #pragma unroll
-    for (int i = 0; i < PART_W; ++i)
-        tr[i] = (chain && i < KL && lane < KL) ? SpLc[(size_t)i * LS + Lc + lane] : cn::zero<T>();
+    for (int i = 0; i < PART_H; ++i) {
+        const int u = chalf * PART_H + i;
+        tr[i] = (chain && u < KL && crow < KL) ? SpLc[(size_t)u * LS + Lc + crow] : cn::zero<T>();
+    }
     if (c == 0 && lane < KL) qv[PART_W + lane] = yw[Lc + lane];
     __syncthreads();
     for (int st = 1; st <= P - 2; ++st) {   // q_{st+1} = out_st + T_st q_st
-        if (c == st && lane < KL) qv[(st + 1) * PART_W + lane] = part_dot<T>(tr, qv + st * PART_W, KL, yw[Lc + lane]);
+        if (c == st) {
+            const T v = part_chain_step<T>(tr, qv + st * PART_W, crow < KL ? yw[Lc + crow] : cn::zero<T>(), lane);
+            if (lane < KL) qv[(st + 1) * PART_W + lane] = v;
+        }
         __syncthreads();
     }
     part_fixup<T>(SpLc, LS, 0, KL, qv + c * PART_W, yw, yv + a, Lc, c >= 1, lane);
@@ -1900,12 +1915,17 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
     wave_backward_any<T, SP>(Uc + (mn + a) * US, yw + WB, zslot, bufs, bufbytes, Lc, M, US, lane, -WB, true);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < PART_W; ++i)
-        tr[i] = (chain && i < WB && lane < WB) ? SpUc[(size_t)i * LSU + lane] : cn::zero<T>();
+    for (int i = 0; i < PART_H; ++i) {
+        const int u = chalf * PART_H + i;
+        tr[i] = (chain && u < WB && crow < WB) ? SpUc[(size_t)u * LSU + crow] : cn::zero<T>();
+    }
     if (c == P - 1 && lane < WB) qv[(P - 2) * PART_W + lane] = yw[lane];
     __syncthreads();
     for (int st = P - 2; st >= 1; --st) {   // q_{st-1} = out_st + T_st q_st
-        if (c == st && lane < WB) qv[(st - 1) * PART_W + lane] = part_dot<T>(tr, qv + st * PART_W, WB, yw[lane]);
+        if (c == st) {
+            const T v = part_chain_step<T>(tr, qv + st * PART_W, crow < WB ? yw[crow] : cn::zero<T>(), lane);
+            if (lane < WB) qv[(st - 1) * PART_W + lane] = v;
+        }
         __syncthreads();
     }
     part_fixup<T>(SpUc, LSU, WB, WB, qv + c * PART_W, yw + WB, yv + a, Lc, c <= P - 2, lane);
@@ -1922,7 +1942,7 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
         const int i = i0 + lane, j = a + min(i, Lc - 1);
         T vv[MAXR];
 #pragma unroll
-        for (int q = 0; q < MAXR; ++q) vv[q] = i0 == 0 ? vpre[q] : V0[(size_t)min(q, r - 1) * n + j];
+        for (int q = 0; q < MAXR; ++q) vv[q] = V0[(size_t)min(q, r - 1) * n + j];
         const T yj = i < Lc ? yv[j] : cn::zero<T>();
 #pragma unroll
         for (int q = 0; q < MAXR; ++q)
@@ -1960,7 +1980,7 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
         const int i = i0 + lane, j = a + min(i, Lc - 1);
         T wv[MAXR];
 #pragma unroll
-        for (int q = 0; q < MAXR; ++q) wv[q] = i0 == 0 ? wpre[q] : W0[(size_t)min(q, r - 1) * n + j];
+        for (int q = 0; q < MAXR; ++q) wv[q] = W0[(size_t)min(q, r - 1) * n + j];
         if (i < Lc) {
             T v = yv[j];
 #pragma unroll

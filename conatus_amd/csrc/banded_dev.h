@@ -1318,6 +1318,29 @@ __device__ __forceinline__ zc xor_shfl(zc v, int mask) { return zc{xor_shfl(v.re
 __device__ __forceinline__ double addv(double a, double b) { return a + b; }
 __device__ __forceinline__ zc addv(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
 
+// DPP move of a double (both halves); lanes the control does not write (row_mask) receive 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// Sum over the 64 lanes, in every lane: pairs, quads (quad_perm), half rows (row_half_mirror),
+// rows of 16 (row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast:15), rows 0-1
+// into rows 2-3 (row_bcast:31), read from lane 63.  Moves within the VALU, no LDS round trips.
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_d<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141, 0xf>(v);   // row_half_mirror
+    v += dpp_d<0x140, 0xf>(v);   // row_mirror
+    v += dpp_d<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_d<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ zc wave_total(zc v) { return zc{wave_total(v.re), wave_total(v.im)}; }
+
 // contiguous copy of nbytes (multiple of 4) into LDS, 256 bytes per instruction
 __device__ __forceinline__ void dma4(const void* src_, char* lds_dst, int nbytes, int lane) {
     const char* src = static_cast<const char*>(src_);
@@ -1591,12 +1614,8 @@ __device__ __forceinline__ void wave_vty(const T* __restrict__ V0, const T* yv, 
         }
     }
 #pragma unroll
-    for (int a = 0; a < MAXR; ++a) {
-        if (a < r) {
-#pragma unroll
-            for (int mask = 32; mask >= 1; mask >>= 1) t[a] = addv(t[a], xor_shfl(t[a], mask));
-        }
-    }
+    for (int a = 0; a < MAXR; ++a)
+        if (a < r) t[a] = wave_total(t[a]);
 }
 
 // Solve in wave mode: grid (B, nrhs), one wavefront each.  LDS: [yv: n entries][zero slot, S,
@@ -1949,12 +1968,8 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
             if (q < r) t[q] = cn::fmac(vv[q], yj, t[q]);
     }
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q) {
-        if (q < r) {
-#pragma unroll
-            for (int mask = 32; mask >= 1; mask >>= 1) t[q] = addv(t[q], xor_shfl(t[q], mask));
-        }
-    }
+    for (int q = 0; q < MAXR; ++q)
+        if (q < r) t[q] = wave_total(t[q]);
     if (lane < r) {
 #pragma unroll
         for (int q = 0; q < MAXR; ++q)

@@ -1970,6 +1970,14 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
 #pragma unroll
     for (int q = 0; q < MAXR; ++q)
         if (q < r) t[q] = wave_total(t[q]);
+    // W rows of the chunk's first 64 rows, loaded now: their latency passes during the
+    // capacitance solve
+    T wpre[MAXR];
+    {
+        const int j = a + min(lane, Lc - 1);
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) wpre[q] = W0[(size_t)min(q, r - 1) * n + j];
+    }
     if (lane < r) {
 #pragma unroll
         for (int q = 0; q < MAXR; ++q)
@@ -1995,7 +2003,7 @@ k_solve_p(const T* __restrict__ Uc, const T* __restrict__ Lf, const int* __restr
         const int i = i0 + lane, j = a + min(i, Lc - 1);
         T wv[MAXR];
 #pragma unroll
-        for (int q = 0; q < MAXR; ++q) wv[q] = W0[(size_t)min(q, r - 1) * n + j];
+        for (int q = 0; q < MAXR; ++q) wv[q] = i0 == 0 ? wpre[q] : W0[(size_t)min(q, r - 1) * n + j];
         if (i < Lc) {
             T v = yv[j];
 #pragma unroll

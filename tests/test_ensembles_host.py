@@ -56,13 +56,22 @@ def test_channel_keys_parameters_and_footprint():
 def test_memory_plan_handle_layout():
     """A factor handle's bytes by hand for B = 100 systems, n = 50, kl = 3 (kernel width 4),
     ku = 5 (kl + ku = 9 padded to the kernel width 16, U rows of 17), r = 2, complex: systems
-    padded to 128."""
+    padded to 128; 100 systems of 50 rows are a partitioned batch (four chunks of at most 13
+    rows, coupling widths 4 forward and 9 backward), whose spikes count too; 300 systems are
+    not."""
     from conatus_amd.memory import handle_bytes, plan
     from conatus_amd.problems import RayleighBenard2D
-    Bp, n, es = 128, 50, 16
-    band = Bp * n * (4 + 5 + 1) * es + Bp * n * 17 * es + Bp * n * 4 * es + Bp * n * 4
-    border = 3 * Bp * 2 * n * es + 2 * (Bp * 4 * es + Bp * 2 * 4 + Bp * 8 + Bp * 4) + n * 4
-    assert handle_bytes(100, n, 3, 5, 2) == band + border + 100 * n * es
+    n, es = 50, 16
+
+    def fixed(B):
+        Bp = (B + 63) // 64 * 64
+        band = Bp * n * (4 + 5 + 1) * es + Bp * n * 17 * es + Bp * n * 4 * es + Bp * n * 4
+        border = 3 * Bp * 2 * n * es + 2 * (Bp * 4 * es + Bp * 2 * 4 + Bp * 8 + Bp * 4) + n * 4
+        return band + border + B * n * es
+
+    spikes = 100 * 4 * (4 * (13 + 4) + 9 * (13 + 9)) * es
+    assert handle_bytes(100, n, 3, 5, 2) == fixed(100) + spikes
+    assert handle_bytes(300, n, 3, 5, 2) == fixed(300)
     pb = RayleighBenard2D(Nx=64, Nz=32, Lx=4.0, Ra=1e5, Pr=1.0)
     p2, p3 = plan(pb, "SBDF2"), plan(pb, "SBDF3")
     assert p3.lines["factors"] == 3 * p2.lines["factors"] // 2   # one handle per startup order

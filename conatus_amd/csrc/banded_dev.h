@@ -1407,6 +1407,8 @@ __device__ __forceinline__ zc rol1(zc v) { return zc{rol1(v.re), rol1(v.im)}; }
 // lanes 64-SB .. 63 hold its finished rows: they are stored, and the lanes above the window are
 // reloaded with the next rows of c.  The multipliers of a block of SP steps are read from LDS
 // into registers before the block.  A partial last block runs step by step (SB = 0: every block).
+// A chunk of a partitioned sweep passes nrow = n + KL (the KL rows after its n steps are in yv)
+// and keep = true: those rows, still in the window at the end, are stored back.
 template <class T>
 __device__ __forceinline__ void fwd_step(T& y, const T& l, int p, int lane) {
     const T v0 = bcast(y, 0), vp = bcast(y, p);
@@ -1486,7 +1488,9 @@ __device__ __forceinline__ void wave_forward(const T* __restrict__ L0, const int
 // (y_j moves to lane 63).  Groups, flushes and the partial last block as in the forward sweep;
 // the division x_j = y_j / U(j, j) is applied to the finished rows when they are stored, from the
 // column's entry 0 (1 / U(j, j)), read from the staged block when the row is in it (always for the
-// groups of full blocks) and from the factors otherwise.
+// groups of full blocks) and from the factors otherwise.  A chunk of a partitioned sweep passes
+// rlo = -(w - 1) (the rows before its first row are in yv at negative indices) and keep = true:
+// those rows, still in the window at the end, are stored back.
 template <class T>
 __device__ __forceinline__ void bwd_step(T& y, const T& u, int lane) {
     const T yj = bcast(y, 0);

@@ -285,6 +285,7 @@ struct cn_bb {
     int mode = 0;       // 0: lane mode (one lane per system and rhs), 1: wave mode (one wavefront each)
     int il = 16;        // interleave of the factor arrays: 16 systems (lane mode) or 1 (wave mode)
     int parts = 1;      // wave mode: wavefronts per system of the solve (>= 2: partitioned sweeps, spikes)
+    int wave_sp = 0;    // wave mode: staged block length asked for (0: by the launch size, wave_steps)
     int plen = 0, ls = 0, lsu = 0;   // longest chunk; row strides of the forward / backward spikes
     int nrhs_cap = 0;
     size_t esz = 16;
@@ -437,7 +438,7 @@ void run_factor(cn_bb& h, hipStream_t s) {
     if (h.r == 0) return;
     // W = U^-1 (L^-1 P_B U) in place and S = I + V^T W, then the LU of S
     if (h.mode == 1) {
-        const int sp = wave_steps((size_t)h.B * h.r);
+        const int sp = h.wave_sp ? h.wave_sp : wave_steps((size_t)h.B * h.r);
         const size_t lds = wave_lds_bytes(h, sp);
         if (h.dtype == CN_C128)
             hipLaunchKernelGGL((sp == SPW_MANY ? k_border_w<zc, SPW_MANY> : k_border_w<zc, SPW_FEW>), dim3(h.B, h.r),
@@ -623,11 +624,18 @@ void create_impl(cn_bb& h, const int32_t* const* indptr, const int32_t* const* i
             (size_t)align16(n * e) + WAVE_SMALL + 3 * (size_t)wave_buf_bytes(h.KLP, h.US, e, SPW_FEW) <=
                 (size_t)LDS_CAP;
         const char* em = std::getenv("CNHIP_MODE");
-        const std::string want = g_mode_request == 1   ? "lane"
-                                 : g_mode_request == 2 ? "wave"
-                                 : g_mode_request == 3 ? "part"
-                                 : em                  ? em
-                                                       : "";
+        std::string want = g_mode_request == 1   ? "lane"
+                           : g_mode_request == 2 ? "wave"
+                           : g_mode_request == 3 ? "part"
+                           : g_mode_request == 4 ? "wave8"
+                           : g_mode_request == 5 ? "wave32"
+                           : em                  ? em
+                                                 : "";
+        // wave8 / wave32: wave mode with that staged block length (results do not depend on it)
+        if (want == "wave8" || want == "wave32") {
+            h.wave_sp = want == "wave8" ? SPW_MANY : SPW_FEW;
+            want = "wave";
+        }
         if (want == "wave" || want == "part") h.mode = wave_ok ? 1 : 0;
         else if (want == "lane") h.mode = 0;
         else h.mode = (lane_fast && B >= 1024) || !wave_ok ? 0 : 1;
@@ -788,7 +796,7 @@ void solve_impl(cn_bb& h, SolveIO io, int nrhs, hipStream_t s, void* ws = nullpt
         return;
     }
     if (h.mode == 1) {
-        const int sp = wave_steps((size_t)h.B * nrhs);
+        const int sp = h.wave_sp ? h.wave_sp : wave_steps((size_t)h.B * nrhs);
         const size_t lds = wave_lds_bytes(h, sp);
         if (h.dtype == CN_C128)
             hipLaunchKernelGGL((sp == SPW_MANY ? k_solve_w<zc, SPW_MANY> : k_solve_w<zc, SPW_FEW>), dim3(h.B, nrhs),
@@ -1101,8 +1109,9 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
 }
 
 int cn_bb_request_mode(int mode) {
-    if (mode < 0 || mode > 3) {
-        g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane), 2 (wave) or 3 (partitioned wave)";
+    if (mode < 0 || mode > 5) {
+        g_err = "cn_bb_request_mode: mode must be 0 (automatic), 1 (lane), 2 (wave), 3 (partitioned wave), "
+                "4 (wave, blocks of 8 steps) or 5 (wave, blocks of 32 steps)";
         return CN_EINVAL;
     }
     g_mode_request = mode;

@@ -207,7 +207,8 @@ class Ensemble:
 def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, reps: int = 5,
                   pivot: bool = True) -> tuple[str, dict]:
     """Kernel mode for one size class: the full-order operator a0/dt M + L of every member's modes
-    factored in lane mode, wave mode and partitioned wave mode, ``reps`` timed solves of each; returns the faster mode
+    factored in lane mode, wave mode with staged blocks of 8 and of 32 steps, and partitioned wave
+    mode, ``reps`` timed solves of each; returns the fastest mode
     and the timings (ms per solve).  A mode that does not fit the shape is refused by the library
     and skipped here."""
     torch = _torch()
@@ -219,10 +220,10 @@ def autotune_mode(problems: Sequence[RayleighBenard2D], scheme: str, dt: float, 
     times: dict = {}
     rng = np.random.default_rng(0)
     b = torch.from_numpy(rng.standard_normal((len(mats), n)) + 1j * rng.standard_normal((len(mats), n))).cuda()
-    for mode in ("lane", "wave", "part"):
+    for mode in ("lane", "wave8", "wave32", "part"):
         h = BatchHandle(mats, border, np.complex128, pivot, mode=mode)
         got = SOLVE_MODES[h.props().solve_mode]
-        if got != mode:
+        if got != mode.rstrip("0123456789"):
             continue
         h.reserve(1)
         d = b.clone()

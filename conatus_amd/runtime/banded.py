@@ -30,9 +30,10 @@ _check_rhs_shape = check_rhs_shape
 
 
 #: kernel modes a handle can be asked for (cn_bb_request_mode) and the solve modes it reports
-#: (cn_bb_props.solve_mode): one lane per (system, rhs); one wavefront per (system, rhs); one
-#: workgroup per (system, rhs) with the sweeps partitioned over its wavefronts
-MODE_REQUEST = {None: 0, "lane": 1, "wave": 2, "part": 3}
+#: (cn_bb_props.solve_mode): one lane per (system, rhs); one wavefront per (system, rhs) (wave8,
+#: wave32: with staged blocks of 8 or 32 steps, else by the launch size); one workgroup per
+#: (system, rhs) with the sweeps partitioned over its wavefronts
+MODE_REQUEST = {None: 0, "lane": 1, "wave": 2, "part": 3, "wave8": 4, "wave32": 5}
 SOLVE_MODES = {0: "lane", 1: "wave", 2: "part"}
 
 
@@ -43,7 +44,8 @@ class BatchHandle:
                  col_border: bool = False, mode: str | None = None):
         L = _lib.lib()
         if mode not in MODE_REQUEST:
-            raise OptionError(f"BandedBatch: mode must be None, 'lane', 'wave' or 'part', got {mode!r}")
+            raise OptionError(f"BandedBatch: mode must be None or one of {sorted(k for k in MODE_REQUEST if k)}, "
+                              f"got {mode!r}")
         self.dtype = np.dtype(dtype)
         B = len(csrs)
         if B == 0:

@@ -144,8 +144,10 @@ int cn_bb_copy_systems(cn_bb_t dst, int dst_first, cn_bb_t src, int src_first, i
  * many narrow-band systems, partitioned wave mode for up to 256 systems long enough to split,
  * wave mode otherwise), 1 lane mode (one lane per system and right-hand side, streamed sweeps),
  * 2 wave mode (one wavefront per system and right-hand side), 3 partitioned wave mode (the sweeps
- * of a system split over 2-8 wavefronts, coupled through spikes computed at factor time); a mode
- * that does not fit the shape falls back to the next one.  For the per-size-class autotuner;
+ * of a system split over 2-8 wavefronts, coupled through spikes computed at factor time), 4 and 5
+ * wave mode with staged blocks of 8 and of 32 steps (else chosen by the launch size; the results
+ * do not depend on it); a mode that does not fit the shape falls back to the next one.  For the
+ * per-size-class autotuner;
  * no counterpart in the reference (its CUDA batch picks band vs level-scheduled solves by flags,
  * core/include/conatus/cuda.hpp:57-120). */
 int cn_bb_request_mode(int mode);

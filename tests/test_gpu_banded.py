@@ -372,6 +372,27 @@ def test_auto_mode_selection(rng, monkeypatch, solve_mode):
     assert "lane per (system, rhs)" in many.explain() and "partitioned over" in few.explain()
 
 
+def test_wave_block_length_requests(rng, solve_mode):
+    """wave8 / wave32 (cn_bb_request_mode 4 / 5, the autotuner's candidates): wave mode with staged
+    blocks of 8 or of 32 steps; the block length only decides when rows are staged, so the
+    solutions are bit for bit the same, and within the oracle bar."""
+    if solve_mode != "wave":
+        pytest.skip("checked once")
+    from conatus_amd.runtime.hip import BandedBatch, to_device
+    n, r = 150, 7
+    mats = [conditioned(rng, n, 12, 18, r, shift=38.0, dtype=complex) for _ in range(6)]
+    b = _rhs(rng, (6, n), np.complex128)
+    out = []
+    for mode in ("wave8", "wave32"):
+        bb = BandedBatch(mats, list(range(r)), mode=mode)
+        assert bb.solve_mode == "wave"
+        d = to_device(b)
+        bb.solve(d)
+        out.append(d.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+    _check_against_oracle(mats, list(range(r)), b, out[0], np.complex128)
+
+
 @pytest.mark.parametrize("kl,ku,r", [(12, 18, 7), (4, 4, 2), (16, 16, 3)])
 def test_many_wide_band_systems(kl, ku, r, rng, solve_mode):
     """Enough systems that the wave-mode sweeps take their short staged blocks (more than 512

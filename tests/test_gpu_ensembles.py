@@ -80,7 +80,7 @@ def test_uneven_set_in_one_graph_equals_single_runs():
     print(es.explain(), flush=True)
     assert len(es.groups) == 3
     for tu in es.tuning:
-        assert tu["mode"] in ("lane", "wave") and tu["ms_per_solve"]
+        assert tu["mode"] in ("lane", "wave8", "wave32", "part") and tu["ms_per_solve"]
     es.set_states(X0)
     es.run_graph(10)
     es.run_graph(8)                 # continuation of the same trajectories
